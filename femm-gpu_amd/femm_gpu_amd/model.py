@@ -1,0 +1,259 @@
+"""Model container and its C view (the femm_model_t contract of include/femm_model.h).
+
+A :class:`Model` holds the fields of the reference's ``NonlinearModel``
+(gpu_solver/src/physics/nonlinear_model.h:74-97) as numpy arrays in SI units.
+``Model.c_struct()`` returns a ``femm_model_t`` whose pointers stay valid while
+the Model object is alive.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+MU0 = 4.0e-7 * math.pi
+PLANAR = 0
+AXISYMMETRIC = 1
+
+STATUS_NAMES = [
+    "OK", "INPUT_IO", "INVALID_ARGUMENT", "UNSUPPORTED_FEATURE", "MESH_INVALID",
+    "BOUNDARY_INVALID", "GPU_UNAVAILABLE", "GPU_ALLOCATION_FAILED", "ASSEMBLY_FAILED",
+    "LINEAR_SOLVE_NOT_CONVERGED", "LINEAR_SOLVE_BREAKDOWN", "NUMERICAL_NONFINITE",
+    "OUTPUT_IO", "INTERNAL_ERROR", "NONLINEAR_SOLVE_NOT_CONVERGED", "INVALID_MATERIAL",
+]
+
+
+def status_name(code: int) -> str:
+    """Stable status names of gpu_solver/src/core/types.cpp:6-43."""
+    return STATUS_NAMES[code] if 0 <= code < len(STATUS_NAMES) else "UNKNOWN"
+
+
+class CMaterial(ctypes.Structure):
+    _fields_ = [
+        ("reluctivity_zero", ctypes.c_double),
+        ("alpha", ctypes.c_double),
+        ("source_j_per_a", ctypes.c_double),
+        ("hc", ctypes.c_double),
+        ("magnetization_deg", ctypes.c_double),
+        ("bh_curve", ctypes.c_int32),
+        ("circuit", ctypes.c_int32),
+        ("group", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class CConstraint(ctypes.Structure):
+    _fields_ = [("node_a", ctypes.c_int32), ("node_b", ctypes.c_int32), ("sign", ctypes.c_int32)]
+
+
+class CModel(ctypes.Structure):
+    _fields_ = [
+        ("problem_type", ctypes.c_int32),
+        ("node_count", ctypes.c_int32),
+        ("node_xy", ctypes.POINTER(ctypes.c_double)),
+        ("triangle_count", ctypes.c_int32),
+        ("triangle_nodes", ctypes.POINTER(ctypes.c_int32)),
+        ("triangle_material", ctypes.POINTER(ctypes.c_int32)),
+        ("material_count", ctypes.c_int32),
+        ("materials", ctypes.POINTER(CMaterial)),
+        ("curve_count", ctypes.c_int32),
+        ("curve_offsets", ctypes.POINTER(ctypes.c_int32)),
+        ("curve_b", ctypes.POINTER(ctypes.c_double)),
+        ("curve_h", ctypes.POINTER(ctypes.c_double)),
+        ("dirichlet_count", ctypes.c_int32),
+        ("dirichlet_nodes", ctypes.POINTER(ctypes.c_int32)),
+        ("dirichlet_values", ctypes.POINTER(ctypes.c_double)),
+        ("constraint_count", ctypes.c_int32),
+        ("constraints", ctypes.POINTER(CConstraint)),
+        ("circuit_count", ctypes.c_int32),
+        ("depth_m", ctypes.c_double),
+    ]
+
+
+class CNewtonOptions(ctypes.Structure):
+    _fields_ = [
+        ("relative_tolerance", ctypes.c_double),
+        ("linear_relative_tolerance", ctypes.c_double),
+        ("max_newton_iterations", ctypes.c_int32),
+        ("max_linear_iterations", ctypes.c_int32),
+        ("recover_nonconvergence", ctypes.c_int32),
+        ("adaptive_linear_tolerance", ctypes.c_int32),
+    ]
+
+
+def production_options(**overrides) -> CNewtonOptions:
+    """ProductionSampleOptions (gpu_solver/src/physics/nonlinear_model.cpp:195-205)."""
+    values = dict(relative_tolerance=1e-8, linear_relative_tolerance=1e-10,
+                  max_newton_iterations=128, max_linear_iterations=16384,
+                  recover_nonconvergence=1, adaptive_linear_tolerance=1)
+    values.update(overrides)
+    return CNewtonOptions(**values)
+
+
+def plain_options(**overrides) -> CNewtonOptions:
+    """Default NonlinearOptions (nonlinear_model.h:99-106) with a chosen tolerance."""
+    values = dict(relative_tolerance=1e-12, linear_relative_tolerance=1e-13,
+                  max_newton_iterations=32, max_linear_iterations=128,
+                  recover_nonconvergence=0, adaptive_linear_tolerance=0)
+    values.update(overrides)
+    return CNewtonOptions(**values)
+
+
+def _ptr(array: np.ndarray, ctype):
+    return array.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+@dataclass
+class Material:
+    reluctivity_zero: float
+    alpha: float = 0.0
+    source_j_per_a: float = 0.0
+    hc: float = 0.0
+    magnetization_deg: float = 0.0
+    bh_curve: int = -1
+    circuit: int = -1
+    group: int = -1
+
+
+@dataclass
+class Model:
+    node_xy: np.ndarray                      # [N, 2] float64, metres
+    triangles: np.ndarray                    # [T, 3] int32, CCW
+    triangle_material: np.ndarray            # [T] int32
+    materials: list                          # list[Material]
+    curves: list = field(default_factory=list)   # list[(B array, H array)] raw knots
+    dirichlet_nodes: np.ndarray = None
+    dirichlet_values: np.ndarray = None
+    constraints: np.ndarray = None           # [K, 3] int32 (node_a, node_b, sign)
+    circuit_count: int = 1
+    depth_m: float = 1.0
+    problem_type: int = PLANAR
+    name: str = "model"
+
+    def __post_init__(self):
+        self.node_xy = np.ascontiguousarray(self.node_xy, dtype=np.float64).reshape(-1, 2)
+        self.triangles = np.ascontiguousarray(self.triangles, dtype=np.int32).reshape(-1, 3)
+        self.triangle_material = np.ascontiguousarray(self.triangle_material, dtype=np.int32)
+        if self.dirichlet_nodes is None:
+            self.dirichlet_nodes = np.zeros(0, np.int32)
+        self.dirichlet_nodes = np.ascontiguousarray(self.dirichlet_nodes, dtype=np.int32)
+        if self.dirichlet_values is None:
+            self.dirichlet_values = np.zeros(len(self.dirichlet_nodes))
+        self.dirichlet_values = np.ascontiguousarray(self.dirichlet_values, dtype=np.float64)
+        if self.constraints is None:
+            self.constraints = np.zeros((0, 3), np.int32)
+        self.constraints = np.ascontiguousarray(self.constraints, dtype=np.int32).reshape(-1, 3)
+        self._c = None
+
+    @property
+    def node_count(self) -> int:
+        return int(self.node_xy.shape[0])
+
+    @property
+    def triangle_count(self) -> int:
+        return int(self.triangles.shape[0])
+
+    def c_struct(self) -> CModel:
+        """Build (once) and return the femm_model_t view of this model."""
+        if self._c is not None:
+            return self._c
+        mats = (CMaterial * max(1, len(self.materials)))()
+        for i, m in enumerate(self.materials):
+            mats[i] = CMaterial(m.reluctivity_zero, m.alpha, m.source_j_per_a, m.hc,
+                                m.magnetization_deg, m.bh_curve, m.circuit, m.group, 0)
+        offsets = [0]
+        bs, hs = [], []
+        for b, h in self.curves:
+            bs.append(np.asarray(b, np.float64))
+            hs.append(np.asarray(h, np.float64))
+            offsets.append(offsets[-1] + len(b))
+        self._curve_offsets = np.asarray(offsets, np.int32)
+        self._curve_b = np.ascontiguousarray(np.concatenate(bs) if bs else np.zeros(1))
+        self._curve_h = np.ascontiguousarray(np.concatenate(hs) if hs else np.zeros(1))
+        cons = (CConstraint * max(1, len(self.constraints)))()
+        for i, (a, b, s) in enumerate(self.constraints):
+            cons[i] = CConstraint(int(a), int(b), int(s))
+        self._mats, self._cons = mats, cons
+        self._c = CModel(
+            self.problem_type, self.node_count, _ptr(self.node_xy, ctypes.c_double),
+            self.triangle_count, _ptr(self.triangles, ctypes.c_int32),
+            _ptr(self.triangle_material, ctypes.c_int32), len(self.materials), mats,
+            len(self.curves), _ptr(self._curve_offsets, ctypes.c_int32),
+            _ptr(self._curve_b, ctypes.c_double), _ptr(self._curve_h, ctypes.c_double),
+            len(self.dirichlet_nodes), _ptr(self.dirichlet_nodes, ctypes.c_int32),
+            _ptr(self.dirichlet_values, ctypes.c_double), len(self.constraints), cons,
+            self.circuit_count, self.depth_m)
+        return self._c
+
+    def element_areas(self) -> np.ndarray:
+        p = self.node_xy[self.triangles]
+        return 0.5 * ((p[:, 1, 0] - p[:, 0, 0]) * (p[:, 2, 1] - p[:, 0, 1])
+                      - (p[:, 2, 0] - p[:, 0, 0]) * (p[:, 1, 1] - p[:, 0, 1]))
+
+
+def load_artifact(path):
+    """Load a ``gpu_femm_magnetostatic_mesh_v1`` / ``gpu_femm_planar_dc_mesh_v1`` artifact.
+
+    Follows the unit and label mapping of gpu_solver/src/io/mesh_artifact.cpp:198-418:
+    mm -> m, nu0 = 1/(mu0 mu_r), base materials first and then one expanded material per
+    region (group, circuit, PM angle, J_per_A = turns / region area), zero outer
+    Dirichlet, signed node constraints.  Returns (Model, circuit currents).
+    """
+    import json
+
+    with open(path, "r", encoding="utf-8") as handle:
+        doc = json.load(handle)
+    schema = doc["schema_version"]
+    if schema not in ("gpu_femm_magnetostatic_mesh_v1", "gpu_femm_planar_dc_mesh_v1"):
+        raise ValueError(f"unsupported artifact schema {schema}")
+    res = doc["resolved"]
+    model_info = res["model"]
+    if model_info["frequency_hz"] != 0:
+        raise ValueError("unsupported model physics")
+    problem = AXISYMMETRIC if model_info["problem_type"] == "axisymmetric" else PLANAR
+    nodes = np.asarray(res["nodes_mm"], np.float64) * 1e-3
+    materials, curves, lookup = [], [], {}
+    for entry in res["materials"]:
+        if entry["mu_x"] != entry["mu_y"] or entry["lam_type"] != 0 or entry["lam_fill"] != 1:
+            raise ValueError("invalid material definition")
+        mat = Material(1.0 / (MU0 * entry["mu_x"]), hc=float(entry["H_c_A_per_m"]))
+        if entry["B_T"]:
+            mat.bh_curve = len(curves)
+            curves.append((np.asarray(entry["B_T"], np.float64),
+                           np.asarray(entry["H_A_per_m"], np.float64)))
+        lookup[entry["id"]] = len(materials)
+        materials.append(mat)
+    currents = [float(c["current_A"]) for c in res["circuits"]]
+    regions = res["regions"]
+    region_index = {r["id"]: i for i, r in enumerate(regions)}
+    tris = np.asarray(res["triangles"]["node_indices"], np.int32)
+    tri_region = np.asarray([region_index[r] for r in res["triangles"]["region_ids"]], np.int32)
+    p = nodes[tris]
+    area = 0.5 * ((p[:, 1, 0] - p[:, 0, 0]) * (p[:, 2, 1] - p[:, 0, 1])
+                  - (p[:, 2, 0] - p[:, 0, 0]) * (p[:, 1, 1] - p[:, 0, 1]))
+    if np.any(area <= 0):
+        raise ValueError("triangle is not CCW")
+    region_area = np.zeros(len(regions))
+    for e in range(len(tris)):   # element order, as the reference sums areas
+        region_area[tri_region[e]] += area[e]
+    base = len(materials)
+    for i, r in enumerate(regions):
+        src = materials[lookup[r["material_id"]]]
+        circuit = int(r["circuit_index"])
+        materials.append(Material(src.reluctivity_zero, src.alpha,
+                                  0.0 if circuit < 0 else r["turns"] / region_area[i], src.hc,
+                                  float(r["pm_magnetization_deg"]), src.bh_curve, circuit,
+                                  int(r["group_number"])))
+    od = res["outer_dirichlet"]
+    cons = []
+    for c in res.get("node_constraints", []):
+        cons.append((c["node_a"], c["node_b"], 1 if c["relation"] == "periodic" else -1))
+    model = Model(nodes, tris, base + tri_region, materials, curves,
+                  np.asarray(od["node_indices"], np.int32),
+                  np.asarray(od["A_Wb_per_m"], np.float64),
+                  np.asarray(cons, np.int32).reshape(-1, 3), len(currents),
+                  float(model_info["depth_mm"]) * 1e-3, problem, name=str(path))
+    return model, np.asarray(currents, np.float64)

@@ -1,0 +1,243 @@
+"""ctypes access to the CPU oracle and to the reference host-source bridge.
+
+Test infrastructure: the oracle is the checker, never the thing measured.
+"""
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+from femm_gpu_amd.model import CModel, CNewtonOptions, CMaterial, CConstraint
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_SO = ROOT / "oracle" / "_build" / "liboracle.so"
+REF_SO = ROOT / "oracle" / "_ref" / "libref_bridge.so"
+
+P = ctypes.POINTER
+c_d, c_i32, c_i8 = ctypes.c_double, ctypes.c_int32, ctypes.c_int8
+
+
+class OracleResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("newton_iterations", ctypes.c_int32),
+                ("linear_iterations", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("relative_step", ctypes.c_double), ("equation_residual", ctypes.c_double)]
+
+
+def _dp(a):
+    return a.ctypes.data_as(P(c_d))
+
+
+def _ip(a):
+    return a.ctypes.data_as(P(c_i32))
+
+
+def _bp(a):
+    return a.ctypes.data_as(P(c_i8))
+
+
+_oracle = None
+_ref = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not ORACLE_SO.exists():
+            raise RuntimeError(f"oracle not built: {ORACLE_SO} (run make -C oracle)")
+        _oracle = ctypes.CDLL(str(ORACLE_SO))
+    return _oracle
+
+
+def ref_available() -> bool:
+    return REF_SO.exists()
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        _ref = ctypes.CDLL(str(REF_SO), mode=os.RTLD_LAZY)
+    return _ref
+
+
+def bh_spline(b, h, lib="oracle"):
+    l = oracle() if lib == "oracle" else ref()
+    fn = l.oracle_bh_spline if lib == "oracle" else l.ref_bh_spline
+    n = len(b)
+    b = np.ascontiguousarray(b, np.float64)
+    h = np.ascontiguousarray(h, np.float64)
+    ob, oh, os_ = np.zeros(n), np.zeros(n), np.zeros(n)
+    cold = c_d(0)
+    passes = ctypes.c_int(0)
+    ok = fn(_dp(b), _dp(h), n, _dp(ob), _dp(oh), _dp(os_), ctypes.byref(cold), ctypes.byref(passes))
+    return bool(ok), ob, oh, os_, cold.value, passes.value
+
+
+def bh_eval(b, h, s, field, lib="oracle"):
+    l = oracle() if lib == "oracle" else ref()
+    fn = l.oracle_bh_eval if lib == "oracle" else l.ref_bh_eval
+    out = np.zeros(4)
+    b, h, s = (np.ascontiguousarray(x, np.float64) for x in (b, h, s))
+    fn(_dp(b), _dp(h), _dp(s), len(b), c_d(field), _dp(out))
+    return out
+
+
+def validate(model, lib="oracle"):
+    if lib == "oracle":
+        return oracle().oracle_validate(ctypes.byref(model.c_struct()))
+    return ref().ref_validate(ctypes.byref(model.c_struct()))
+
+
+def assemble(model, a_old, currents, newton, lib="oracle"):
+    """Reduced Newton system as a dict of numpy arrays, or raises on status."""
+    l = oracle() if lib == "oracle" else ref()
+    pre = "oracle_" if lib == "oracle" else "ref_"
+    handle = ctypes.c_void_p()
+    a_old = np.ascontiguousarray(a_old, np.float64)
+    currents = np.ascontiguousarray(currents, np.float64)
+    st = getattr(l, pre + "assemble")(ctypes.byref(model.c_struct()), _dp(a_old), _dp(currents),
+                                      int(newton), ctypes.byref(handle))
+    if st != 0:
+        return st, None
+    n, nnz = c_i32(), c_i32()
+    getattr(l, pre + "assembly_sizes")(handle, ctypes.byref(n), ctypes.byref(nnz))
+    n, nnz, N = n.value, nnz.value, model.node_count
+    out = dict(rows=np.zeros(n + 1, np.int32), cols=np.zeros(nnz, np.int32),
+               values=np.zeros(nnz), diag=np.zeros(n), rhs_per_amp=np.zeros(n),
+               rhs_offset=np.zeros(n), free_nodes=np.zeros(n, np.int32),
+               node_to_free=np.zeros(N, np.int32), node_sign=np.zeros(N, np.int8),
+               boundary=np.zeros(N))
+    getattr(l, pre + "assembly_copy")(handle, _ip(out["rows"]), _ip(out["cols"]),
+                                      _dp(out["values"]), _dp(out["diag"]),
+                                      _dp(out["rhs_per_amp"]), _dp(out["rhs_offset"]),
+                                      _ip(out["free_nodes"]), _ip(out["node_to_free"]),
+                                      _bp(out["node_sign"]), _dp(out["boundary"]))
+    getattr(l, pre + "assembly_destroy")(handle)
+    return 0, out
+
+
+def prepare_symbolic(model, lib="oracle"):
+    l = oracle() if lib == "oracle" else ref()
+    pre = "oracle_" if lib == "oracle" else "ref_"
+    handle = ctypes.c_void_p()
+    st = getattr(l, pre + "prepare_symbolic")(ctypes.byref(model.c_struct()), ctypes.byref(handle))
+    if st != 0:
+        return st, None
+    n, nnz, me, re = c_i32(), c_i32(), c_i32(), c_i32()
+    getattr(l, pre + "symbolic_sizes")(handle, ctypes.byref(n), ctypes.byref(nnz),
+                                       ctypes.byref(me), ctypes.byref(re))
+    n, nnz, me, re, N = n.value, nnz.value, me.value, re.value, model.node_count
+    out = dict(rows=np.zeros(n + 1, np.int32), cols=np.zeros(nnz, np.int32),
+               diag_slots=np.zeros(n, np.int32), mat_offsets=np.zeros(nnz + 1, np.int32),
+               mat_entries=np.zeros((me, 4), np.int32), rhs_offsets=np.zeros(n + 1, np.int32),
+               rhs_entries=np.zeros((re, 3), np.int32), pm_rhs=np.zeros(n),
+               circuit_rhs=np.zeros(max(1, model.circuit_count) * n),
+               free_nodes=np.zeros(n, np.int32), node_to_free=np.zeros(N, np.int32),
+               node_sign=np.zeros(N, np.int8), boundary=np.zeros(N))
+    getattr(l, pre + "symbolic_copy")(
+        handle, _ip(out["rows"]), _ip(out["cols"]), _ip(out["diag_slots"]),
+        _ip(out["mat_offsets"]), _ip(out["mat_entries"]), _ip(out["rhs_offsets"]),
+        _ip(out["rhs_entries"]), _dp(out["pm_rhs"]), _dp(out["circuit_rhs"]),
+        _ip(out["free_nodes"]), _ip(out["node_to_free"]), _bp(out["node_sign"]),
+        _dp(out["boundary"]))
+    getattr(l, pre + "symbolic_destroy")(handle)
+    out["circuit_rhs"] = out["circuit_rhs"][: model.circuit_count * n].reshape(
+        model.circuit_count, n)
+    return 0, out
+
+
+def pcg(rows, cols, values, diag, rhs, tol, maxit):
+    n = len(rhs)
+    x = np.zeros(n)
+    info = np.zeros(3)
+    rows, cols = (np.ascontiguousarray(a, np.int32) for a in (rows, cols))
+    values, diag, rhs = (np.ascontiguousarray(a, np.float64) for a in (values, diag, rhs))
+    oracle().oracle_pcg(n, _ip(rows), _ip(cols), _dp(values), _dp(diag), _dp(rhs), c_d(tol),
+                        int(maxit), _dp(x), _dp(info))
+    return x, int(info[0]), int(info[1]), info[2]
+
+
+def newton_solve(model, currents, options, warm_start=None):
+    N, T, C = model.node_count, model.triangle_count, model.circuit_count
+    a, bx, by, flux = np.zeros(N), np.zeros(T), np.zeros(T), np.zeros(max(1, C))
+    res = OracleResult()
+    currents = np.ascontiguousarray(currents, np.float64)
+    warm = None if warm_start is None else np.ascontiguousarray(warm_start, np.float64)
+    st = oracle().oracle_newton_solve(ctypes.byref(model.c_struct()), _dp(currents),
+                                      ctypes.byref(options), None if warm is None else _dp(warm),
+                                      _dp(a), _dp(bx), _dp(by), _dp(flux), ctypes.byref(res))
+    return dict(status=st, a=a, bx=bx, by=by, flux=flux[:C], newton=res.newton_iterations,
+                linear=res.linear_iterations, step=res.relative_step,
+                eq=res.equation_residual)
+
+
+def newton_batch(model, currents, options, threads):
+    currents = np.ascontiguousarray(currents, np.float64)
+    B = currents.shape[0]
+    N, C = model.node_count, model.circuit_count
+    a, flux = np.zeros((B, N)), np.zeros((B, C))
+    results = (OracleResult * B)()
+    oracle().oracle_newton_batch(ctypes.byref(model.c_struct()), B, _dp(currents),
+                                 ctypes.byref(options), int(threads), _dp(a), _dp(flux), results)
+    return a, flux, [dict(status=r.status, newton=r.newton_iterations, linear=r.linear_iterations,
+                          step=r.relative_step, eq=r.equation_residual) for r in results]
+
+
+def equation_residual(model, a, currents):
+    oracle().oracle_equation_residual.restype = c_d
+    a = np.ascontiguousarray(a, np.float64)
+    currents = np.ascontiguousarray(currents, np.float64)
+    return oracle().oracle_equation_residual(ctypes.byref(model.c_struct()), _dp(a),
+                                             _dp(currents))
+
+
+def finalize(model, a):
+    T, C = model.triangle_count, model.circuit_count
+    bx, by, flux = np.zeros(T), np.zeros(T), np.zeros(max(1, C))
+    a = np.ascontiguousarray(a, np.float64)
+    st = oracle().oracle_finalize(ctypes.byref(model.c_struct()), _dp(a), _dp(bx), _dp(by),
+                                  _dp(flux))
+    return st, bx, by, flux[:C]
+
+
+def weighted_stress(model, a, bx, by, selected_label=1, air_label=3, selected_group=-1,
+                    air_group=-1, radius_m=0.0, angles_rad=()):
+    angles = np.ascontiguousarray(angles_rad, np.float64)
+    out, br, mask = np.zeros(3), np.zeros(max(1, len(angles))), np.zeros(model.node_count)
+    a, bx, by = (np.ascontiguousarray(v, np.float64) for v in (a, bx, by))
+    st = oracle().oracle_weighted_stress(ctypes.byref(model.c_struct()), _dp(a), _dp(bx), _dp(by),
+                                         selected_label, air_label, selected_group, air_group,
+                                         c_d(radius_m), _dp(angles), len(angles), _dp(out),
+                                         _dp(br), _dp(mask))
+    return st, out, br[: len(angles)], mask
+
+
+def ref_parse_artifact(path):
+    """Model arrays from the reference artifact parser (for loader parity)."""
+    text = Path(path).read_bytes()
+    err = ctypes.create_string_buffer(512)
+    l = ref()
+    l.ref_parse_artifact.restype = ctypes.c_void_p
+    handle = l.ref_parse_artifact(text, len(text), err, 512)
+    if not handle:
+        raise ValueError(err.value.decode())
+    handle = ctypes.c_void_p(handle)
+    sizes = np.zeros(6, np.int32)
+    l.ref_artifact_sizes(handle, _ip(sizes))
+    N, T, M, K, D, S = (int(s) for s in sizes)
+    xy, tri, mat = np.zeros(2 * N), np.zeros(3 * T, np.int32), np.zeros(T, np.int32)
+    mats = (CMaterial * M)()
+    dn = np.zeros(D, np.int32)
+    currents = np.zeros(64)
+    cons = (CConstraint * max(1, S))()
+    depth, problem, circuits = c_d(), c_i32(), c_i32()
+    l.ref_artifact_copy(handle, _dp(xy), _ip(tri), _ip(mat), mats, _ip(dn), _dp(currents), cons,
+                        ctypes.byref(depth), ctypes.byref(problem), ctypes.byref(circuits))
+    l.ref_artifact_destroy(handle)
+    return dict(node_xy=xy.reshape(N, 2), triangles=tri.reshape(T, 3), triangle_material=mat,
+                materials=[(m.reluctivity_zero, m.alpha, m.source_j_per_a, m.hc,
+                            m.magnetization_deg, m.bh_curve, m.circuit, m.group) for m in mats],
+                dirichlet_nodes=dn, currents=currents[: circuits.value],
+                constraints=[(c.node_a, c.node_b, c.sign) for c in list(cons)[:S]],
+                depth_m=depth.value, problem_type=problem.value, curve_count=K)
