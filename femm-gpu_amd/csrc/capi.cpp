@@ -1,0 +1,388 @@
+// C ABI of the MI355X solve path (declared in include/femm_gpu_amd.h).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "femm_gpu_amd.h"
+#include "plan.hpp"
+
+struct femm_plan {
+  femm::HostPlan host;
+  femm::DeviceBuffers dev;
+  // staging for the host-pointer entry points
+  void *io = nullptr;
+  size_t io_bytes = 0;
+};
+
+namespace
+{
+
+constexpr const char *kStatusNames[] = {
+    "OK", "INPUT_IO", "INVALID_ARGUMENT", "UNSUPPORTED_FEATURE", "MESH_INVALID",
+    "BOUNDARY_INVALID", "GPU_UNAVAILABLE", "GPU_ALLOCATION_FAILED", "ASSEMBLY_FAILED",
+    "LINEAR_SOLVE_NOT_CONVERGED", "LINEAR_SOLVE_BREAKDOWN", "NUMERICAL_NONFINITE", "OUTPUT_IO",
+    "INTERNAL_ERROR", "NONLINEAR_SOLVE_NOT_CONVERGED", "INVALID_MATERIAL"};
+
+bool GpuPresent()
+{
+  int count = 0;
+  return hipGetDeviceCount(&count) == hipSuccess && count > 0;
+}
+
+int EnsureIo(femm_plan *plan, size_t bytes)
+{
+  if (plan->io_bytes >= bytes)
+    return FEMM_OK;
+  if (plan->io) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(plan->io);
+    plan->io = nullptr;
+    plan->io_bytes = 0;
+  }
+  if (hipMalloc(&plan->io, bytes) != hipSuccess)
+    return FEMM_GPU_ALLOCATION_FAILED;
+  plan->io_bytes = bytes;
+  return FEMM_OK;
+}
+
+size_t Round(size_t bytes) { return (bytes + 255) / 256 * 256; }
+
+int EnsureDevice(femm_plan *plan)
+{
+  if (plan->dev.base != nullptr)
+    return FEMM_OK;
+  if (!GpuPresent())
+    return FEMM_GPU_UNAVAILABLE;
+  const int st = femm::UploadPlan(plan->host, &plan->dev);
+  if (st != FEMM_OK)
+    femm::ReleaseDevice(&plan->dev);
+  return st;
+}
+
+femm_newton_options_t DefaultOptions()
+{
+  femm_newton_options_t o;
+  o.relative_tolerance = 1e-12;
+  o.linear_relative_tolerance = 1e-13;
+  o.max_newton_iterations = 32;
+  o.max_linear_iterations = 128;
+  o.recover_nonconvergence = 0;
+  o.adaptive_linear_tolerance = 0;
+  return o;
+}
+
+femm_solve_config_t DefaultConfig()
+{
+  femm_solve_config_t c;
+  c.precision = FEMM_PRECISION_FP64;
+  c.pcg_warm_start = 0;
+  c.verify_equation_residual = 1;
+  c.max_refinements = 64;
+  c.inner_relative_tolerance = 1e-4;
+  return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+int femm_plan_create(const femm_model_t *model, femm_plan **out)
+{
+  if (out == nullptr)
+    return FEMM_INVALID_ARGUMENT;
+  *out = nullptr;
+  femm_plan *plan = new (std::nothrow) femm_plan;
+  if (plan == nullptr)
+    return FEMM_INTERNAL_ERROR;
+  std::string error;
+  int st = femm::BuildHostPlan(model, &plan->host, &error);
+  if (st != FEMM_OK) {
+    delete plan;
+    return st;
+  }
+  // The device copy is uploaded by the first call that computes, so the
+  // symbolic plan is available on hosts without a GPU.
+  *out = plan;
+  return FEMM_OK;
+}
+
+void femm_plan_destroy(femm_plan *plan)
+{
+  if (plan == nullptr)
+    return;
+  if (plan->dev.base != nullptr || plan->io != nullptr)
+    (void)hipDeviceSynchronize();
+  femm::ReleaseDevice(&plan->dev);
+  if (plan->io)
+    (void)hipFree(plan->io);
+  delete plan;
+}
+
+int femm_plan_sizes(const femm_plan *plan, int32_t sizes[6])
+{
+  if (plan == nullptr || sizes == nullptr)
+    return FEMM_INVALID_ARGUMENT;
+  const femm::HostPlan &h = plan->host;
+  sizes[0] = h.N;
+  sizes[1] = h.T;
+  sizes[2] = h.n;
+  sizes[3] = h.nnz;
+  sizes[4] = h.C;
+  sizes[5] = h.Q;
+  return FEMM_OK;
+}
+
+int femm_plan_copy_symbolic(const femm_plan *plan, int32_t *row_offsets, int32_t *column_indices,
+                            int32_t *diagonal_slots, int32_t *free_nodes, int32_t *node_to_free,
+                            int8_t *node_sign, double *boundary_values)
+{
+  if (plan == nullptr)
+    return FEMM_INVALID_ARGUMENT;
+  const femm::HostPlan &h = plan->host;
+  if (row_offsets)
+    std::copy(h.row_offsets.begin(), h.row_offsets.end(), row_offsets);
+  if (column_indices)
+    std::copy(h.cols.begin(), h.cols.end(), column_indices);
+  if (diagonal_slots)
+    std::copy(h.diag_slot.begin(), h.diag_slot.end(), diagonal_slots);
+  if (free_nodes)
+    std::copy(h.free_nodes.begin(), h.free_nodes.end(), free_nodes);
+  if (node_to_free)
+    std::copy(h.node_to_free.begin(), h.node_to_free.end(), node_to_free);
+  if (node_sign)
+    std::copy(h.node_sign.begin(), h.node_sign.end(), node_sign);
+  if (boundary_values)
+    std::copy(h.boundary.begin(), h.boundary.end(), boundary_values);
+  return FEMM_OK;
+}
+
+int femm_assemble_batch(femm_plan *plan, int32_t items, const double *states,
+                        const double *currents, int32_t use_newton, double *values, double *rhs,
+                        double *diagonal)
+{
+  if (plan == nullptr || items <= 0 || states == nullptr || currents == nullptr)
+    return FEMM_INVALID_ARGUMENT;
+  const femm::HostPlan &h = plan->host;
+  int st = EnsureDevice(plan);
+  if (st == FEMM_OK)
+    st = femm::EnsureWork(h, items, false, &plan->dev);
+  if (st != FEMM_OK)
+    return st;
+  const size_t ns = Round(sizeof(double) * items * static_cast<size_t>(h.N));
+  const size_t nc = Round(sizeof(double) * items * static_cast<size_t>(std::max(h.C, 1)));
+  const size_t nv = Round(sizeof(int32_t) * items);
+  if ((st = EnsureIo(plan, ns + nc + nv)) != FEMM_OK)
+    return st;
+  char *io = static_cast<char *>(plan->io);
+  double *d_states = reinterpret_cast<double *>(io);
+  double *d_cur = reinterpret_cast<double *>(io + ns);
+  int32_t *d_valid = reinterpret_cast<int32_t *>(io + ns + nc);
+  if (hipMemcpy(d_states, states, sizeof(double) * items * static_cast<size_t>(h.N),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d_cur, currents, sizeof(double) * items * static_cast<size_t>(h.C),
+                hipMemcpyHostToDevice) != hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  if ((st = femm::LaunchAssemble(plan->dev, items, d_states, d_cur, use_newton, d_valid,
+                                 nullptr)) != FEMM_OK)
+    return st;
+  std::vector<double> sell(static_cast<size_t>(items) * h.sell_size);
+  std::vector<double> r(static_cast<size_t>(items) * h.n);
+  std::vector<int32_t> valid(items);
+  if (hipDeviceSynchronize() != hipSuccess ||
+      hipMemcpy(sell.data(), plan->dev.work.vals, sizeof(double) * sell.size(),
+                hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(r.data(), plan->dev.work.rhs, sizeof(double) * r.size(), hipMemcpyDeviceToHost) !=
+          hipSuccess ||
+      hipMemcpy(valid.data(), d_valid, sizeof(int32_t) * items, hipMemcpyDeviceToHost) !=
+          hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  for (int32_t item = 0; item < items; ++item) {
+    const double *src = sell.data() + static_cast<size_t>(item) * h.sell_size;
+    if (values)
+      for (int32_t slot = 0; slot < h.nnz; ++slot)
+        values[static_cast<size_t>(item) * h.nnz + slot] = src[h.slot_to_sell[slot]];
+    if (diagonal)
+      for (int32_t row = 0; row < h.n; ++row)
+        diagonal[static_cast<size_t>(item) * h.n + row] = src[h.diag_sell[row]];
+    if (rhs)
+      std::copy(r.begin() + static_cast<size_t>(item) * h.n,
+                r.begin() + static_cast<size_t>(item + 1) * h.n,
+                rhs + static_cast<size_t>(item) * h.n);
+  }
+  for (int32_t v : valid)
+    if (!v)
+      return FEMM_ASSEMBLY_FAILED;
+  return FEMM_OK;
+}
+
+int femm_pcg_batch(femm_plan *plan, int32_t items, const double *values, const double *rhs,
+                   double relative_tolerance, int32_t max_iterations, double *x, double *info)
+{
+  if (plan == nullptr || items <= 0 || values == nullptr || rhs == nullptr)
+    return FEMM_INVALID_ARGUMENT;
+  const femm::HostPlan &h = plan->host;
+  int st = EnsureDevice(plan);
+  if (st == FEMM_OK)
+    st = femm::EnsureWork(h, items, false, &plan->dev);
+  if (st != FEMM_OK)
+    return st;
+  std::vector<double> sell(static_cast<size_t>(items) * h.sell_size, 0.0);
+  for (int32_t item = 0; item < items; ++item)
+    for (int32_t slot = 0; slot < h.nnz; ++slot)
+      sell[static_cast<size_t>(item) * h.sell_size + h.slot_to_sell[slot]] =
+          values[static_cast<size_t>(item) * h.nnz + slot];
+  const size_t ni = Round(sizeof(double) * 3 * items);
+  if ((st = EnsureIo(plan, ni)) != FEMM_OK)
+    return st;
+  double *d_info = static_cast<double *>(plan->io);
+  if (hipMemcpy(plan->dev.work.vals, sell.data(), sizeof(double) * sell.size(),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(plan->dev.work.rhs, rhs, sizeof(double) * items * static_cast<size_t>(h.n),
+                hipMemcpyHostToDevice) != hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  if ((st = femm::LaunchPcg(plan->dev, items, relative_tolerance, max_iterations, d_info,
+                            nullptr)) != FEMM_OK)
+    return st;
+  if (hipDeviceSynchronize() != hipSuccess ||
+      (x && hipMemcpy(x, plan->dev.work.x, sizeof(double) * items * static_cast<size_t>(h.n),
+                      hipMemcpyDeviceToHost) != hipSuccess) ||
+      (info && hipMemcpy(info, d_info, sizeof(double) * 3 * items, hipMemcpyDeviceToHost) !=
+                   hipSuccess))
+    return FEMM_INTERNAL_ERROR;
+  return FEMM_OK;
+}
+
+int femm_newton_batch_device(femm_plan *plan, int32_t items, const double *currents,
+                             const femm_newton_options_t *options,
+                             const femm_solve_config_t *config, const double *warm_starts,
+                             double *a, double *flux, femm_item_result_t *results, void *stream)
+{
+  if (plan == nullptr || items <= 0 || currents == nullptr || a == nullptr || flux == nullptr ||
+      results == nullptr)
+    return FEMM_INVALID_ARGUMENT;
+  const femm_newton_options_t o = options ? *options : DefaultOptions();
+  const femm_solve_config_t c = config ? *config : DefaultConfig();
+  if (c.precision != FEMM_PRECISION_FP64 && c.precision != FEMM_PRECISION_MIXED)
+    return FEMM_INVALID_ARGUMENT;
+  int st = EnsureDevice(plan);
+  if (st == FEMM_OK)
+    st = femm::EnsureWork(plan->host, items, c.precision == FEMM_PRECISION_MIXED, &plan->dev);
+  if (st != FEMM_OK)
+    return st;
+  return femm::LaunchNewton(plan->dev, items, currents, o, c, warm_starts, a, flux, results,
+                            static_cast<hipStream_t>(stream));
+}
+
+int femm_newton_batch(femm_plan *plan, int32_t items, const double *currents,
+                      const femm_newton_options_t *options, const femm_solve_config_t *config,
+                      const double *warm_starts, double *a, double *bx, double *by, double *flux,
+                      femm_item_result_t *results)
+{
+  if (plan == nullptr || items <= 0 || currents == nullptr || a == nullptr || flux == nullptr ||
+      results == nullptr)
+    return FEMM_INVALID_ARGUMENT;
+  const femm::HostPlan &h = plan->host;
+  const size_t B = static_cast<size_t>(items);
+  const size_t sa = Round(sizeof(double) * B * h.N), sc = Round(sizeof(double) * B * std::max(h.C, 1));
+  const size_t sr = Round(sizeof(femm_item_result_t) * B);
+  const size_t sb = (bx && by) ? Round(sizeof(double) * B * h.T) : 0;
+  const size_t sw = warm_starts ? sa : 0;
+  int st = EnsureDevice(plan);
+  if (st == FEMM_OK)
+    st = EnsureIo(plan, sa + 2 * sc + sr + 2 * sb + sw);
+  if (st != FEMM_OK)
+    return st;
+  char *io = static_cast<char *>(plan->io);
+  double *d_a = reinterpret_cast<double *>(io);
+  double *d_cur = reinterpret_cast<double *>(io + sa);
+  double *d_flux = reinterpret_cast<double *>(io + sa + sc);
+  auto *d_res = reinterpret_cast<femm_item_result_t *>(io + sa + 2 * sc);
+  double *d_bx = sb ? reinterpret_cast<double *>(io + sa + 2 * sc + sr) : nullptr;
+  double *d_by = sb ? reinterpret_cast<double *>(io + sa + 2 * sc + sr + sb) : nullptr;
+  double *d_warm = sw ? reinterpret_cast<double *>(io + sa + 2 * sc + sr + 2 * sb) : nullptr;
+  if (hipMemcpy(d_cur, currents, sizeof(double) * B * h.C, hipMemcpyHostToDevice) != hipSuccess ||
+      (d_warm && hipMemcpy(d_warm, warm_starts, sizeof(double) * B * h.N,
+                           hipMemcpyHostToDevice) != hipSuccess))
+    return FEMM_INTERNAL_ERROR;
+  st = femm_newton_batch_device(plan, items, d_cur, options, config, d_warm, d_a, d_flux, d_res,
+                                nullptr);
+  if (st != FEMM_OK)
+    return st;
+  if (sb && (st = femm::LaunchFinalize(plan->dev, items, d_a, d_bx, d_by, nullptr, nullptr)) !=
+                FEMM_OK)
+    return st;
+  if (hipDeviceSynchronize() != hipSuccess ||
+      hipMemcpy(a, d_a, sizeof(double) * B * h.N, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(flux, d_flux, sizeof(double) * B * h.C, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(results, d_res, sizeof(femm_item_result_t) * B, hipMemcpyDeviceToHost) !=
+          hipSuccess ||
+      (sb && (hipMemcpy(bx, d_bx, sizeof(double) * B * h.T, hipMemcpyDeviceToHost) != hipSuccess ||
+              hipMemcpy(by, d_by, sizeof(double) * B * h.T, hipMemcpyDeviceToHost) != hipSuccess)))
+    return FEMM_INTERNAL_ERROR;
+  return FEMM_OK;
+}
+
+int femm_finalize_batch(femm_plan *plan, int32_t items, const double *states, double *bx,
+                        double *by, double *flux)
+{
+  if (plan == nullptr || items <= 0 || states == nullptr)
+    return FEMM_INVALID_ARGUMENT;
+  const femm::HostPlan &h = plan->host;
+  const size_t B = static_cast<size_t>(items);
+  const size_t sa = Round(sizeof(double) * B * h.N), st_ = Round(sizeof(double) * B * h.T);
+  const size_t sc = Round(sizeof(double) * B * std::max(h.C, 1));
+  int st = EnsureDevice(plan);
+  if (st == FEMM_OK)
+    st = EnsureIo(plan, sa + 2 * st_ + sc);
+  if (st != FEMM_OK)
+    return st;
+  char *io = static_cast<char *>(plan->io);
+  double *d_a = reinterpret_cast<double *>(io);
+  double *d_bx = reinterpret_cast<double *>(io + sa);
+  double *d_by = reinterpret_cast<double *>(io + sa + st_);
+  double *d_flux = reinterpret_cast<double *>(io + sa + 2 * st_);
+  if (hipMemcpy(d_a, states, sizeof(double) * B * h.N, hipMemcpyHostToDevice) != hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  if ((st = femm::LaunchFinalize(plan->dev, items, d_a, d_bx, d_by, d_flux, nullptr)) != FEMM_OK)
+    return st;
+  if (hipDeviceSynchronize() != hipSuccess ||
+      (bx && hipMemcpy(bx, d_bx, sizeof(double) * B * h.T, hipMemcpyDeviceToHost) != hipSuccess) ||
+      (by && hipMemcpy(by, d_by, sizeof(double) * B * h.T, hipMemcpyDeviceToHost) != hipSuccess) ||
+      (flux && hipMemcpy(flux, d_flux, sizeof(double) * B * h.C, hipMemcpyDeviceToHost) !=
+                   hipSuccess))
+    return FEMM_INTERNAL_ERROR;
+  return FEMM_OK;
+}
+
+const char *femm_status_name(int32_t status)
+{
+  if (status < 0 || status >= static_cast<int32_t>(sizeof(kStatusNames) / sizeof(kStatusNames[0])))
+    return "UNKNOWN";
+  return kStatusNames[status];
+}
+
+int femm_device_info(char *text, size_t capacity)
+{
+  if (text == nullptr || capacity == 0)
+    return FEMM_INVALID_ARGUMENT;
+  hipDeviceProp_t prop;
+  int dev = 0;
+  if (!GpuPresent() || hipGetDevice(&dev) != hipSuccess ||
+      hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+    std::snprintf(text, capacity, "no HIP device");
+    return FEMM_GPU_UNAVAILABLE;
+  }
+  std::snprintf(text, capacity, "%s %s CUs=%d HBM=%.1fGiB L2=%dKiB", prop.name, prop.gcnArchName,
+                prop.multiProcessorCount, prop.totalGlobalMem / 1073741824.0,
+                prop.l2CacheSize / 1024);
+  return FEMM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1138 @@
+// HIP kernels of the MI355X nonlinear magnetostatic solve path (gfx950, wave64).
+//
+// Work decomposition: one 1024-thread workgroup owns one batch item for the
+// whole Newton solve.  Every reduction is a fixed-shape wave64 shuffle tree
+// followed by a fixed-order sum of the 16 wave partials, so results are
+// bitwise reproducible from run to run and independent of batch composition.
+// There are no atomics and no inter-workgroup synchronisation; the hardware
+// dispatcher load-balances items across the 256 CUs as workgroups retire.
+//
+// Memory layout (HBM, item-major): per item the nodal state A[N], the element
+// Newton blocks elem[T][12], the operator values in SELL-64 order, and the PCG
+// vectors x, r, p, Ap, D^-1 [n].  The SELL column indices, contributor lists and
+// element geometry are shared by all items and stay resident in L2 / MALL.
+
+#include "device.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <vector>
+
+namespace femm
+{
+namespace
+{
+
+constexpr int kWaves = kBlockThreads / 64;
+
+#define HIP_TRY(expr)                                                                              \
+  do {                                                                                             \
+    if ((expr) != hipSuccess)                                                                      \
+      return FEMM_INTERNAL_ERROR;                                                                  \
+  } while (0)
+
+// ----------------------------------------------------------------------------- reductions
+
+struct BlockReducer {
+  double *lds;   // [2][kWaves][4]
+  int buf;
+
+  template <int K>
+  __device__ void Sum(double (&v)[K])
+  {
+    static_assert(K <= 4, "at most four simultaneous sums");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double x = v[k];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1)
+        x += __shfl_down(x, off, 64);
+      v[k] = x;
+    }
+    double *b = lds + buf * (kWaves * 4);
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        b[wave * 4 + k] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double s = b[k];
+#pragma unroll
+      for (int w = 1; w < kWaves; ++w)
+        s += b[w * 4 + k];
+      v[k] = s;
+    }
+    buf ^= 1;
+  }
+
+  __device__ double Max(double v)
+  {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+      v = fmax(v, __shfl_down(v, off, 64));
+    double *b = lds + buf * (kWaves * 4);
+    if (lane == 0)
+      b[wave * 4] = v;
+    __syncthreads();
+    double s = b[0];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w)
+      s = fmax(s, b[w * 4]);
+    buf ^= 1;
+    return s;
+  }
+
+  __device__ int AnyTrue(bool local)
+  {
+    double v = local ? 1.0 : 0.0;
+    return Max(v) > 0.0;
+  }
+};
+
+// ----------------------------------------------------------------------------- B-H curve
+
+// Hermite B-H evaluation of the reference (physics/nonlinear_model.cpp:115-148,
+// physics/nonlinear_assembly.cu:12-54): linear scan for the segment, linear
+// extrapolation beyond the last knot, |B| = 0 gives nu = slope[0], nu' = 0.
+__device__ bool EvalBh(const DevicePlan &P, int off, int cnt, double field, double *nu,
+                       double *dnu)
+{
+  const double b = fabs(field);
+  if (b == 0.0) {
+    *nu = P.curve_s[off];
+    *dnu = 0.0;
+    return isfinite(*nu);
+  }
+  const double *cb = P.curve_b + off, *ch = P.curve_h + off, *cs = P.curve_s + off;
+  const int last = cnt - 1;
+  double h = 0.0, dh = 0.0;
+  if (b > cb[last]) {
+    h = ch[last] + cs[last] * (b - cb[last]);
+    dh = cs[last];
+  } else {
+    bool found = false;
+    for (int i = 0; i < last; ++i) {
+      if (b >= cb[i] && b <= cb[i + 1]) {
+        const double L = cb[i + 1] - cb[i];
+        const double z = (b - cb[i]) / L, z2 = z * z;
+        h = (1.0 - 3.0 * z2 + 2.0 * z2 * z) * ch[i] + z * (1.0 - 2.0 * z + z2) * L * cs[i] +
+            z2 * (3.0 - 2.0 * z) * ch[i + 1] + z2 * (z - 1.0) * L * cs[i + 1];
+        dh = 6.0 * z * (z - 1.0) * ch[i] / L + (1.0 - 4.0 * z + 3.0 * z2) * cs[i] +
+             6.0 * z * (1.0 - z) * ch[i + 1] / L + z * (3.0 * z - 2.0) * cs[i + 1];
+        found = true;
+        break;
+      }
+    }
+    if (!found)
+      return false;
+  }
+  *nu = h / b;
+  *dnu = 0.5 * (dh / (b * b) - h / (b * b * b));
+  return isfinite(*nu) && isfinite(*dnu);
+}
+
+// ----------------------------------------------------------------------------- assembly
+
+// Element pass: B at every quadrature point from the nodal state, nu and
+// dnu/dB^2, then the 3x3 Newton block K + C and the per-node rhs terms
+// C a_old - (K + C) A_boundary (physics/nonlinear_assembly.cu:56-98 and the
+// term-by-term formulas of :100-186).  Returns true if every element is valid.
+__device__ bool ElementPass(const DevicePlan &P, const double *a, double *elem, bool newton)
+{
+  bool ok = true;
+  for (int t = threadIdx.x; t < P.T; t += kBlockThreads) {
+    const int n0 = P.tri_nodes[3 * t], n1 = P.tri_nodes[3 * t + 1], n2 = P.tri_nodes[3 * t + 2];
+    const double av[3] = {a[n0], a[n1], a[n2]};
+    double bv[3] = {0.0, 0.0, 0.0};
+    if (P.any_boundary_value) {
+      bv[0] = P.boundary[n0];
+      bv[1] = P.boundary[n1];
+      bv[2] = P.boundary[n2];
+    }
+    const int mat = P.tri_mat[t];
+    const int coff = P.mat_curve_off[mat], ccnt = P.mat_curve_cnt[mat];
+    const double nu0 = P.mat_nu0[mat], alpha = P.mat_alpha[mat];
+    double K[9], e[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      K[k] = 0.0;
+    for (int q = 0; q < P.Q; ++q) {
+      const size_t g = (static_cast<size_t>(t) * P.Q + q) * 3;
+      const double w = P.tri_w[static_cast<size_t>(t) * P.Q + q];
+      const double gx[3] = {P.tri_gx[g], P.tri_gx[g + 1], P.tri_gx[g + 2]};
+      const double gy[3] = {P.tri_gy[g], P.tri_gy[g + 1], P.tri_gy[g + 2]};
+      const double bx = av[0] * gx[0] + av[1] * gx[1] + av[2] * gx[2];
+      const double by = av[0] * gy[0] + av[1] * gy[1] + av[2] * gy[2];
+      const double mag = hypot(bx, by);
+      double nu = 0.0, dnu = 0.0;
+      bool valid = isfinite(mag);
+      if (valid && ccnt > 0) {
+        if (!newton) {
+          nu = P.mat_cold[mat];
+          dnu = 0.0;
+        } else {
+          valid = EvalBh(P, coff, ccnt, mag, &nu, &dnu);
+        }
+      } else if (valid) {
+        nu = nu0 * (1.0 + alpha * mag * mag);
+        dnu = newton ? nu0 * alpha : 0.0;
+      }
+      if (!valid || !(nu > 0.0) || !isfinite(dnu))
+        ok = false;
+      double bdot[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        bdot[i] = bx * gx[i] + by * gy[i];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const double k = w * nu * (gx[i] * gx[j] + gy[i] * gy[j]);
+          const double c = newton ? 2.0 * w * dnu * bdot[i] * bdot[j] : 0.0;
+          K[3 * i + j] += k + c;
+          if (newton)
+            e[i] += c * av[j];
+          if (bv[j] != 0.0)
+            e[i] -= (k + c) * bv[j];
+        }
+      }
+    }
+    double *out = elem + static_cast<size_t>(t) * kElemStride;
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      out[k] = K[k];
+    out[9] = e[0];
+    out[10] = e[1];
+    out[11] = e[2];
+  }
+  return ok;
+}
+
+// Slot-owned ordered gather into SELL positions (physics/nonlinear_assembly.cu:100-140):
+// contributor order is the host element/i/j emission order; one lane owns a slot.
+template <typename Real32>
+__device__ void GatherValues(const DevicePlan &P, const double *elem, double *vals, Real32 *vals32)
+{
+  for (int slot = threadIdx.x; slot < P.nnz; slot += kBlockThreads) {
+    double sum = 0.0;
+    const int end = P.slot_off[slot + 1];
+    for (int k = P.slot_off[slot]; k < end; ++k) {
+      const int c = P.contrib[k];
+      const double v = elem[static_cast<size_t>(c >> 5) * kElemStride + ((c >> 3) & 3) * 3 +
+                            ((c >> 1) & 3)];
+      sum += (c & 1) ? -v : v;
+    }
+    const int pos = P.slot_to_sell[slot];
+    vals[pos] = sum;
+    if (vals32)
+      vals32[pos] = static_cast<float>(sum);
+  }
+}
+
+// Row-owned rhs (physics/nonlinear_assembly.cu:142-186): PM + circuit sources, then the
+// ordered element terms.  Also the Jacobi inverse (:188-206).
+__device__ bool GatherRhs(const DevicePlan &P, const double *elem, const double *vals,
+                          const double *cur, double *rhs, double *inv, float *inv32)
+{
+  bool ok = true;
+  for (int row = threadIdx.x; row < P.n; row += kBlockThreads) {
+    double s = P.pm_rhs[row];
+    for (int c = 0; c < P.C; ++c)
+      s += cur[c] * P.circuit_rhs[static_cast<size_t>(c) * P.n + row];
+    const int end = P.row_off[row + 1];
+    for (int k = P.row_off[row]; k < end; ++k) {
+      const int c = P.rcontrib[k];
+      const double v = elem[static_cast<size_t>(c >> 3) * kElemStride + 9 + ((c >> 1) & 3)];
+      s += (c & 1) ? -v : v;
+    }
+    rhs[row] = s;
+    const double d = vals[P.diag_sell[row]];
+    const bool good = isfinite(d) && d > 0.0;
+    inv[row] = good ? 1.0 / d : 0.0;
+    if (inv32)
+      inv32[row] = good ? static_cast<float>(1.0 / d) : 0.0f;
+    ok = ok && good;
+  }
+  return ok;
+}
+
+// ----------------------------------------------------------------------------- SpMV
+
+// y = A v over SELL-64 (lane = row of the slice); returns the local sum of
+// w[row] * y[row] when w is given.
+template <typename T, typename V>
+__device__ double Spmv(const DevicePlan &P, const T *vals, const V *v, V *y, const V *w)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double local = 0.0;
+  for (int s = wave; s < P.nslices; s += kWaves) {
+    const int base = P.slice_ptr[s] + lane, width = P.slice_width[s];
+    T sum = 0;
+    for (int k = 0; k < width; ++k) {
+      const int pos = base + k * kSliceRows;
+      sum += vals[pos] * static_cast<T>(v[P.sell_cols[pos]]);
+    }
+    const int row = s * kSliceRows + lane;
+    if (row < P.n) {
+      y[row] = static_cast<V>(sum);
+      if (w)
+        local += static_cast<double>(w[row]) * static_cast<double>(sum);
+    }
+  }
+  return local;
+}
+
+// ----------------------------------------------------------------------------- FP64 PCG
+
+struct LinearInfo {
+  int status;
+  int iterations;
+  double residual;
+};
+
+// Jacobi PCG of the reference (solvers/pcg_kernels.cu:10-344) for one item:
+// rhs scaled by max|b|, fused update of x, r and z = D^-1 r, acceptance only on
+// the recomputed true residual with restart otherwise.  With warm != 0 the
+// iteration starts from the x already in `x` (unscaled) instead of zero; the
+// acceptance test ||b - A x|| <= tol ||b|| is unchanged.
+__device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const double *inv,
+                            const double *rhs, double *x, double *r, double *p, double *ap,
+                            double tol, int maxit, bool warm, BlockReducer &R)
+{
+  LinearInfo info{FEMM_INTERNAL_ERROR, 0, INFINITY};
+  const int n = P.n;
+  if (maxit < 0 || !(tol > 0.0) || !(tol < 1.0) || !isfinite(tol)) {
+    info.status = FEMM_INVALID_ARGUMENT;
+    return info;
+  }
+  // First-invalid-row precedence of the reference's validation scan.
+  double bad = 0.0, bmax = 0.0;
+  for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+    const double d = vals[P.diag_sell[i]];
+    if (!isfinite(rhs[i]) || !isfinite(d) || !(d > 0.0))
+      bad = fmax(bad, static_cast<double>(n - i));
+    bmax = fmax(bmax, fabs(rhs[i]));
+  }
+  const double first = R.Max(bad);
+  if (first != 0.0) {
+    const int row = n - static_cast<int>(first);
+    const double d = vals[P.diag_sell[row]];
+    info.status = (!isfinite(rhs[row]) || !isfinite(d)) ? FEMM_NUMERICAL_NONFINITE
+                                                        : FEMM_LINEAR_SOLVE_BREAKDOWN;
+    return info;
+  }
+  const double scale = R.Max(bmax);
+  if (scale == 0.0) {
+    for (int i = threadIdx.x; i < n; i += kBlockThreads)
+      x[i] = 0.0;
+    __syncthreads();
+    info.status = FEMM_OK;
+    info.residual = 0.0;
+    return info;
+  }
+  if (warm) {
+    for (int i = threadIdx.x; i < n; i += kBlockThreads)
+      x[i] = x[i] / scale;
+    __syncthreads();
+    Spmv(P, vals, x, ap, static_cast<const double *>(nullptr));
+    __syncthreads();
+  }
+  double s0[3] = {0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+    const double b = rhs[i] / scale;
+    const double ri = warm ? b - ap[i] : b;
+    if (!warm)
+      x[i] = 0.0;
+    const double zi = ri * inv[i];
+    r[i] = ri;
+    p[i] = zi;
+    s0[0] += ri * ri;
+    s0[1] += ri * zi;
+    s0[2] += b * b;
+  }
+  R.Sum(s0);
+  if (!isfinite(s0[0]) || !isfinite(s0[1])) {
+    info.status = FEMM_NUMERICAL_NONFINITE;
+    return info;
+  }
+  info.residual = sqrt(s0[0]);
+  double rho = s0[1];
+  const double threshold = tol * sqrt(s0[2]);
+  if (warm && info.residual <= threshold) {
+    // The start already satisfies the acceptance test.
+    for (int i = threadIdx.x; i < n; i += kBlockThreads)
+      x[i] *= scale;
+    __syncthreads();
+    info.status = FEMM_OK;
+    info.residual *= scale;
+    return info;
+  }
+  for (int it = 0; it < maxit; ++it) {
+    double pap[1] = {Spmv(P, vals, p, ap, p)};
+    R.Sum(pap);
+    if (!isfinite(pap[0]) || !isfinite(rho)) {
+      info.status = FEMM_NUMERICAL_NONFINITE;
+      return info;
+    }
+    if (!(pap[0] > 0.0) || !(rho > 0.0)) {
+      info.status = FEMM_LINEAR_SOLVE_BREAKDOWN;
+      return info;
+    }
+    const double alpha = rho / pap[0];
+    double s1[2] = {0.0, 0.0};
+    for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+      x[i] += alpha * p[i];
+      const double ri = r[i] - alpha * ap[i];
+      r[i] = ri;
+      s1[0] += ri * ri;
+      s1[1] += ri * (ri * inv[i]);
+    }
+    R.Sum(s1);
+    info.iterations = it + 1;
+    if (!isfinite(s1[0])) {
+      info.status = FEMM_NUMERICAL_NONFINITE;
+      return info;
+    }
+    info.residual = sqrt(s1[0]);
+    if (info.residual <= threshold) {
+      // True residual on the recomputed b/scale - A x (pcg_kernels.cu:192-270).
+      Spmv(P, vals, x, ap, static_cast<const double *>(nullptr));
+      __syncthreads();
+      double tr[1] = {0.0};
+      for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+        const double ri = rhs[i] / scale - ap[i];
+        r[i] = ri;
+        tr[0] += ri * ri;
+      }
+      R.Sum(tr);
+      if (!isfinite(tr[0])) {
+        info.status = FEMM_NUMERICAL_NONFINITE;
+        return info;
+      }
+      info.residual = sqrt(tr[0]);
+      if (info.residual <= threshold) {
+        bool finite = true;
+        for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+          x[i] *= scale;
+          finite = finite && isfinite(x[i]);
+        }
+        const bool nonfinite = R.AnyTrue(!finite);
+        info.residual *= scale;
+        info.status = (nonfinite || !isfinite(info.residual)) ? FEMM_NUMERICAL_NONFINITE : FEMM_OK;
+        return info;
+      }
+      double rs[1] = {0.0};
+      for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+        const double zi = r[i] * inv[i];
+        p[i] = zi;
+        rs[0] += r[i] * zi;
+      }
+      R.Sum(rs);
+      if (!isfinite(rs[0]) || !(rs[0] > 0.0)) {
+        info.status = FEMM_LINEAR_SOLVE_BREAKDOWN;
+        return info;
+      }
+      rho = rs[0];
+      continue;
+    }
+    if (!isfinite(s1[1]) || !(s1[1] > 0.0)) {
+      info.status = FEMM_LINEAR_SOLVE_BREAKDOWN;
+      return info;
+    }
+    const double beta = s1[1] / rho;
+    for (int i = threadIdx.x; i < n; i += kBlockThreads)
+      p[i] = r[i] * inv[i] + beta * p[i];
+    rho = s1[1];
+    __syncthreads();
+  }
+  // Iteration cap: report the true residual (pcg_kernels.cu:302-343).
+  Spmv(P, vals, x, ap, static_cast<const double *>(nullptr));
+  __syncthreads();
+  double fr[1] = {0.0};
+  for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+    const double ri = rhs[i] / scale - ap[i];
+    fr[0] += ri * ri;
+  }
+  R.Sum(fr);
+  for (int i = threadIdx.x; i < n; i += kBlockThreads)
+    x[i] *= scale;
+  __syncthreads();
+  info.residual = sqrt(fr[0]) * scale;
+  info.status = isfinite(info.residual) ? FEMM_LINEAR_SOLVE_NOT_CONVERGED : FEMM_NUMERICAL_NONFINITE;
+  return info;
+}
+
+// ----------------------------------------------------------------------------- mixed precision
+
+// FP32 Jacobi PCG for A32 d = r (r given in FP64, scaled by rscale before the
+// FP32 conversion so small residuals do not underflow).  Dot products are
+// accumulated in FP64.  Stops when the recurrence residual drops below
+// eps * ||r|| or at maxit; a breakdown (p'Ap <= 0 from FP32 rounding) stops
+// early with the progress made so far.  Returns the iteration count; d32 holds
+// the correction in units of rscale.
+__device__ int PcgInner32(const DevicePlan &P, const float *vals32, const float *inv32,
+                          const double *r64, double rscale, float *d32, float *r32, float *p32,
+                          float *ap32, double eps, int maxit, BlockReducer &R)
+{
+  const int n = P.n;
+  double s0[2] = {0.0, 0.0};
+  for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+    const float ri = static_cast<float>(r64[i] / rscale);
+    const float zi = ri * inv32[i];
+    r32[i] = ri;
+    p32[i] = zi;
+    d32[i] = 0.0f;
+    s0[0] += static_cast<double>(ri) * ri;
+    s0[1] += static_cast<double>(ri) * zi;
+  }
+  R.Sum(s0);
+  double rho = s0[1];
+  const double stop2 = eps * eps * s0[0];
+  int it = 0;
+  for (; it < maxit; ++it) {
+    double pap[1] = {Spmv(P, vals32, p32, ap32, p32)};
+    R.Sum(pap);
+    if (!(pap[0] > 0.0) || !(rho > 0.0) || !isfinite(pap[0]))
+      break;
+    const float alpha = static_cast<float>(rho / pap[0]);
+    double s1[2] = {0.0, 0.0};
+    for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+      d32[i] += alpha * p32[i];
+      const float ri = r32[i] - alpha * ap32[i];
+      r32[i] = ri;
+      s1[0] += static_cast<double>(ri) * ri;
+      s1[1] += static_cast<double>(ri) * (ri * inv32[i]);
+    }
+    R.Sum(s1);
+    if (!(s1[0] > stop2) || !(s1[1] > 0.0) || !isfinite(s1[1])) {
+      ++it;
+      break;
+    }
+    const float beta = static_cast<float>(s1[1] / rho);
+    for (int i = threadIdx.x; i < n; i += kBlockThreads)
+      p32[i] = r32[i] * inv32[i] + beta * p32[i];
+    rho = s1[1];
+    __syncthreads();
+  }
+  __syncthreads();
+  return it;
+}
+
+struct RefinedInfo {
+  LinearInfo linear;
+  int refinements;
+};
+
+// FP64 iterative refinement around FP32 PCG: r = b - A64 x in FP64 (the
+// acceptance test ||b - A x|| <= tol ||b|| is evaluated exactly as in the FP64
+// path), correction from the FP32 solve, x += d.  Starts from the x in `x`
+// (warm) or zero.  If a refinement fails to halve the FP64 residual, the solve
+// finishes in FP64 PCG from the current x.
+__device__ RefinedInfo RefinedSolve(const DevicePlan &P, const double *vals, const double *inv,
+                                    const float *vals32, const float *inv32, const double *rhs,
+                                    double *x, double *r, double *p, double *ap, float *d32,
+                                    float *r32, float *p32, float *ap32, double tol,
+                                    int maxit, bool warm, int max_refinements, double inner_eps,
+                                    BlockReducer &R)
+{
+  RefinedInfo out{{FEMM_INTERNAL_ERROR, 0, INFINITY}, 0};
+  const int n = P.n;
+  double b2[1] = {0.0};
+  bool finite = true;
+  for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+    b2[0] += rhs[i] * rhs[i];
+    finite = finite && isfinite(rhs[i]);
+    if (!warm)
+      x[i] = 0.0;
+  }
+  R.Sum(b2);
+  if (R.AnyTrue(!finite)) {
+    out.linear.status = FEMM_NUMERICAL_NONFINITE;
+    return out;
+  }
+  const double bnorm = sqrt(b2[0]);
+  if (bnorm == 0.0) {
+    for (int i = threadIdx.x; i < n; i += kBlockThreads)
+      x[i] = 0.0;
+    __syncthreads();
+    out.linear = {FEMM_OK, 0, 0.0};
+    return out;
+  }
+  double prev = INFINITY;
+  for (int k = 0; k <= max_refinements; ++k) {
+    Spmv(P, vals, x, ap, static_cast<const double *>(nullptr));
+    __syncthreads();
+    double rr[1] = {0.0};
+    double rmax = 0.0;
+    for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+      const double ri = rhs[i] - ap[i];
+      r[i] = ri;
+      rr[0] += ri * ri;
+      rmax = fmax(rmax, fabs(ri));
+    }
+    R.Sum(rr);
+    const double rscale = R.Max(rmax);
+    const double rnorm = sqrt(rr[0]);
+    out.refinements = k + 1;
+    out.linear.residual = rnorm;
+    if (!isfinite(rnorm)) {
+      out.linear.status = FEMM_NUMERICAL_NONFINITE;
+      return out;
+    }
+    if (rnorm <= tol * bnorm) {
+      out.linear.status = FEMM_OK;
+      return out;
+    }
+    if (k == max_refinements || rnorm > 0.5 * prev) {
+      // Stalled in FP32 (or out of refinements): finish in FP64 from this x.
+      const LinearInfo li = Pcg64(P, vals, inv, rhs, x, r, p, ap, tol,
+                                  max(0, maxit - out.linear.iterations), true, R);
+      out.linear.status = li.status;
+      out.linear.iterations += li.iterations;
+      out.linear.residual = li.residual;
+      return out;
+    }
+    prev = rnorm;
+    // Aim a little below the remaining gap, never tighter than FP32 can deliver.
+    const double eps = fmax(inner_eps, 0.5 * tol * bnorm / rnorm);
+    const int iters = PcgInner32(P, vals32, inv32, r, rscale, d32, r32, p32, ap32, eps,
+                                 maxit - out.linear.iterations, R);
+    out.linear.iterations += iters;
+    for (int i = threadIdx.x; i < n; i += kBlockThreads)
+      x[i] += rscale * static_cast<double>(d32[i]);
+    __syncthreads();
+    if (out.linear.iterations >= maxit) {
+      out.linear.status = FEMM_LINEAR_SOLVE_NOT_CONVERGED;
+      return out;
+    }
+  }
+  out.linear.status = FEMM_LINEAR_SOLVE_NOT_CONVERGED;
+  return out;
+}
+
+// ----------------------------------------------------------------------------- Newton
+
+struct ItemPointers {
+  double *elem, *vals, *rhs, *inv, *x, *r, *p, *ap;
+  float *vals32, *r32, *p32, *ap32, *d32, *inv32;
+};
+
+__device__ ItemPointers ItemWork(const DevicePlan &P, const DeviceWork &W, int item, bool mixed)
+{
+  ItemPointers ip;
+  const size_t n = P.n, s = P.sell_size;
+  ip.elem = W.elem + static_cast<size_t>(item) * P.T * kElemStride;
+  ip.vals = W.vals + item * s;
+  ip.rhs = W.rhs + item * n;
+  ip.inv = W.inv + item * n;
+  ip.x = W.x + item * n;
+  ip.r = W.r + item * n;
+  ip.p = W.p + item * n;
+  ip.ap = W.ap + item * n;
+  ip.vals32 = mixed ? W.vals32 + item * s : nullptr;
+  ip.r32 = mixed ? W.r32 + item * n : nullptr;
+  ip.p32 = mixed ? W.p32 + item * n : nullptr;
+  ip.ap32 = mixed ? W.ap32 + item * n : nullptr;
+  ip.d32 = mixed ? W.d32 + item * n : nullptr;
+  ip.inv32 = mixed ? W.inv32 + item * n : nullptr;
+  return ip;
+}
+
+// Full device assembly for one item; returns false if any element or diagonal is invalid.
+__device__ bool AssembleItem(const DevicePlan &P, const ItemPointers &w, const double *a,
+                             const double *cur, bool newton, BlockReducer &R)
+{
+  const bool e_ok = ElementPass(P, a, w.elem, newton);
+  __syncthreads();
+  GatherValues(P, w.elem, w.vals, w.vals32);
+  __syncthreads();
+  const bool r_ok = GatherRhs(P, w.elem, w.vals, cur, w.rhs, w.inv, w.inv32);
+  return !R.AnyTrue(!(e_ok && r_ok));
+}
+
+// ||K(A) A - b(A)|| over the free rows, with A taken at each free root
+// (solvers/nonlinear_solver.cu:549-558).  Requires a fresh assembly at `a`.
+__device__ double EquationNorm(const DevicePlan &P, const ItemPointers &w, const double *a,
+                               BlockReducer &R)
+{
+  for (int i = threadIdx.x; i < P.n; i += kBlockThreads)
+    w.x[i] = a[P.free_nodes[i]];
+  __syncthreads();
+  Spmv(P, w.vals, w.x, w.ap, static_cast<const double *>(nullptr));
+  __syncthreads();
+  double s[1] = {0.0};
+  for (int i = threadIdx.x; i < P.n; i += kBlockThreads) {
+    const double v = w.ap[i] - w.rhs[i];
+    s[0] += v * v;
+  }
+  R.Sum(s);
+  return sqrt(s[0]);
+}
+
+struct NewtonArgs {
+  femm_newton_options_t options;
+  femm_solve_config_t config;
+};
+
+// Inexact-Newton inner tolerance (physics/nonlinear_model.cpp:166-174).
+__device__ double LinearTolerance(const femm_newton_options_t &o, bool adaptive, double prev)
+{
+  const double exact = o.linear_relative_tolerance;
+  if (!adaptive || !isfinite(prev) || prev <= 100.0 * o.relative_tolerance || !(exact > 0.0) ||
+      !isfinite(exact))
+    return exact;
+  return fmax(exact, 1e-8 * fmin(1.0, prev));
+}
+
+__device__ void WriteFlux(const DevicePlan &P, const double *a, double *flux, BlockReducer &R)
+{
+  for (int c = 0; c < P.C; ++c) {
+    double s[1] = {0.0};
+    for (int t = threadIdx.x; t < P.T; t += kBlockThreads) {
+      if (P.tri_circuit[t] != c)
+        continue;
+      const double *fw = P.tri_flux + 3 * static_cast<size_t>(t);
+      s[0] += fw[0] * a[P.tri_nodes[3 * t]] + fw[1] * a[P.tri_nodes[3 * t + 1]] +
+              fw[2] * a[P.tri_nodes[3 * t + 2]];
+    }
+    R.Sum(s);
+    if (threadIdx.x == 0)
+      flux[c] = P.planar ? s[0] * P.depth : s[0];
+  }
+}
+
+__global__ __launch_bounds__(kBlockThreads) void NewtonKernel(DevicePlan P, DeviceWork W,
+                                                              const double *currents,
+                                                              NewtonArgs args, const double *warm,
+                                                              double *A, double *flux_out,
+                                                              femm_item_result_t *results)
+{
+  __shared__ double lds[2 * kWaves * 4];
+  BlockReducer R{lds, 0};
+  const int item = blockIdx.x;
+  const femm_newton_options_t &o = args.options;
+  const femm_solve_config_t &cfg = args.config;
+  const bool mixed = cfg.precision == FEMM_PRECISION_MIXED;
+  const ItemPointers w = ItemWork(P, W, item, mixed);
+  double *a = A + static_cast<size_t>(item) * P.N;
+  const double *cur = currents + static_cast<size_t>(item) * P.C;
+  const bool has_warm = warm != nullptr;
+  const bool adaptive = o.adaptive_linear_tolerance && P.any_nonlinear;
+
+  femm_item_result_t res{};
+  res.status = FEMM_INVALID_ARGUMENT;
+  res.relative_step = INFINITY;
+  res.equation_residual = INFINITY;
+  res.last_linear_tolerance = NAN;
+  res.linear_residual = INFINITY;
+  bool inputs_ok = o.relative_tolerance > 0.0 && isfinite(o.relative_tolerance) &&
+                   o.max_newton_iterations >= 0;
+  for (int c = 0; c < P.C; ++c)
+    inputs_ok = inputs_ok && isfinite(cur[c]);
+
+  // Start state: prescribed values on every fixed node (signed roots expanded),
+  // zero or the warm start elsewhere.
+  for (int v = threadIdx.x; v < P.N; v += kBlockThreads)
+    a[v] = P.node_to_free[v] < 0 ? P.boundary[v]
+                                 : (has_warm ? warm[static_cast<size_t>(item) * P.N + v] : 0.0);
+  __syncthreads();
+
+  double eq_scale = 1.0;
+  int status = inputs_ok ? FEMM_NONLINEAR_SOLVE_NOT_CONVERGED : FEMM_INVALID_ARGUMENT;
+  if (inputs_ok && cfg.verify_equation_residual) {
+    // Scale of the equation residual: the norm at the Dirichlet-expanded start
+    // (nonlinear_solver.cu:565-576), independent of any warm start.  Assemble at
+    // the boundary-only state held temporarily in `a`, then restore.
+    if (has_warm) {
+      for (int v = threadIdx.x; v < P.N; v += kBlockThreads)
+        if (P.node_to_free[v] >= 0)
+          a[v] = 0.0;
+      __syncthreads();
+    }
+    if (AssembleItem(P, w, a, cur, true, R)) {
+      __syncthreads();
+      eq_scale = fmax(EquationNorm(P, w, a, R), DBL_MIN);
+    }
+    if (has_warm) {
+      for (int v = threadIdx.x; v < P.N; v += kBlockThreads)
+        if (P.node_to_free[v] >= 0)
+          a[v] = warm[static_cast<size_t>(item) * P.N + v];
+    }
+    __syncthreads();
+  }
+
+  double omega = 1.0, prev = INFINITY;
+  for (int it = 0; inputs_ok && it < o.max_newton_iterations; ++it) {
+    const bool newton = has_warm || it > 0;
+    if (!AssembleItem(P, w, a, cur, newton, R)) {
+      status = FEMM_ASSEMBLY_FAILED;
+      break;
+    }
+    __syncthreads();
+    const double tol = LinearTolerance(o, adaptive, prev);
+    if (cfg.pcg_warm_start) {
+      for (int i = threadIdx.x; i < P.n; i += kBlockThreads)
+        w.x[i] = a[P.free_nodes[i]];
+      __syncthreads();
+    }
+    LinearInfo li;
+    if (mixed) {
+      const RefinedInfo ri = RefinedSolve(
+          P, w.vals, w.inv, w.vals32, w.inv32, w.rhs, w.x, w.r, w.p, w.ap, w.d32, w.r32, w.p32,
+          w.ap32, tol, o.max_linear_iterations, cfg.pcg_warm_start != 0, cfg.max_refinements,
+          cfg.inner_relative_tolerance, R);
+      li = ri.linear;
+      res.refinements += ri.refinements;
+    } else {
+      li = Pcg64(P, w.vals, w.inv, w.rhs, w.x, w.r, w.p, w.ap, tol, o.max_linear_iterations,
+                 cfg.pcg_warm_start != 0, R);
+    }
+    __syncthreads();
+    res.linear_iterations += li.iterations;
+    res.last_linear_tolerance = tol;
+    res.linear_residual = li.residual;
+    if (li.status != FEMM_OK) {
+      status = li.status;
+      break;
+    }
+    // Undamped step ||A_cand - A|| / ||A_cand|| over all nodes (nonlinear_solver.cu:326-335).
+    double s[2] = {0.0, 0.0};
+    for (int v = threadIdx.x; v < P.N; v += kBlockThreads) {
+      const int f = P.node_to_free[v];
+      const double cand = f >= 0 ? static_cast<double>(P.node_sign[v]) * w.x[f] : P.boundary[v];
+      const double d = cand - a[v];
+      s[0] += d * d;
+      s[1] += cand * cand;
+    }
+    R.Sum(s);
+    const double step = sqrt(s[0]) / fmax(sqrt(s[1]), DBL_MIN);
+    res.newton_iterations = it + 1;
+    res.relative_step = step;
+    if (!isfinite(step)) {
+      status = FEMM_NUMERICAL_NONFINITE;
+      break;
+    }
+    // fkn-style relaxation after iteration 5 (nonlinear_solver.cu:344-351).
+    if (it > 5) {
+      if (step > prev && omega > 0.125)
+        omega *= 0.5;
+      else
+        omega += 0.1 * (1.0 - omega);
+    }
+    for (int v = threadIdx.x; v < P.N; v += kBlockThreads) {
+      const int f = P.node_to_free[v];
+      const double cand = f >= 0 ? static_cast<double>(P.node_sign[v]) * w.x[f] : P.boundary[v];
+      a[v] += omega * (cand - a[v]);
+    }
+    __syncthreads();
+    // NewtonStepConverged (nonlinear_model.cpp:186-193).
+    if (it > 0 && step < 100.0 * o.relative_tolerance && tol <= o.linear_relative_tolerance) {
+      status = FEMM_OK;
+      break;
+    }
+    prev = step;
+  }
+  if (inputs_ok && cfg.verify_equation_residual &&
+      (status == FEMM_OK || status == FEMM_NONLINEAR_SOLVE_NOT_CONVERGED)) {
+    if (AssembleItem(P, w, a, cur, true, R)) {
+      __syncthreads();
+      res.equation_residual = EquationNorm(P, w, a, R) / eq_scale;
+    } else {
+      res.equation_residual = INFINITY;
+    }
+  }
+  WriteFlux(P, a, flux_out + static_cast<size_t>(item) * P.C, R);
+  res.status = status;
+  if (threadIdx.x == 0)
+    results[item] = res;
+}
+
+__global__ __launch_bounds__(kBlockThreads) void AssembleKernel(DevicePlan P, DeviceWork W,
+                                                                const double *states,
+                                                                const double *currents,
+                                                                int newton, int32_t *valid)
+{
+  __shared__ double lds[2 * kWaves * 4];
+  BlockReducer R{lds, 0};
+  const int item = blockIdx.x;
+  const ItemPointers w = ItemWork(P, W, item, false);
+  const bool ok = AssembleItem(P, w, states + static_cast<size_t>(item) * P.N,
+                               currents + static_cast<size_t>(item) * P.C, newton != 0, R);
+  if (threadIdx.x == 0)
+    valid[item] = ok ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kBlockThreads) void PcgKernel(DevicePlan P, DeviceWork W, double tol,
+                                                           int maxit, double *info)
+{
+  __shared__ double lds[2 * kWaves * 4];
+  BlockReducer R{lds, 0};
+  const int item = blockIdx.x;
+  const ItemPointers w = ItemWork(P, W, item, false);
+  for (int row = threadIdx.x; row < P.n; row += kBlockThreads) {
+    const double d = w.vals[P.diag_sell[row]];
+    w.inv[row] = (isfinite(d) && d > 0.0) ? 1.0 / d : 0.0;
+  }
+  __syncthreads();
+  const LinearInfo li = Pcg64(P, w.vals, w.inv, w.rhs, w.x, w.r, w.p, w.ap, tol, maxit, false, R);
+  if (threadIdx.x == 0) {
+    info[3 * item] = li.status;
+    info[3 * item + 1] = li.iterations;
+    info[3 * item + 2] = li.residual;
+  }
+}
+
+// Element B (centroid basis) and flux per item (nonlinear_solver.cu:16-83).
+__global__ void ElementBKernel(DevicePlan P, const double *states, int items, double *bx,
+                               double *by)
+{
+  const size_t gid = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t total = static_cast<size_t>(items) * P.T;
+  if (gid >= total)
+    return;
+  const int item = static_cast<int>(gid / P.T), t = static_cast<int>(gid % P.T);
+  const double *a = states + static_cast<size_t>(item) * P.N;
+  const size_t g = static_cast<size_t>(t) * P.Q * 3;   // quadrature point 0 = centroid
+  double x = 0.0, y = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    const double av = a[P.tri_nodes[3 * t + k]];
+    x += av * P.tri_gx[g + k];
+    y += av * P.tri_gy[g + k];
+  }
+  bx[gid] = x;
+  by[gid] = y;
+}
+
+__global__ __launch_bounds__(kBlockThreads) void FluxKernel(DevicePlan P, const double *states,
+                                                            double *flux)
+{
+  __shared__ double lds[2 * kWaves * 4];
+  BlockReducer R{lds, 0};
+  const int item = blockIdx.x;
+  WriteFlux(P, states + static_cast<size_t>(item) * P.N, flux + static_cast<size_t>(item) * P.C, R);
+}
+
+template <typename T>
+T *Carve(char *&cursor, size_t count)
+{
+  T *p = reinterpret_cast<T *>(cursor);
+  cursor += (count * sizeof(T) + 255) / 256 * 256;
+  return p;
+}
+
+template <typename T>
+size_t Bytes(size_t count)
+{
+  return (count * sizeof(T) + 255) / 256 * 256;
+}
+
+}  // namespace
+
+int UploadPlan(const HostPlan &h, DeviceBuffers *dev)
+{
+  size_t total = 0;
+  auto add = [&total](size_t bytes) { total += (bytes + 255) / 256 * 256; };
+  add(h.tri_nodes.size() * 4);
+  add(h.tri_mat.size() * 4);
+  add(h.tri_circuit.size() * 4);
+  add(h.tri_w.size() * 8);
+  add(h.tri_gx.size() * 8);
+  add(h.tri_gy.size() * 8);
+  add(h.tri_flux.size() * 8);
+  add(h.mat_nu0.size() * 8 * 3);
+  add(h.mat_curve_off.size() * 4 * 2);
+  add(h.curve_b.size() * 8 * 3);
+  add(h.node_to_free.size() * 4);
+  add(h.free_nodes.size() * 4);
+  add(h.node_sign.size());
+  add(h.boundary.size() * 8);
+  add(h.slot_off.size() * 4);
+  add(h.contrib.size() * 4);
+  add(h.row_off.size() * 4);
+  add(h.rcontrib.size() * 4);
+  add(h.pm_rhs.size() * 8);
+  add(h.circuit_rhs.size() * 8);
+  add(h.slice_ptr.size() * 4);
+  add(h.slice_width.size() * 4);
+  add(h.sell_cols.size() * 4);
+  add(h.slot_to_sell.size() * 4);
+  add(h.diag_sell.size() * 4);
+  void *base = nullptr;
+  if (hipMalloc(&base, total) != hipSuccess)
+    return FEMM_GPU_ALLOCATION_FAILED;
+  dev->base = base;
+  char *cur = static_cast<char *>(base);
+  bool copy_ok = true;
+  auto up = [&cur, &copy_ok](const auto &vec) {
+    using T = typename std::decay_t<decltype(vec)>::value_type;
+    T *p = Carve<T>(cur, vec.size());
+    if (!vec.empty() &&
+        hipMemcpy(p, vec.data(), vec.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+      copy_ok = false;
+    return static_cast<const T *>(p);
+  };
+  DevicePlan &P = dev->plan;
+  P.N = h.N;
+  P.T = h.T;
+  P.n = h.n;
+  P.nnz = h.nnz;
+  P.C = h.C;
+  P.Q = h.Q;
+  P.nslices = h.nslices;
+  P.sell_size = h.sell_size;
+  P.planar = h.planar;
+  P.any_nonlinear = h.any_nonlinear ? 1 : 0;
+  P.any_boundary_value = h.any_boundary_value ? 1 : 0;
+  P.depth = h.depth;
+  P.tri_nodes = up(h.tri_nodes);
+  P.tri_mat = up(h.tri_mat);
+  P.tri_circuit = up(h.tri_circuit);
+  P.tri_w = up(h.tri_w);
+  P.tri_gx = up(h.tri_gx);
+  P.tri_gy = up(h.tri_gy);
+  P.tri_flux = up(h.tri_flux);
+  std::vector<double> mats;
+  P.mat_nu0 = up(h.mat_nu0);
+  P.mat_alpha = up(h.mat_alpha);
+  P.mat_cold = up(h.mat_cold);
+  P.mat_curve_off = up(h.mat_curve_off);
+  P.mat_curve_cnt = up(h.mat_curve_cnt);
+  P.curve_b = up(h.curve_b);
+  P.curve_h = up(h.curve_h);
+  P.curve_s = up(h.curve_s);
+  P.node_to_free = up(h.node_to_free);
+  P.free_nodes = up(h.free_nodes);
+  P.node_sign = up(h.node_sign);
+  P.boundary = up(h.boundary);
+  P.slot_off = up(h.slot_off);
+  P.contrib = up(h.contrib);
+  P.row_off = up(h.row_off);
+  P.rcontrib = up(h.rcontrib);
+  P.pm_rhs = up(h.pm_rhs);
+  P.circuit_rhs = up(h.circuit_rhs);
+  P.slice_ptr = up(h.slice_ptr);
+  P.slice_width = up(h.slice_width);
+  P.sell_cols = up(h.sell_cols);
+  P.slot_to_sell = up(h.slot_to_sell);
+  P.diag_sell = up(h.diag_sell);
+  if (static_cast<size_t>(cur - static_cast<char *>(base)) > total || !copy_ok)
+    return FEMM_INTERNAL_ERROR;
+  return hipDeviceSynchronize() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
+}
+
+int EnsureWork(const HostPlan &h, int32_t items, bool mixed, DeviceBuffers *dev)
+{
+  const bool have_mixed = dev->work.vals32 != nullptr;
+  if (dev->work_base && dev->work.capacity >= items && (!mixed || have_mixed))
+    return FEMM_OK;
+  const int32_t cap = std::max(items, dev->work.capacity);
+  const bool want_mixed = mixed || have_mixed;
+  if (dev->work_base) {
+    if (hipDeviceSynchronize() != hipSuccess)
+      return FEMM_INTERNAL_ERROR;
+    (void)hipFree(dev->work_base);
+    dev->work_base = nullptr;
+  }
+  const size_t ci = static_cast<size_t>(cap);
+  size_t bytes = Bytes<double>(ci * h.T * kElemStride) + Bytes<double>(ci * h.sell_size) +
+                 6 * Bytes<double>(ci * h.n);
+  if (want_mixed)
+    bytes += Bytes<float>(ci * h.sell_size) + 5 * Bytes<float>(ci * h.n);
+  void *base = nullptr;
+  if (hipMalloc(&base, bytes) != hipSuccess)
+    return FEMM_GPU_ALLOCATION_FAILED;
+  // SELL padding entries must read as zero; they are never written afterwards.
+  if (hipMemset(base, 0, bytes) != hipSuccess) {
+    (void)hipFree(base);
+    return FEMM_INTERNAL_ERROR;
+  }
+  char *cur = static_cast<char *>(base);
+  DeviceWork &w = dev->work;
+  w = DeviceWork{};
+  w.capacity = cap;
+  w.elem = Carve<double>(cur, ci * h.T * kElemStride);
+  w.vals = Carve<double>(cur, ci * h.sell_size);
+  w.rhs = Carve<double>(cur, ci * h.n);
+  w.inv = Carve<double>(cur, ci * h.n);
+  w.x = Carve<double>(cur, ci * h.n);
+  w.r = Carve<double>(cur, ci * h.n);
+  w.p = Carve<double>(cur, ci * h.n);
+  w.ap = Carve<double>(cur, ci * h.n);
+  if (want_mixed) {
+    w.vals32 = Carve<float>(cur, ci * h.sell_size);
+    w.r32 = Carve<float>(cur, ci * h.n);
+    w.p32 = Carve<float>(cur, ci * h.n);
+    w.ap32 = Carve<float>(cur, ci * h.n);
+    w.d32 = Carve<float>(cur, ci * h.n);
+    w.inv32 = Carve<float>(cur, ci * h.n);
+  }
+  dev->work_base = base;
+  dev->work_bytes = bytes;
+  return hipDeviceSynchronize() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
+}
+
+void ReleaseDevice(DeviceBuffers *dev)
+{
+  if (dev->work_base)
+    (void)hipFree(dev->work_base);
+  if (dev->base)
+    (void)hipFree(dev->base);
+  dev->work_base = nullptr;
+  dev->base = nullptr;
+}
+
+int LaunchNewton(const DeviceBuffers &dev, int32_t items, const double *currents,
+                 const femm_newton_options_t &options, const femm_solve_config_t &config,
+                 const double *warm, double *a, double *flux, femm_item_result_t *results,
+                 hipStream_t stream)
+{
+  if (items <= 0 || items > dev.work.capacity)
+    return FEMM_INVALID_ARGUMENT;
+  NewtonArgs args{options, config};
+  hipLaunchKernelGGL(NewtonKernel, dim3(items), dim3(kBlockThreads), 0, stream, dev.plan,
+                     dev.work, currents, args, warm, a, flux, results);
+  return hipGetLastError() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
+}
+
+int LaunchAssemble(const DeviceBuffers &dev, int32_t items, const double *states,
+                   const double *currents, int32_t newton, int32_t *valid, hipStream_t stream)
+{
+  if (items <= 0 || items > dev.work.capacity)
+    return FEMM_INVALID_ARGUMENT;
+  hipLaunchKernelGGL(AssembleKernel, dim3(items), dim3(kBlockThreads), 0, stream, dev.plan,
+                     dev.work, states, currents, newton, valid);
+  return hipGetLastError() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
+}
+
+int LaunchPcg(const DeviceBuffers &dev, int32_t items, double tol, int32_t maxit, double *info,
+              hipStream_t stream)
+{
+  if (items <= 0 || items > dev.work.capacity)
+    return FEMM_INVALID_ARGUMENT;
+  hipLaunchKernelGGL(PcgKernel, dim3(items), dim3(kBlockThreads), 0, stream, dev.plan, dev.work,
+                     tol, maxit, info);
+  return hipGetLastError() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
+}
+
+int LaunchFinalize(const DeviceBuffers &dev, int32_t items, const double *states, double *bx,
+                   double *by, double *flux, hipStream_t stream)
+{
+  if (items <= 0)
+    return FEMM_INVALID_ARGUMENT;
+  if (bx && by) {
+    const size_t total = static_cast<size_t>(items) * dev.plan.T;
+    hipLaunchKernelGGL(ElementBKernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256),
+                       0, stream, dev.plan, states, items, bx, by);
+  }
+  if (flux)
+    hipLaunchKernelGGL(FluxKernel, dim3(items), dim3(kBlockThreads), 0, stream, dev.plan, states,
+                       flux);
+  return hipGetLastError() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
+}
+
+}  // namespace femm

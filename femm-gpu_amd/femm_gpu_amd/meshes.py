@@ -34,6 +34,12 @@ def tanh_bh_curve(ms=1.9, a=300.0, h_max=2e5, points=40):
     return b, h
 
 
+def _area(xy, tri):
+    p = xy[tri]
+    return 0.5 * np.abs((p[:, 1, 0] - p[:, 0, 0]) * (p[:, 2, 1] - p[:, 0, 1])
+                        - (p[:, 2, 0] - p[:, 0, 0]) * (p[:, 1, 1] - p[:, 0, 1]))
+
+
 def _ccw(xy, tri):
     p = xy[tri]
     det = ((p[:, 1, 0] - p[:, 0, 0]) * (p[:, 2, 1] - p[:, 0, 1])
@@ -77,7 +83,7 @@ def square_coil_ring(cells=64, seed=12044, side=0.2, coil_side=0.04, ring_in=0.0
     coil = (np.abs(c[:, 0]) <= coil_side / 2) & (np.abs(c[:, 1]) <= coil_side / 2)
     ring = (r >= ring_in) & (r <= ring_out) & ~coil
     label = np.where(coil, 1, np.where(ring, 2, 0)).astype(np.int32)
-    area = 0.5 * np.abs(np.cross(xy[tri[:, 1]] - xy[tri[:, 0]], xy[tri[:, 2]] - xy[tri[:, 0]]))
+    area = _area(xy, tri)
     coil_area = float(area[coil].sum())
     mats = [Material(1.0 / MU0, group=0),
             Material(1.0 / MU0, source_j_per_a=coil_area / coil_area, circuit=0, group=1),
@@ -174,7 +180,7 @@ def motor_sector(angular_cells=480, seed=20251203, ms=1.9, hc=(9e5, 9e5), fill=(
             tlabel += [label[i, j]] * 2
     tri = _ccw(xy, np.asarray(tri, np.int32))
     tlabel = np.asarray(tlabel, np.int32)
-    area = 0.5 * np.abs(np.cross(xy[tri[:, 1]] - xy[tri[:, 0]], xy[tri[:, 2]] - xy[tri[:, 0]]))
+    area = _area(xy, tri)
     b, h = tanh_bh_curve(ms=ms)
     mats = [Material(1.0 / MU0, group=0),
             Material(1.0, bh_curve=0, group=1),
@@ -261,9 +267,7 @@ def c_core(edge_iron=0.5e-3, edge_air=2e-3, seed=2, scale=1.0):
     xy = np.unique(np.round(xy / (1e-6 * s)) * (1e-6 * s), axis=0)
     tri = Delaunay(xy).simplices.astype(np.int32)
     tri = _ccw(xy, tri)
-    p = xy[tri]
-    area = 0.5 * np.abs(np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]))
-    tri = tri[area > 1e-6 * edge_iron ** 2]
+    tri = tri[_area(xy, tri) > 1e-6 * edge_iron ** 2]
     c = xy[tri].mean(axis=1)
     AIR, IRON, MAG, COIL_P, COIL_N = range(5)
     label = np.full(len(tri), AIR, np.int32)
@@ -275,7 +279,7 @@ def c_core(edge_iron=0.5e-3, edge_air=2e-3, seed=2, scale=1.0):
     cn = (c[:, 0] > -20 * s) & (c[:, 0] < 20 * s) & (c[:, 1] > 72 * s) & (c[:, 1] < 82 * s)
     label[cp] = COIL_P
     label[cn] = COIL_N
-    area = 0.5 * np.abs(np.cross(xy[tri[:, 1]] - xy[tri[:, 0]], xy[tri[:, 2]] - xy[tri[:, 0]]))
+    area = _area(xy, tri)
     bcurve, hcurve = tanh_bh_curve()
     mats = [Material(1.0 / MU0, group=0), Material(1.0, bh_curve=0, group=1),
             Material(1.0 / (MU0 * 1.05), hc=9e5, magnetization_deg=0.0, group=2)]

@@ -1,0 +1,204 @@
+"""ctypes binding of the C ABI (include/femm_gpu_amd.h).
+
+Mirrors the reference's C++ solver interface for this path
+(``NonlinearP1Solver`` in gpu_solver/src/solvers/nonlinear_solver.cuh:32-85):
+``Plan(model)`` is Initialize + device plan, ``Plan.solve_batch`` is SolveBatch
+(+ FinalizeResult), ``Plan.assemble_batch`` / ``Plan.pcg_batch`` expose the device
+assembly and PCG of DeviceNonlinearAssemblyPlan / GpuCsrSolver for tests.
+
+The native library is required: if it is missing or was built without a GPU
+target, loading fails loudly.  There is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+from .model import CModel, CNewtonOptions, Model, production_options, status_name
+
+LIB_PATH = Path(__file__).resolve().parents[1] / "lib" / "libfemm_gpu_amd.so"
+
+P = ctypes.POINTER
+c_d, c_i32 = ctypes.c_double, ctypes.c_int32
+
+FP64 = 0
+MIXED = 1
+
+
+class SolveConfig(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int32), ("pcg_warm_start", ctypes.c_int32),
+                ("verify_equation_residual", ctypes.c_int32), ("max_refinements", ctypes.c_int32),
+                ("inner_relative_tolerance", ctypes.c_double)]
+
+
+class ItemResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("newton_iterations", ctypes.c_int32),
+                ("linear_iterations", ctypes.c_int32), ("refinements", ctypes.c_int32),
+                ("relative_step", ctypes.c_double), ("equation_residual", ctypes.c_double),
+                ("last_linear_tolerance", ctypes.c_double), ("linear_residual", ctypes.c_double)]
+
+
+RESULT_DTYPE = np.dtype([("status", np.int32), ("newton_iterations", np.int32),
+                         ("linear_iterations", np.int32), ("refinements", np.int32),
+                         ("relative_step", np.float64), ("equation_residual", np.float64),
+                         ("last_linear_tolerance", np.float64), ("linear_residual", np.float64)])
+
+
+def reference_config(**kw) -> SolveConfig:
+    """The reference algorithm: FP64 Jacobi PCG from x0 = 0 in every Newton correction."""
+    values = dict(precision=FP64, pcg_warm_start=0, verify_equation_residual=1,
+                  max_refinements=64, inner_relative_tolerance=1e-4)
+    values.update(kw)
+    return SolveConfig(**values)
+
+
+def mixed_config(**kw) -> SolveConfig:
+    """FP32 Jacobi PCG inside FP64 iterative refinement, starting from the current state."""
+    values = dict(precision=MIXED, pcg_warm_start=1, verify_equation_residual=1,
+                  max_refinements=64, inner_relative_tolerance=1e-4)
+    values.update(kw)
+    return SolveConfig(**values)
+
+
+_lib = None
+
+
+def library():
+    """Load libfemm_gpu_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"native library missing: {LIB_PATH} (run __graft_entry__.build())")
+        lib = ctypes.CDLL(str(LIB_PATH))
+        lib.femm_status_name.restype = ctypes.c_char_p
+        _lib = lib
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(P(c_d))
+
+
+def _ip(a):
+    return a.ctypes.data_as(P(c_i32))
+
+
+class FemmError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what}: {status_name(code)} ({code})")
+        self.code = code
+
+
+def _check(code, what):
+    if code != 0:
+        raise FemmError(code, what)
+
+
+def device_info() -> str:
+    buf = ctypes.create_string_buffer(256)
+    library().femm_device_info(buf, 256)
+    return buf.value.decode()
+
+
+class Plan:
+    """One geometry uploaded to the GPU (reference NonlinearP1Solver::Initialize)."""
+
+    def __init__(self, model: Model):
+        self.model = model
+        self._handle = ctypes.c_void_p()
+        self._lib = library()
+        _check(self._lib.femm_plan_create(ctypes.byref(model.c_struct()),
+                                          ctypes.byref(self._handle)), "femm_plan_create")
+        sizes = np.zeros(6, np.int32)
+        _check(self._lib.femm_plan_sizes(self._handle, _ip(sizes)), "femm_plan_sizes")
+        self.N, self.T, self.n, self.nnz, self.C, self.Q = (int(s) for s in sizes)
+
+    def close(self):
+        if self._handle:
+            self._lib.femm_plan_destroy(self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._handle
+
+    def symbolic(self):
+        out = dict(rows=np.zeros(self.n + 1, np.int32), cols=np.zeros(self.nnz, np.int32),
+                   diag_slots=np.zeros(self.n, np.int32), free_nodes=np.zeros(self.n, np.int32),
+                   node_to_free=np.zeros(self.N, np.int32), node_sign=np.zeros(self.N, np.int8),
+                   boundary=np.zeros(self.N))
+        _check(self._lib.femm_plan_copy_symbolic(
+            self._handle, _ip(out["rows"]), _ip(out["cols"]), _ip(out["diag_slots"]),
+            _ip(out["free_nodes"]), _ip(out["node_to_free"]),
+            out["node_sign"].ctypes.data_as(P(ctypes.c_int8)), _dp(out["boundary"])),
+            "femm_plan_copy_symbolic")
+        return out
+
+    def assemble_batch(self, states, currents, newton):
+        states = np.ascontiguousarray(states, np.float64).reshape(-1, self.N)
+        B = states.shape[0]
+        currents = np.ascontiguousarray(currents, np.float64).reshape(B, self.C)
+        values, rhs, diag = np.zeros((B, self.nnz)), np.zeros((B, self.n)), np.zeros((B, self.n))
+        code = self._lib.femm_assemble_batch(self._handle, B, _dp(states), _dp(currents),
+                                             int(newton), _dp(values), _dp(rhs), _dp(diag))
+        return code, values, rhs, diag
+
+    def pcg_batch(self, values, rhs, tol, maxit):
+        values = np.ascontiguousarray(values, np.float64).reshape(-1, self.nnz)
+        B = values.shape[0]
+        rhs = np.ascontiguousarray(rhs, np.float64).reshape(B, self.n)
+        x, info = np.zeros((B, self.n)), np.zeros((B, 3))
+        _check(self._lib.femm_pcg_batch(self._handle, B, _dp(values), _dp(rhs), c_d(tol),
+                                        int(maxit), _dp(x), _dp(info)), "femm_pcg_batch")
+        return x, info
+
+    def solve_batch(self, currents, options: CNewtonOptions = None, config: SolveConfig = None,
+                    warm_starts=None, fields=False):
+        """Batched Newton solve (SolveBatch + FinalizeResult).  Returns a dict of arrays."""
+        currents = np.ascontiguousarray(currents, np.float64).reshape(-1, self.C)
+        B = currents.shape[0]
+        options = options or production_options()
+        config = config or reference_config()
+        a, flux = np.zeros((B, self.N)), np.zeros((B, self.C))
+        bx = np.zeros((B, self.T)) if fields else None
+        by = np.zeros((B, self.T)) if fields else None
+        results = np.zeros(B, RESULT_DTYPE)
+        warm = None if warm_starts is None else np.ascontiguousarray(warm_starts, np.float64)
+        _check(self._lib.femm_newton_batch(
+            self._handle, B, _dp(currents), ctypes.byref(options), ctypes.byref(config),
+            None if warm is None else _dp(warm), _dp(a), None if bx is None else _dp(bx),
+            None if by is None else _dp(by), _dp(flux),
+            results.ctypes.data_as(ctypes.c_void_p)), "femm_newton_batch")
+        out = dict(a=a, flux=flux, results=results)
+        if fields:
+            out.update(bx=bx, by=by)
+        return out
+
+    def solve_batch_device(self, currents_ptr, items, a_ptr, flux_ptr, results_ptr, stream_ptr,
+                           options: CNewtonOptions = None, config: SolveConfig = None,
+                           warm_ptr=None):
+        """Device-pointer entry (femm_newton_batch_device); nothing is synchronised."""
+        options = options or production_options()
+        config = config or reference_config()
+        _check(self._lib.femm_newton_batch_device(
+            self._handle, int(items), ctypes.c_void_p(currents_ptr), ctypes.byref(options),
+            ctypes.byref(config), None if warm_ptr is None else ctypes.c_void_p(warm_ptr),
+            ctypes.c_void_p(a_ptr), ctypes.c_void_p(flux_ptr), ctypes.c_void_p(results_ptr),
+            ctypes.c_void_p(stream_ptr)), "femm_newton_batch_device")
+
+    def finalize_batch(self, states):
+        states = np.ascontiguousarray(states, np.float64).reshape(-1, self.N)
+        B = states.shape[0]
+        bx, by, flux = np.zeros((B, self.T)), np.zeros((B, self.T)), np.zeros((B, self.C))
+        _check(self._lib.femm_finalize_batch(self._handle, B, _dp(states), _dp(bx), _dp(by),
+                                             _dp(flux)), "femm_finalize_batch")
+        return bx, by, flux
