@@ -12,10 +12,25 @@
 #include "device.hpp"
 #include "femm_gpu_amd.h"
 #include "plan.hpp"
+#include "post.hpp"
+
+// Retained copy of the model arrays the postprocess plan needs.
+struct ModelCopy {
+  std::vector<double> node_xy, dirichlet_values;
+  std::vector<int32_t> tri_nodes, tri_mat, dirichlet_nodes, curve_offsets{0};
+  std::vector<femm_material_t> materials;
+  std::vector<femm_node_constraint_t> constraints;
+  femm_model_t view{};
+};
 
 struct femm_plan {
   femm::HostPlan host;
   femm::DeviceBuffers dev;
+  ModelCopy model;
+  femm::HostPostPlan post;
+  femm::DevicePostPlan post_dev{};
+  void *post_base = nullptr;
+  bool post_ready = false;
   // staging for the host-pointer entry points
   void *io = nullptr;
   size_t io_bytes = 0;
@@ -107,6 +122,28 @@ int femm_plan_create(const femm_model_t *model, femm_plan **out)
     delete plan;
     return st;
   }
+  ModelCopy &mc = plan->model;
+  const size_t N = static_cast<size_t>(model->node_count), T = static_cast<size_t>(model->triangle_count);
+  mc.node_xy.assign(model->node_xy, model->node_xy + 2 * N);
+  mc.tri_nodes.assign(model->triangle_nodes, model->triangle_nodes + 3 * T);
+  mc.tri_mat.assign(model->triangle_material, model->triangle_material + T);
+  mc.materials.assign(model->materials, model->materials + model->material_count);
+  mc.dirichlet_nodes.assign(model->dirichlet_nodes, model->dirichlet_nodes + model->dirichlet_count);
+  mc.dirichlet_values.assign(model->dirichlet_values,
+                             model->dirichlet_values + model->dirichlet_count);
+  mc.constraints.assign(model->constraints, model->constraints + model->constraint_count);
+  mc.view = *model;
+  mc.view.node_xy = mc.node_xy.data();
+  mc.view.triangle_nodes = mc.tri_nodes.data();
+  mc.view.triangle_material = mc.tri_mat.data();
+  mc.view.materials = mc.materials.data();
+  mc.view.dirichlet_nodes = mc.dirichlet_nodes.data();
+  mc.view.dirichlet_values = mc.dirichlet_values.data();
+  mc.view.constraints = mc.constraints.data();
+  mc.view.curve_count = 0;
+  mc.view.curve_offsets = mc.curve_offsets.data();
+  mc.view.curve_b = nullptr;
+  mc.view.curve_h = nullptr;
   // The device copy is uploaded by the first call that computes, so the
   // symbolic plan is available on hosts without a GPU.
   *out = plan;
@@ -122,6 +159,8 @@ void femm_plan_destroy(femm_plan *plan)
   femm::ReleaseDevice(&plan->dev);
   if (plan->io)
     (void)hipFree(plan->io);
+  if (plan->post_base)
+    (void)hipFree(plan->post_base);
   delete plan;
 }
 
@@ -357,6 +396,75 @@ int femm_finalize_batch(femm_plan *plan, int32_t items, const double *states, do
       (by && hipMemcpy(by, d_by, sizeof(double) * B * h.T, hipMemcpyDeviceToHost) != hipSuccess) ||
       (flux && hipMemcpy(flux, d_flux, sizeof(double) * B * h.C, hipMemcpyDeviceToHost) !=
                    hipSuccess))
+    return FEMM_INTERNAL_ERROR;
+  return FEMM_OK;
+}
+
+int femm_plan_set_postprocess(femm_plan *plan, const femm_postprocess_options_t *options)
+{
+  if (plan == nullptr || options == nullptr)
+    return FEMM_INVALID_ARGUMENT;
+  int st = EnsureDevice(plan);
+  if (st != FEMM_OK)
+    return st;
+  if (plan->post_base) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(plan->post_base);
+    plan->post_base = nullptr;
+  }
+  plan->post_ready = false;
+  st = femm::BuildPostPlan(&plan->model.view, plan->host, *options, &plan->post);
+  if (st != FEMM_OK)
+    return st;
+  st = femm::UploadPostPlan(plan->post, plan->dev, &plan->post_dev, &plan->post_base);
+  if (st != FEMM_OK)
+    return st;
+  plan->post_ready = true;
+  return FEMM_OK;
+}
+
+int femm_postprocess_copy_mask(const femm_plan *plan, double *mask)
+{
+  if (plan == nullptr || mask == nullptr || !plan->post_ready)
+    return FEMM_INVALID_ARGUMENT;
+  std::copy(plan->post.mask.begin(), plan->post.mask.end(), mask);
+  return FEMM_OK;
+}
+
+int femm_postprocess_batch_device(femm_plan *plan, int32_t items, const double *states,
+                                  double *loads, double *radial_b, void *stream)
+{
+  if (plan == nullptr || items <= 0 || states == nullptr || loads == nullptr ||
+      (radial_b == nullptr && plan->post_dev.nprobe > 0) || !plan->post_ready)
+    return FEMM_INVALID_ARGUMENT;
+  return femm::LaunchPostprocess(plan->post_dev, items, states, loads, radial_b,
+                                 static_cast<hipStream_t>(stream));
+}
+
+int femm_postprocess_batch(femm_plan *plan, int32_t items, const double *states, double *loads,
+                           double *radial_b)
+{
+  if (plan == nullptr || items <= 0 || states == nullptr || loads == nullptr || !plan->post_ready)
+    return FEMM_INVALID_ARGUMENT;
+  const femm::HostPlan &h = plan->host;
+  const size_t B = static_cast<size_t>(items), P = static_cast<size_t>(plan->post_dev.nprobe);
+  const size_t sa = Round(sizeof(double) * B * h.N), sl = Round(sizeof(double) * B * 3);
+  const size_t sp = Round(sizeof(double) * B * std::max<size_t>(P, 1));
+  int st = EnsureIo(plan, sa + sl + sp);
+  if (st != FEMM_OK)
+    return st;
+  char *io = static_cast<char *>(plan->io);
+  double *d_a = reinterpret_cast<double *>(io);
+  double *d_l = reinterpret_cast<double *>(io + sa);
+  double *d_p = reinterpret_cast<double *>(io + sa + sl);
+  if (hipMemcpy(d_a, states, sizeof(double) * B * h.N, hipMemcpyHostToDevice) != hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  if ((st = femm::LaunchPostprocess(plan->post_dev, items, d_a, d_l, d_p, nullptr)) != FEMM_OK)
+    return st;
+  if (hipDeviceSynchronize() != hipSuccess ||
+      hipMemcpy(loads, d_l, sizeof(double) * B * 3, hipMemcpyDeviceToHost) != hipSuccess ||
+      (P && radial_b &&
+       hipMemcpy(radial_b, d_p, sizeof(double) * B * P, hipMemcpyDeviceToHost) != hipSuccess))
     return FEMM_INTERNAL_ERROR;
   return FEMM_OK;
 }

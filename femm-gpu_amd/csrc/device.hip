@@ -13,6 +13,7 @@
 // element geometry are shared by all items and stay resident in L2 / MALL.
 
 #include "device.hpp"
+#include "post.hpp"
 
 #include <algorithm>
 #include <cfloat>
@@ -917,6 +918,187 @@ __global__ __launch_bounds__(kBlockThreads) void FluxKernel(DevicePlan P, const 
   WriteFlux(P, states + static_cast<size_t>(item) * P.N, flux + static_cast<size_t>(item) * P.C, R);
 }
 
+// ----------------------------------------------------------------------------- postprocess
+
+__device__ void ElementB(const DevicePostPlan &P, const double *a, int e, double *bx, double *by)
+{
+  const size_t g = static_cast<size_t>(e) * P.Q * 3;   // quadrature point 0 = centroid basis
+  double x = 0.0, y = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    const double av = a[P.tri_nodes[3 * e + k]];
+    x += av * P.tri_gx[g + k];
+    y += av * P.tri_gy[g + k];
+  }
+  *bx = x;
+  *by = y;
+}
+
+// FEMM GetNodalB subset of the reference (postprocess/field_postprocess.cpp:34-190):
+// inverse-centroid-distance average in a single-label patch, otherwise the
+// CCW / CW traversal to the label interface keeping Bn from A and Bt from the
+// source-side element, with the sharp-corner magnitude fallback.
+__device__ bool SmoothedNodalB(const DevicePostPlan &P, const double *a, int el, double nbx[3],
+                               double nby[3])
+{
+  const int own_mat = P.tri_mat[el];
+  for (int loc = 0; loc < 3; ++loc) {
+    const int node = P.tri_nodes[3 * el + loc];
+    const int i0 = P.inc_off[node], i1 = P.inc_off[node + 1];
+    const double nx = P.node_xy[2 * node], ny = P.node_xy[2 * node + 1];
+    bool homo = true;
+    for (int k = i0; k < i1; ++k)
+      homo = homo && own_mat == P.tri_mat[P.inc_elem[k]];
+    if (homo) {
+      double ws = 0.0, sx = 0.0, sy = 0.0;
+      for (int k = i0; k < i1; ++k) {
+        const int nb = P.inc_elem[k];
+        const int *t = P.tri_nodes + 3 * nb;
+        const double cx = (P.node_xy[2 * t[0]] + P.node_xy[2 * t[1]] + P.node_xy[2 * t[2]]) / 3.0;
+        const double cy =
+            (P.node_xy[2 * t[0] + 1] + P.node_xy[2 * t[1] + 1] + P.node_xy[2 * t[2] + 1]) / 3.0;
+        const double w = 1.0 / hypot(nx - cx, ny - cy);
+        if (!(w > 0.0) || !isfinite(w))
+          return false;
+        double ex, ey;
+        ElementB(P, a, nb, &ex, &ey);
+        ws += w;
+        sx += w * ex;
+        sy += w * ey;
+      }
+      nbx[loc] = sx / ws;
+      nby[loc] = sy / ws;
+      continue;
+    }
+    double ws = 0.0, sx = 0.0, sy = 0.0, v1x = 0.0, v1y = 0.0, v2x = 0.0, v2y = 0.0;
+    bool special = false;
+    for (int pass = 0; pass < 2 && !special; ++pass) {
+      int cur = el;
+      for (int scan = i0; scan < i1; ++scan) {
+        int idx = -1;
+        for (int q = 0; q < 3; ++q)
+          if (P.tri_nodes[3 * cur + q] == node)
+            idx = q;
+        if (idx < 0)
+          return false;
+        const int other = P.tri_nodes[3 * cur + (idx + (pass == 0 ? 2 : 1)) % 3];
+        int next = -1;
+        for (int k = i0; k < i1 && next < 0; ++k) {
+          const int cand = P.inc_elem[k];
+          if (cand == cur)
+            continue;
+          for (int q = 0; q < 3; ++q)
+            if (P.tri_nodes[3 * cand + q] == other)
+              next = cand;
+        }
+        if (next < 0) {
+          double ex, ey;
+          ElementB(P, a, cur, &ex, &ey);
+          sx = ex;
+          sy = ey;
+          special = true;
+          break;
+        }
+        if (own_mat != P.tri_mat[next]) {
+          const double dx = P.node_xy[2 * other] - nx, dy = P.node_xy[2 * other + 1] - ny;
+          const double len = hypot(dx, dy);
+          if (!(len > 0.0))
+            return false;
+          const double tx = dx / len, ty = dy / len;
+          double ex, ey;
+          ElementB(P, a, cur, &ex, &ey);
+          const double bt = ex * tx + ey * ty;
+          const double bn = (a[other] - a[node]) / len;
+          const double w = 0.5 / len;
+          sx += w * (tx * bt + ty * bn);
+          sy += w * (ty * bt - tx * bn);
+          ws += w;
+          if (pass == 0) {
+            v1x = tx;
+            v1y = ty;
+          } else {
+            v2x = tx;
+            v2y = ty;
+          }
+          break;
+        }
+        cur = next;
+      }
+    }
+    if (special) {
+      nbx[loc] = sx;
+      nby[loc] = sy;
+      continue;
+    }
+    if (!(ws > 0.0))
+      return false;
+    nbx[loc] = sx / ws;
+    nby[loc] = sy / ws;
+    const bool simple = hypot(v1x, v1y) < 0.9 || hypot(v2x, v2y) < 0.9 ||
+                        (-v1x * v2x - v1y * v2y) > 0.985;
+    if (!simple) {
+      double mag = 0.0;
+      for (int k = i0; k < i1; ++k) {
+        const int nb = P.inc_elem[k];
+        if (P.tri_mat[nb] == own_mat) {
+          double ex, ey;
+          ElementB(P, a, nb, &ex, &ey);
+          mag = fmax(mag, hypot(ex, ey));
+        }
+      }
+      double ox, oy;
+      ElementB(P, a, el, &ox, &oy);
+      const double own = hypot(ox, oy);
+      nbx[loc] = own > 0.0 ? mag * ox / own : 0.0;
+      nby[loc] = own > 0.0 ? mag * oy / own : 0.0;
+    }
+  }
+  return true;
+}
+
+// Per item: WST loads over the elements with a nonzero mask gradient
+// (field_postprocess.cpp:369-381) and the smoothed radial B at every probe (:382-398).
+__global__ __launch_bounds__(kBlockThreads) void PostprocessKernel(DevicePostPlan P,
+                                                                   const double *states,
+                                                                   double *loads, double *radial)
+{
+  __shared__ double lds[2 * kWaves * 4];
+  BlockReducer R{lds, 0};
+  const int item = blockIdx.x;
+  const double *a = states + static_cast<size_t>(item) * P.N;
+  constexpr double kMu0 = 4.0e-7 * 3.141592653589793238462643383279502884;
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int k = threadIdx.x; k < P.nwst; k += kBlockThreads) {
+    double bx, by;
+    ElementB(P, a, P.wst_elem[k], &bx, &by);
+    const double hx = P.wst_hx[k], hy = P.wst_hy[k];
+    const double fx = ((bx * bx - by * by) * hx + 2.0 * bx * by * hy) / (2.0 * kMu0);
+    const double fy = (2.0 * bx * by * hx + (by * by - bx * bx) * hy) / (2.0 * kMu0);
+    const double w = P.wst_w[k];
+    s[0] += w * fx;
+    s[1] += w * fy;
+    s[2] += w * (P.wst_cx[k] * fy - P.wst_cy[k] * fx);
+  }
+  R.Sum(s);
+  if (threadIdx.x == 0) {
+    loads[3 * item] = s[0];
+    loads[3 * item + 1] = s[1];
+    loads[3 * item + 2] = s[2];
+  }
+  for (int p = threadIdx.x; p < P.nprobe; p += kBlockThreads) {
+    double nbx[3], nby[3];
+    double value = NAN;
+    if (SmoothedNodalB(P, a, P.probe_elem[p], nbx, nby)) {
+      double bx = 0.0, by = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        bx += P.probe_lambda[3 * p + k] * nbx[k];
+        by += P.probe_lambda[3 * p + k] * nby[k];
+      }
+      value = bx * P.probe_cos[p] + by * P.probe_sin[p];
+    }
+    radial[static_cast<size_t>(item) * P.nprobe + p] = value;
+  }
+}
+
 template <typename T>
 T *Carve(char *&cursor, size_t count)
 {
@@ -935,48 +1117,61 @@ size_t Bytes(size_t count)
 
 int UploadPlan(const HostPlan &h, DeviceBuffers *dev)
 {
-  size_t total = 0;
-  auto add = [&total](size_t bytes) { total += (bytes + 255) / 256 * 256; };
-  add(h.tri_nodes.size() * 4);
-  add(h.tri_mat.size() * 4);
-  add(h.tri_circuit.size() * 4);
-  add(h.tri_w.size() * 8);
-  add(h.tri_gx.size() * 8);
-  add(h.tri_gy.size() * 8);
-  add(h.tri_flux.size() * 8);
-  add(h.mat_nu0.size() * 8 * 3);
-  add(h.mat_curve_off.size() * 4 * 2);
-  add(h.curve_b.size() * 8 * 3);
-  add(h.node_to_free.size() * 4);
-  add(h.free_nodes.size() * 4);
-  add(h.node_sign.size());
-  add(h.boundary.size() * 8);
-  add(h.slot_off.size() * 4);
-  add(h.contrib.size() * 4);
-  add(h.row_off.size() * 4);
-  add(h.rcontrib.size() * 4);
-  add(h.pm_rhs.size() * 8);
-  add(h.circuit_rhs.size() * 8);
-  add(h.slice_ptr.size() * 4);
-  add(h.slice_width.size() * 4);
-  add(h.sell_cols.size() * 4);
-  add(h.slot_to_sell.size() * 4);
-  add(h.diag_sell.size() * 4);
-  void *base = nullptr;
-  if (hipMalloc(&base, total) != hipSuccess)
-    return FEMM_GPU_ALLOCATION_FAILED;
-  dev->base = base;
-  char *cur = static_cast<char *>(base);
-  bool copy_ok = true;
-  auto up = [&cur, &copy_ok](const auto &vec) {
-    using T = typename std::decay_t<decltype(vec)>::value_type;
-    T *p = Carve<T>(cur, vec.size());
-    if (!vec.empty() &&
-        hipMemcpy(p, vec.data(), vec.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
-      copy_ok = false;
-    return static_cast<const T *>(p);
-  };
+  // Two passes over the same sequence of arrays: the first (null base) measures
+  // the 256-byte-aligned layout, the second allocates once and copies.
   DevicePlan &P = dev->plan;
+  for (int pass = 0; pass < 2; ++pass) {
+    char *base = static_cast<char *>(dev->base);
+    size_t offset = 0;
+    bool copy_ok = true;
+    auto up = [&](const auto &vec) {
+      using T = typename std::decay_t<decltype(vec)>::value_type;
+      T *p = reinterpret_cast<T *>(base + offset);
+      offset += (vec.size() * sizeof(T) + 255) / 256 * 256;
+      if (pass == 1 && !vec.empty() &&
+          hipMemcpy(p, vec.data(), vec.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+        copy_ok = false;
+      return static_cast<const T *>(p);
+    };
+    P.tri_nodes = up(h.tri_nodes);
+    P.tri_mat = up(h.tri_mat);
+    P.tri_circuit = up(h.tri_circuit);
+    P.tri_w = up(h.tri_w);
+    P.tri_gx = up(h.tri_gx);
+    P.tri_gy = up(h.tri_gy);
+    P.tri_flux = up(h.tri_flux);
+    P.mat_nu0 = up(h.mat_nu0);
+    P.mat_alpha = up(h.mat_alpha);
+    P.mat_cold = up(h.mat_cold);
+    P.mat_curve_off = up(h.mat_curve_off);
+    P.mat_curve_cnt = up(h.mat_curve_cnt);
+    P.curve_b = up(h.curve_b);
+    P.curve_h = up(h.curve_h);
+    P.curve_s = up(h.curve_s);
+    P.node_to_free = up(h.node_to_free);
+    P.free_nodes = up(h.free_nodes);
+    P.node_sign = up(h.node_sign);
+    P.boundary = up(h.boundary);
+    P.slot_off = up(h.slot_off);
+    P.contrib = up(h.contrib);
+    P.row_off = up(h.row_off);
+    P.rcontrib = up(h.rcontrib);
+    P.pm_rhs = up(h.pm_rhs);
+    P.circuit_rhs = up(h.circuit_rhs);
+    P.slice_ptr = up(h.slice_ptr);
+    P.slice_width = up(h.slice_width);
+    P.sell_cols = up(h.sell_cols);
+    P.slot_to_sell = up(h.slot_to_sell);
+    P.diag_sell = up(h.diag_sell);
+    if (pass == 0) {
+      void *mem = nullptr;
+      if (hipMalloc(&mem, offset) != hipSuccess)
+        return FEMM_GPU_ALLOCATION_FAILED;
+      dev->base = mem;
+    } else if (!copy_ok) {
+      return FEMM_INTERNAL_ERROR;
+    }
+  }
   P.N = h.N;
   P.T = h.T;
   P.n = h.n;
@@ -989,39 +1184,6 @@ int UploadPlan(const HostPlan &h, DeviceBuffers *dev)
   P.any_nonlinear = h.any_nonlinear ? 1 : 0;
   P.any_boundary_value = h.any_boundary_value ? 1 : 0;
   P.depth = h.depth;
-  P.tri_nodes = up(h.tri_nodes);
-  P.tri_mat = up(h.tri_mat);
-  P.tri_circuit = up(h.tri_circuit);
-  P.tri_w = up(h.tri_w);
-  P.tri_gx = up(h.tri_gx);
-  P.tri_gy = up(h.tri_gy);
-  P.tri_flux = up(h.tri_flux);
-  std::vector<double> mats;
-  P.mat_nu0 = up(h.mat_nu0);
-  P.mat_alpha = up(h.mat_alpha);
-  P.mat_cold = up(h.mat_cold);
-  P.mat_curve_off = up(h.mat_curve_off);
-  P.mat_curve_cnt = up(h.mat_curve_cnt);
-  P.curve_b = up(h.curve_b);
-  P.curve_h = up(h.curve_h);
-  P.curve_s = up(h.curve_s);
-  P.node_to_free = up(h.node_to_free);
-  P.free_nodes = up(h.free_nodes);
-  P.node_sign = up(h.node_sign);
-  P.boundary = up(h.boundary);
-  P.slot_off = up(h.slot_off);
-  P.contrib = up(h.contrib);
-  P.row_off = up(h.row_off);
-  P.rcontrib = up(h.rcontrib);
-  P.pm_rhs = up(h.pm_rhs);
-  P.circuit_rhs = up(h.circuit_rhs);
-  P.slice_ptr = up(h.slice_ptr);
-  P.slice_width = up(h.slice_width);
-  P.sell_cols = up(h.sell_cols);
-  P.slot_to_sell = up(h.slot_to_sell);
-  P.diag_sell = up(h.diag_sell);
-  if (static_cast<size_t>(cur - static_cast<char *>(base)) > total || !copy_ok)
-    return FEMM_INTERNAL_ERROR;
   return hipDeviceSynchronize() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
 }
 
@@ -1132,6 +1294,21 @@ int LaunchFinalize(const DeviceBuffers &dev, int32_t items, const double *states
   if (flux)
     hipLaunchKernelGGL(FluxKernel, dim3(items), dim3(kBlockThreads), 0, stream, dev.plan, states,
                        flux);
+  return hipGetLastError() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
+}
+
+}  // namespace femm
+
+namespace femm
+{
+
+int LaunchPostprocess(const DevicePostPlan &post, int32_t items, const double *states,
+                      double *loads, double *radial_b, hipStream_t stream)
+{
+  if (items <= 0)
+    return FEMM_INVALID_ARGUMENT;
+  hipLaunchKernelGGL(PostprocessKernel, dim3(items), dim3(kBlockThreads), 0, stream, post, states,
+                     loads, radial_b);
   return hipGetLastError() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
 }
 

@@ -1,0 +1,50 @@
+// Weighted-stress force/torque and air-gap radial B: geometry-only plan
+// (mask, mask gradient, probe location) and its device copy.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "device.hpp"
+#include "femm_gpu_amd.h"
+#include "plan.hpp"
+
+namespace femm
+{
+
+struct HostPostPlan {
+  int32_t status = FEMM_INTERNAL_ERROR;
+  std::vector<double> mask;                         // [N] 0/1
+  // elements with a nonzero mask gradient: index, gradient, weight (area * depth), centroid
+  std::vector<int32_t> wst_elem;
+  std::vector<double> wst_hx, wst_hy, wst_w, wst_cx, wst_cy;
+  // probes: containing element, barycentric coordinates, direction
+  std::vector<int32_t> probe_elem;
+  std::vector<double> probe_lambda, probe_cos, probe_sin;
+  // node -> incident elements (element order), node coordinates, triangle labels
+  std::vector<int32_t> inc_off, inc_elem;
+  std::vector<double> node_xy;
+  int32_t mask_iterations = 0;
+};
+
+struct DevicePostPlan {
+  int32_t nwst, nprobe, N, T, Q;
+  const int32_t *wst_elem, *probe_elem, *inc_off, *inc_elem, *tri_nodes, *tri_mat;
+  const double *wst_hx, *wst_hy, *wst_w, *wst_cx, *wst_cy;
+  const double *probe_lambda, *probe_cos, *probe_sin, *node_xy, *tri_gx, *tri_gy;
+};
+
+// Builds the plan (PrepareWeightedStressMask + SolveWeightedStressMask +
+// BuildFrozenPostprocessPlanWithMask of the reference, weighted_stress.cu:24-183 and
+// field_postprocess.cpp:221-339).  The mask Laplace solve runs on the GPU.
+int BuildPostPlan(const femm_model_t *model, const HostPlan &plan,
+                  const femm_postprocess_options_t &options, HostPostPlan *out);
+
+int UploadPostPlan(const HostPostPlan &host, const DeviceBuffers &dev, DevicePostPlan *out,
+                   void **base);
+
+// loads[items][3] = {Fx, Fy, torque}; radial_b[items][nprobe].
+int LaunchPostprocess(const DevicePostPlan &post, int32_t items, const double *states,
+                      double *loads, double *radial_b, hipStream_t stream);
+
+}  // namespace femm

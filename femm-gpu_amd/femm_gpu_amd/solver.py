@@ -41,6 +41,14 @@ class ItemResult(ctypes.Structure):
                 ("last_linear_tolerance", ctypes.c_double), ("linear_residual", ctypes.c_double)]
 
 
+class PostprocessOptions(ctypes.Structure):
+    _fields_ = [("selected_label", ctypes.c_int32), ("air_label", ctypes.c_int32),
+                ("selected_group", ctypes.c_int32), ("air_group", ctypes.c_int32),
+                ("airgap_radius_m", ctypes.c_double), ("angles_rad", P(ctypes.c_double)),
+                ("angle_count", ctypes.c_int32), ("max_mask_iterations", ctypes.c_int32),
+                ("mask_relative_tolerance", ctypes.c_double)]
+
+
 RESULT_DTYPE = np.dtype([("status", np.int32), ("newton_iterations", np.int32),
                          ("linear_iterations", np.int32), ("refinements", np.int32),
                          ("relative_step", np.float64), ("equation_residual", np.float64),
@@ -194,6 +202,39 @@ class Plan:
             ctypes.byref(config), None if warm_ptr is None else ctypes.c_void_p(warm_ptr),
             ctypes.c_void_p(a_ptr), ctypes.c_void_p(flux_ptr), ctypes.c_void_p(results_ptr),
             ctypes.c_void_p(stream_ptr)), "femm_newton_batch_device")
+
+    def set_postprocess(self, selected_label=-1, air_label=-1, selected_group=-1, air_group=-1,
+                        radius_m=0.0, angles_rad=(), max_mask_iterations=256,
+                        mask_relative_tolerance=1e-12):
+        """Weighted-stress mask and probe plan (BuildFrozenPostprocessPlan)."""
+        self._angles = np.ascontiguousarray(angles_rad, np.float64)
+        opts = PostprocessOptions(selected_label, air_label, selected_group, air_group,
+                                  radius_m, _dp(self._angles), len(self._angles),
+                                  max_mask_iterations, mask_relative_tolerance)
+        _check(self._lib.femm_plan_set_postprocess(self._handle, ctypes.byref(opts)),
+               "femm_plan_set_postprocess")
+        self.probe_count = len(self._angles)
+
+    def postprocess_mask(self):
+        mask = np.zeros(self.N)
+        _check(self._lib.femm_postprocess_copy_mask(self._handle, _dp(mask)),
+               "femm_postprocess_copy_mask")
+        return mask
+
+    def postprocess_batch(self, states):
+        """(loads[B][3] = Fx, Fy, torque; radial_b[B][probes]) for nodal states."""
+        states = np.ascontiguousarray(states, np.float64).reshape(-1, self.N)
+        B = states.shape[0]
+        loads, radial = np.zeros((B, 3)), np.zeros((B, max(1, self.probe_count)))
+        _check(self._lib.femm_postprocess_batch(self._handle, B, _dp(states), _dp(loads),
+                                                _dp(radial)), "femm_postprocess_batch")
+        return loads, radial[:, : self.probe_count]
+
+    def postprocess_batch_device(self, items, states_ptr, loads_ptr, radial_ptr, stream_ptr):
+        _check(self._lib.femm_postprocess_batch_device(
+            self._handle, int(items), ctypes.c_void_p(states_ptr), ctypes.c_void_p(loads_ptr),
+            ctypes.c_void_p(radial_ptr), ctypes.c_void_p(stream_ptr)),
+            "femm_postprocess_batch_device")
 
     def finalize_batch(self, states):
         states = np.ascontiguousarray(states, np.float64).reshape(-1, self.N)

@@ -103,6 +103,39 @@ int femm_newton_batch_device(femm_plan *plan, int32_t items, const double *curre
 int femm_finalize_batch(femm_plan *plan, int32_t items, const double *states, double *bx,
                         double *by, double *flux);
 
+/* Weighted-stress load and air-gap B selection (reference FrozenPostprocessOptions,
+ * gpu_solver/src/postprocess/postprocess.h:20-32). */
+typedef struct femm_postprocess_options {
+  int32_t selected_label;      /* material label of the loaded body (used when selected_group < 0) */
+  int32_t air_label;           /* material label of free space (used when air_group < 0) */
+  int32_t selected_group;      /* >= 0: select the loaded body by FEMM group number */
+  int32_t air_group;           /* >= 0: select free space by FEMM group number */
+  double airgap_radius_m;      /* probe circle radius */
+  const double *angles_rad;    /* probe angles */
+  int32_t angle_count;
+  int32_t max_mask_iterations; /* mask Laplace PCG cap (reference default 256) */
+  double mask_relative_tolerance; /* mask PCG tolerance (reference 1e-12) */
+} femm_postprocess_options_t;
+
+/* Geometry-only postprocess plan: weighted-stress mask (Laplace solve on the GPU,
+ * thresholded at 0.5), mask gradients and probe location.  Replaces
+ * PrepareWeightedStressMask / SolveWeightedStressMask (postprocess/weighted_stress.cu:24-183)
+ * and BuildFrozenPostprocessPlanWithMask (postprocess/field_postprocess.cpp:221-339). */
+int femm_plan_set_postprocess(femm_plan *plan, const femm_postprocess_options_t *options);
+
+/* The 0/1 mask of the current postprocess plan, mask[node_count]. */
+int femm_postprocess_copy_mask(const femm_plan *plan, double *mask);
+
+/* Weighted-stress force/torque and FEMM-smoothed radial air-gap B per item
+ * (ComputeFrozenPostprocess, postprocess/field_postprocess.cpp:341-403):
+ * states[items][N] -> loads[items][3] = {Fx, Fy, torque}, radial_b[items][angle_count]. */
+int femm_postprocess_batch(femm_plan *plan, int32_t items, const double *states, double *loads,
+                           double *radial_b);
+
+/* Same on device pointers, enqueued on `stream` (hipStream_t) without synchronising. */
+int femm_postprocess_batch_device(femm_plan *plan, int32_t items, const double *states,
+                                  double *loads, double *radial_b, void *stream);
+
 /* Stable status name (core/types.cpp:6-43). */
 const char *femm_status_name(int32_t status);
 

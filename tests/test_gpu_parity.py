@@ -201,3 +201,46 @@ def test_newton_invalid_inputs_are_isolated_per_item():
     limited = plan.solve_batch(np.array([[2.0]]), plain_options(max_newton_iterations=0))
     assert limited["results"][0]["status"] == 14
     plan.close()
+
+
+def test_postprocess_matches_frozen_femm_and_oracle():
+    """Weighted-stress loads and smoothed air-gap B on the GPU: frozen FEMM limits
+    (reference_commands.cu:144-208) and the oracle's mask and loads."""
+    say("case postprocess pm_coil")
+    model, _, current, _ = ff.nonlinear_pm_coil()
+    post = ff.nonlinear_pm_coil_postprocess()
+    plan = Plan(model)
+    opts = plain_options(relative_tolerance=1e-8, max_newton_iterations=128)
+    out = plan.solve_batch(np.array([[current], [0.5 * current]]), opts, reference_config(),
+                           fields=True)
+    plan.set_postprocess(selected_label=post["selected_label"], air_label=3,
+                         radius_m=post["radius_m"], angles_rad=post["angles"])
+    loads, radial = plan.postprocess_batch(out["a"])
+    f = loads[0]
+    assert near_mixed(f[0], post["fx"], 1e-8, 1e-5) and near_mixed(f[1], post["fy"], 1e-8, 1e-5)
+    assert near_mixed(f[2], post["torque"], 1e-10, 1e-5)
+    assert np.all(near_mixed(radial[0], post["radial_b"], 1e-9, 1e-5))
+    for b in range(2):
+        st, ref_loads, ref_br, ref_mask = ol.weighted_stress(
+            model, out["a"][b], out["bx"][b], out["by"][b], post["selected_label"], 3, -1, -1,
+            post["radius_m"], post["angles"])
+        assert st == 0
+        assert np.array_equal(plan.postprocess_mask(), ref_mask)
+        assert np.all(near_mixed(loads[b], ref_loads, 1e-12, 1e-10))
+        assert np.all(near_mixed(radial[b], ref_br, 1e-14, 1e-12))
+    again = plan.postprocess_batch(out["a"])
+    assert np.array_equal(again[0], loads) and np.array_equal(again[1], radial)
+    plan.close()
+
+
+def test_postprocess_uniform_field_cancels():
+    """physics_regression.cu:88-149: A = x, uniform B = (0, -1): zero loads, Br = 0 / -1."""
+    say("case postprocess uniform field")
+    model = ff.postprocess_ring()
+    plan = Plan(model)
+    plan.set_postprocess(selected_label=1, air_label=3, radius_m=0.75,
+                         angles_rad=[0.0, 0.5 * math.pi])
+    loads, radial = plan.postprocess_batch(model.node_xy[:, 0][None].copy())
+    np.testing.assert_allclose(loads[0], 0.0, atol=1e-8)
+    np.testing.assert_allclose(radial[0], [0.0, -1.0], atol=1e-12)
+    plan.close()
