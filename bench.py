@@ -108,8 +108,8 @@ def main():
 
     import torch
     from femm_gpu_amd import meshes
-    from femm_gpu_amd.model import production_options
-    from femm_gpu_amd.solver import Plan, device_info, reference_config
+    from femm_gpu_amd.model import accepted_options
+    from femm_gpu_amd.solver import Plan, device_info, mixed_config, reference_config
 
     torch.cuda.set_device(0)
     model, _ = meshes.motor_sector(angular_cells=args.angular_cells)
@@ -119,21 +119,32 @@ def main():
     plan = Plan(model)
     setup_s = time.perf_counter() - t0
     currents_dev = torch.from_numpy(currents).to("cuda")
-    options = production_options()
+    options = accepted_options()
     print(f"# device {device_info()}; mesh {model.name}: N={plan.N} T={plan.T} n={plan.n} "
           f"nnz={plan.nnz}; plan setup {setup_s:.2f} s", file=sys.stderr, flush=True)
 
-    plain = run_form(plan, currents_dev, B, args.steps, args.warmup, options, reference_config(),
-                     torch)
-    res = plain["results"]
-    passing = (res["status"] == 0) & (res["relative_step"] < 1e-6) & \
-              (res["equation_residual"] <= 1e-8)
-    print(f"# plain FP64: {plain['ms']:.1f} ms/step, pass {int(passing.sum())}/{B}, newton "
-          f"{res['newton_iterations'].mean():.1f}, pcg/item {res['linear_iterations'].mean():.0f}",
-          file=sys.stderr, flush=True)
-    bytes_total, pcg_iter_bytes = algorithmic_bytes(plan, res, B, 0)
-    achieved = bytes_total / (plain["ms"] * 1e-3) / 1e9
-    value = float(passing.sum()) / (plain["ms"] * 1e-3)
+    forms = {}
+    for name, config in (("plain_fp64", reference_config()), ("mixed_warm", mixed_config())):
+        run = run_form(plan, currents_dev, B, args.steps, args.warmup, options, config, torch)
+        res = run["results"]
+        passing = (res["status"] == 0) & (res["relative_step"] < 1e-6) & \
+                  (res["equation_residual"] <= 1e-8)
+        precision = 0 if name == "plain_fp64" else 1
+        bytes_total, pcg_iter_bytes = algorithmic_bytes(plan, res, B, precision)
+        forms[name] = dict(
+            ms=run["ms"], passing=int(passing.sum()), solves_per_s=float(passing.sum()) /
+            (run["ms"] * 1e-3), newton_mean=float(res["newton_iterations"].mean()),
+            pcg_mean=float(res["linear_iterations"].mean()),
+            achieved_gbs=bytes_total / (run["ms"] * 1e-3) / 1e9, pcg_iter_bytes=pcg_iter_bytes,
+            a=run["a"])
+        print(f"# {name}: {run['ms']:.1f} ms/step, pass {int(passing.sum())}/{B}, newton "
+              f"{forms[name]['newton_mean']:.1f}, pcg/item {forms[name]['pcg_mean']:.0f}, "
+              f"{forms[name]['achieved_gbs']:.0f} GB/s algorithmic", file=sys.stderr, flush=True)
+    plain, chosen = forms["plain_fp64"], forms["mixed_warm"]
+    diff = (chosen["a"] - plain["a"]).double()
+    rel_l2 = (diff.norm(dim=1) / plain["a"].double().norm(dim=1).clamp_min(1e-300)).max().item()
+    value = chosen["solves_per_s"]
+    achieved = chosen["achieved_gbs"]
 
     cpu = None
     if not args.no_cpu:
@@ -147,18 +158,29 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": plain["ms"],
+        "ms_per_step": chosen["ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 inner PCG / f64 refinement, assembly and residuals",
         "data": "synthetic: seeded 12-slot motor sector mesh and seeded phase currents",
         "config": {"workload": f"motor_sector 12-slot/4-pole anti-periodic, N={plan.N} nodes, "
                                f"T={plan.T} triangles, batch {B}, U(-8e6,8e6) A/m^2 phase currents",
-                   "form": "plain FP64 (reference algorithm)", "items_passing": int(passing.sum())},
+                   "form": "FP32 Jacobi PCG in FP64 refinement, PCG started from the current "
+                           "Newton state",
+                   "items_passing": chosen["passing"],
+                   "plain_fp64": {"solves_per_s": plain["solves_per_s"], "ms_per_step": plain["ms"],
+                                  "items_passing": plain["passing"],
+                                  "newton_mean": plain["newton_mean"],
+                                  "pcg_iterations_mean": plain["pcg_mean"],
+                                  "achieved_GBps": plain["achieved_gbs"]},
+                   "speedup_vs_plain_fp64": value / max(plain["solves_per_s"], 1e-30),
+                   "max_A_rel_l2_vs_plain_fp64": rel_l2,
+                   "newton_mean": chosen["newton_mean"], "pcg_iterations_mean": chosen["pcg_mean"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "NewtonKernel", "pcg_iteration_bytes_per_item": pcg_iter_bytes},
+                     "kernel": "NewtonKernel<mixed>",
+                     "pcg_iteration_bytes_per_item": chosen["pcg_iter_bytes"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -90,6 +90,8 @@ femm_newton_options_t DefaultOptions()
   o.max_linear_iterations = 128;
   o.recover_nonconvergence = 0;
   o.adaptive_linear_tolerance = 0;
+  o.require_equation_residual = 0;
+  o.reserved = 0;
   return o;
 }
 

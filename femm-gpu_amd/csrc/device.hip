@@ -1,8 +1,8 @@
 // HIP kernels of the MI355X nonlinear magnetostatic solve path (gfx950, wave64).
 //
-// Work decomposition: one 1024-thread workgroup owns one batch item for the
+// Work decomposition: one 512-thread workgroup owns one batch item for the
 // whole Newton solve.  Every reduction is a fixed-shape wave64 shuffle tree
-// followed by a fixed-order sum of the 16 wave partials, so results are
+// followed by a fixed-order sum of the 8 wave partials, so results are
 // bitwise reproducible from run to run and independent of batch composition.
 // There are no atomics and no inter-workgroup synchronisation; the hardware
 // dispatcher load-balances items across the 256 CUs as workgroups retire.
@@ -265,18 +265,46 @@ __device__ bool GatherRhs(const DevicePlan &P, const double *elem, const double 
 // ----------------------------------------------------------------------------- SpMV
 
 // y = A v over SELL-64 (lane = row of the slice); returns the local sum of
-// w[row] * y[row] when w is given.
+// w[row] * y[row] when w is given.  A workgroup owns one item, so the memory
+// parallelism of one CU must come from within each wave: every slice of width
+// <= kUnrolledWidth issues all of its column and value loads before the first
+// gather, then all gathers before the first FMA.  The sum order over a row's
+// entries is the CSR order either way.
+constexpr int kUnrolledWidth = 10;
+
 template <typename T, typename V>
-__device__ double Spmv(const DevicePlan &P, const T *vals, const V *v, V *y, const V *w)
+__device__ double Spmv(const DevicePlan &P, const T *__restrict__ vals, const V *__restrict__ v,
+                       V *__restrict__ y, const V *__restrict__ w)
 {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int *__restrict__ cols = P.sell_cols;
   double local = 0.0;
   for (int s = wave; s < P.nslices; s += kWaves) {
     const int base = P.slice_ptr[s] + lane, width = P.slice_width[s];
     T sum = 0;
-    for (int k = 0; k < width; ++k) {
-      const int pos = base + k * kSliceRows;
-      sum += vals[pos] * static_cast<T>(v[P.sell_cols[pos]]);
+    if (width <= kUnrolledWidth) {
+      int c[kUnrolledWidth];
+      T a[kUnrolledWidth];
+#pragma unroll
+      for (int k = 0; k < kUnrolledWidth; ++k)
+        if (k < width) {
+          c[k] = cols[base + k * kSliceRows];
+          a[k] = vals[base + k * kSliceRows];
+        }
+      V g[kUnrolledWidth];
+#pragma unroll
+      for (int k = 0; k < kUnrolledWidth; ++k)
+        if (k < width)
+          g[k] = v[c[k]];
+#pragma unroll
+      for (int k = 0; k < kUnrolledWidth; ++k)
+        if (k < width)
+          sum += a[k] * static_cast<T>(g[k]);
+    } else {
+      for (int k = 0; k < width; ++k) {
+        const int pos = base + k * kSliceRows;
+        sum += vals[pos] * static_cast<T>(v[cols[pos]]);
+      }
     }
     const int row = s * kSliceRows + lane;
     if (row < P.n) {
@@ -287,6 +315,10 @@ __device__ double Spmv(const DevicePlan &P, const T *vals, const V *v, V *y, con
   }
   return local;
 }
+
+// Vector passes keep kVecUnroll independent rows in flight per thread (all
+// loads of a step are issued before any arithmetic).
+constexpr int kVecUnroll = 4;
 
 // ----------------------------------------------------------------------------- FP64 PCG
 
@@ -386,12 +418,30 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
     }
     const double alpha = rho / pap[0];
     double s1[2] = {0.0, 0.0};
-    for (int i = threadIdx.x; i < n; i += kBlockThreads) {
-      x[i] += alpha * p[i];
-      const double ri = r[i] - alpha * ap[i];
-      r[i] = ri;
-      s1[0] += ri * ri;
-      s1[1] += ri * (ri * inv[i]);
+    for (int i0 = threadIdx.x; i0 < n; i0 += kBlockThreads * kVecUnroll) {
+      double xv[kVecUnroll], pv[kVecUnroll], rv[kVecUnroll], av[kVecUnroll], iv[kVecUnroll];
+#pragma unroll
+      for (int u = 0; u < kVecUnroll; ++u) {
+        const int i = i0 + u * kBlockThreads;
+        if (i < n) {
+          xv[u] = x[i];
+          pv[u] = p[i];
+          rv[u] = r[i];
+          av[u] = ap[i];
+          iv[u] = inv[i];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kVecUnroll; ++u) {
+        const int i = i0 + u * kBlockThreads;
+        if (i < n) {
+          x[i] = xv[u] + alpha * pv[u];
+          const double ri = rv[u] - alpha * av[u];
+          r[i] = ri;
+          s1[0] += ri * ri;
+          s1[1] += ri * (ri * iv[u]);
+        }
+      }
     }
     R.Sum(s1);
     info.iterations = it + 1;
@@ -446,8 +496,24 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
       return info;
     }
     const double beta = s1[1] / rho;
-    for (int i = threadIdx.x; i < n; i += kBlockThreads)
-      p[i] = r[i] * inv[i] + beta * p[i];
+    for (int i0 = threadIdx.x; i0 < n; i0 += kBlockThreads * kVecUnroll) {
+      double rv[kVecUnroll], iv[kVecUnroll], pv[kVecUnroll];
+#pragma unroll
+      for (int u = 0; u < kVecUnroll; ++u) {
+        const int i = i0 + u * kBlockThreads;
+        if (i < n) {
+          rv[u] = r[i];
+          iv[u] = inv[i];
+          pv[u] = p[i];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kVecUnroll; ++u) {
+        const int i = i0 + u * kBlockThreads;
+        if (i < n)
+          p[i] = rv[u] * iv[u] + beta * pv[u];
+      }
+    }
     rho = s1[1];
     __syncthreads();
   }
@@ -502,12 +568,30 @@ __device__ int PcgInner32(const DevicePlan &P, const float *vals32, const float 
       break;
     const float alpha = static_cast<float>(rho / pap[0]);
     double s1[2] = {0.0, 0.0};
-    for (int i = threadIdx.x; i < n; i += kBlockThreads) {
-      d32[i] += alpha * p32[i];
-      const float ri = r32[i] - alpha * ap32[i];
-      r32[i] = ri;
-      s1[0] += static_cast<double>(ri) * ri;
-      s1[1] += static_cast<double>(ri) * (ri * inv32[i]);
+    for (int i0 = threadIdx.x; i0 < n; i0 += kBlockThreads * kVecUnroll) {
+      float dv[kVecUnroll], pv[kVecUnroll], rv[kVecUnroll], av[kVecUnroll], iv[kVecUnroll];
+#pragma unroll
+      for (int u = 0; u < kVecUnroll; ++u) {
+        const int i = i0 + u * kBlockThreads;
+        if (i < n) {
+          dv[u] = d32[i];
+          pv[u] = p32[i];
+          rv[u] = r32[i];
+          av[u] = ap32[i];
+          iv[u] = inv32[i];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kVecUnroll; ++u) {
+        const int i = i0 + u * kBlockThreads;
+        if (i < n) {
+          d32[i] = dv[u] + alpha * pv[u];
+          const float ri = rv[u] - alpha * av[u];
+          r32[i] = ri;
+          s1[0] += static_cast<double>(ri) * ri;
+          s1[1] += static_cast<double>(ri) * (ri * iv[u]);
+        }
+      }
     }
     R.Sum(s1);
     if (!(s1[0] > stop2) || !(s1[1] > 0.0) || !isfinite(s1[1])) {
@@ -515,8 +599,24 @@ __device__ int PcgInner32(const DevicePlan &P, const float *vals32, const float 
       break;
     }
     const float beta = static_cast<float>(s1[1] / rho);
-    for (int i = threadIdx.x; i < n; i += kBlockThreads)
-      p32[i] = r32[i] * inv32[i] + beta * p32[i];
+    for (int i0 = threadIdx.x; i0 < n; i0 += kBlockThreads * kVecUnroll) {
+      float rv[kVecUnroll], iv[kVecUnroll], pv[kVecUnroll];
+#pragma unroll
+      for (int u = 0; u < kVecUnroll; ++u) {
+        const int i = i0 + u * kBlockThreads;
+        if (i < n) {
+          rv[u] = r32[i];
+          iv[u] = inv32[i];
+          pv[u] = p32[i];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kVecUnroll; ++u) {
+        const int i = i0 + u * kBlockThreads;
+        if (i < n)
+          p32[i] = rv[u] * iv[u] + beta * pv[u];
+      }
+    }
     rho = s1[1];
     __syncthreads();
   }
@@ -707,6 +807,7 @@ __device__ void WriteFlux(const DevicePlan &P, const double *a, double *flux, Bl
   }
 }
 
+template <bool kMixed>
 __global__ __launch_bounds__(kBlockThreads) void NewtonKernel(DevicePlan P, DeviceWork W,
                                                               const double *currents,
                                                               NewtonArgs args, const double *warm,
@@ -718,7 +819,7 @@ __global__ __launch_bounds__(kBlockThreads) void NewtonKernel(DevicePlan P, Devi
   const int item = blockIdx.x;
   const femm_newton_options_t &o = args.options;
   const femm_solve_config_t &cfg = args.config;
-  const bool mixed = cfg.precision == FEMM_PRECISION_MIXED;
+  constexpr bool mixed = kMixed;
   const ItemPointers w = ItemWork(P, W, item, mixed);
   double *a = A + static_cast<size_t>(item) * P.N;
   const double *cur = currents + static_cast<size_t>(item) * P.C;
@@ -744,8 +845,9 @@ __global__ __launch_bounds__(kBlockThreads) void NewtonKernel(DevicePlan P, Devi
   __syncthreads();
 
   double eq_scale = 1.0;
+  const bool gate_eq = o.require_equation_residual != 0;
   int status = inputs_ok ? FEMM_NONLINEAR_SOLVE_NOT_CONVERGED : FEMM_INVALID_ARGUMENT;
-  if (inputs_ok && cfg.verify_equation_residual) {
+  if (inputs_ok && (cfg.verify_equation_residual || gate_eq)) {
     // Scale of the equation residual: the norm at the Dirichlet-expanded start
     // (nonlinear_solver.cu:565-576), independent of any warm start.  Assemble at
     // the boundary-only state held temporarily in `a`, then restore.
@@ -768,13 +870,30 @@ __global__ __launch_bounds__(kBlockThreads) void NewtonKernel(DevicePlan P, Devi
   }
 
   double omega = 1.0, prev = INFINITY;
-  for (int it = 0; inputs_ok && it < o.max_newton_iterations; ++it) {
+  bool pending = false;   // step gate passed; equation residual not yet checked
+  bool eq_current = false;
+  for (int it = 0; inputs_ok; ++it) {
+    if (!pending && it >= o.max_newton_iterations)
+      break;
     const bool newton = has_warm || it > 0;
     if (!AssembleItem(P, w, a, cur, newton, R)) {
       status = FEMM_ASSEMBLY_FAILED;
       break;
     }
     __syncthreads();
+    if (pending) {
+      // The Newton assembly at the returned state also yields its equation
+      // residual (nonlinear_solver.cu:549-558); accept only if it passes too.
+      res.equation_residual = EquationNorm(P, w, a, R) / eq_scale;
+      eq_current = true;
+      pending = false;
+      if (res.equation_residual <= o.relative_tolerance) {
+        status = FEMM_OK;
+        break;
+      }
+      if (it >= o.max_newton_iterations)
+        break;
+    }
     const double tol = LinearTolerance(o, adaptive, prev);
     if (cfg.pcg_warm_start) {
       for (int i = threadIdx.x; i < P.n; i += kBlockThreads)
@@ -782,7 +901,7 @@ __global__ __launch_bounds__(kBlockThreads) void NewtonKernel(DevicePlan P, Devi
       __syncthreads();
     }
     LinearInfo li;
-    if (mixed) {
+    if constexpr (kMixed) {
       const RefinedInfo ri = RefinedSolve(
           P, w.vals, w.inv, w.vals32, w.inv32, w.rhs, w.x, w.r, w.p, w.ap, w.d32, w.r32, w.p32,
           w.ap32, tol, o.max_linear_iterations, cfg.pcg_warm_start != 0, cfg.max_refinements,
@@ -831,14 +950,18 @@ __global__ __launch_bounds__(kBlockThreads) void NewtonKernel(DevicePlan P, Devi
       a[v] += omega * (cand - a[v]);
     }
     __syncthreads();
+    eq_current = false;
     // NewtonStepConverged (nonlinear_model.cpp:186-193).
     if (it > 0 && step < 100.0 * o.relative_tolerance && tol <= o.linear_relative_tolerance) {
-      status = FEMM_OK;
-      break;
+      if (!gate_eq) {
+        status = FEMM_OK;
+        break;
+      }
+      pending = true;
     }
     prev = step;
   }
-  if (inputs_ok && cfg.verify_equation_residual &&
+  if (inputs_ok && cfg.verify_equation_residual && !eq_current &&
       (status == FEMM_OK || status == FEMM_NONLINEAR_SOLVE_NOT_CONVERGED)) {
     if (AssembleItem(P, w, a, cur, true, R)) {
       __syncthreads();
@@ -1256,8 +1379,12 @@ int LaunchNewton(const DeviceBuffers &dev, int32_t items, const double *currents
   if (items <= 0 || items > dev.work.capacity)
     return FEMM_INVALID_ARGUMENT;
   NewtonArgs args{options, config};
-  hipLaunchKernelGGL(NewtonKernel, dim3(items), dim3(kBlockThreads), 0, stream, dev.plan,
-                     dev.work, currents, args, warm, a, flux, results);
+  if (config.precision == FEMM_PRECISION_MIXED)
+    hipLaunchKernelGGL(NewtonKernel<true>, dim3(items), dim3(kBlockThreads), 0, stream, dev.plan,
+                       dev.work, currents, args, warm, a, flux, results);
+  else
+    hipLaunchKernelGGL(NewtonKernel<false>, dim3(items), dim3(kBlockThreads), 0, stream, dev.plan,
+                       dev.work, currents, args, warm, a, flux, results);
   return hipGetLastError() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
 }
 

@@ -11,7 +11,7 @@
 namespace femm
 {
 
-constexpr int kBlockThreads = 1024;   // one workgroup (16 wave64) per batch item
+constexpr int kBlockThreads = 512;    // one workgroup (8 wave64) per batch item
 constexpr int kElemStride = 12;       // per-triangle 3x3 Newton matrix + 3 rhs terms
 
 // Device pointers of the immutable plan (uploaded once per geometry).

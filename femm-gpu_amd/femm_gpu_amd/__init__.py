@@ -4,5 +4,5 @@ Python access to the C-ABI library ``lib/libfemm_gpu_amd.so`` and the model /
 mesh utilities around it.
 """
 
-from .model import (AXISYMMETRIC, MU0, PLANAR, Material, Model, load_artifact,  # noqa: F401
-                    plain_options, production_options, status_name)
+from .model import (AXISYMMETRIC, MU0, PLANAR, Material, Model, accepted_options,  # noqa: F401
+                    load_artifact, plain_options, production_options, status_name)

@@ -81,6 +81,8 @@ class CNewtonOptions(ctypes.Structure):
         ("max_linear_iterations", ctypes.c_int32),
         ("recover_nonconvergence", ctypes.c_int32),
         ("adaptive_linear_tolerance", ctypes.c_int32),
+        ("require_equation_residual", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -91,6 +93,14 @@ def production_options(**overrides) -> CNewtonOptions:
                   recover_nonconvergence=1, adaptive_linear_tolerance=1)
     values.update(overrides)
     return CNewtonOptions(**values)
+
+
+def accepted_options(**overrides) -> CNewtonOptions:
+    """ProductionSampleOptions plus the equation-residual gate: an item is accepted only when
+    its undamped step is < 1e-6 and its equation residual is <= 1e-8."""
+    values = dict(require_equation_residual=1)
+    values.update(overrides)
+    return production_options(**values)
 
 
 def plain_options(**overrides) -> CNewtonOptions:

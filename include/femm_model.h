@@ -94,6 +94,12 @@ typedef struct femm_newton_options {
   int32_t max_linear_iterations;
   int32_t recover_nonconvergence;   /* bounded Armijo recovery for exhausted items */
   int32_t adaptive_linear_tolerance;
+  int32_t require_equation_residual; /* 1: accept an item only when its undamped step is
+                                        below 100x relative_tolerance AND its equation residual
+                                        is <= relative_tolerance (the two gates of the
+                                        reference's SolveSafeguarded, nonlinear_solver.cu:623-627);
+                                        0: the step gate alone (NewtonStepConverged) */
+  int32_t reserved;
 } femm_newton_options_t;
 
 #ifdef __cplusplus

@@ -1176,8 +1176,18 @@ int NewtonSolve(const Model &m, const std::vector<double> &cur, femm_newton_opti
     res->newton_iterations = it + 1;
     res->relative_step = step;
     if (StepConverged(o, it, step, tol)) {
-      status = FEMM_OK;
-      break;
+      if (!o.require_equation_residual) {
+        status = FEMM_OK;
+        break;
+      }
+      // Both gates of nonlinear_solver.cu:623-627: the equation residual at the
+      // returned state must also pass before the item is accepted.
+      const double eq = EquationResidual(m, a, cur);
+      res->equation_residual = eq;
+      if (eq <= o.relative_tolerance) {
+        status = FEMM_OK;
+        break;
+      }
     }
     prev = step;
   }

@@ -18,8 +18,8 @@ import pytest
 import femm_fixtures as ff
 import oracle_lib as ol
 from femm_gpu_amd import meshes
-from femm_gpu_amd.model import plain_options, production_options
-from femm_gpu_amd.solver import Plan, reference_config
+from femm_gpu_amd.model import accepted_options, plain_options, production_options
+from femm_gpu_amd.solver import Plan, mixed_config, reference_config
 
 pytestmark = pytest.mark.gpu
 
@@ -169,7 +169,7 @@ def test_newton_motor_matches_oracle_and_is_deterministic():
     model, _ = meshes.motor_sector(angular_cells=160)
     currents = np.vstack([meshes.motor_currents(3, seed=5), np.zeros((1, 3))])
     plan = Plan(model)
-    opts = production_options()
+    opts = accepted_options()
     out = plan.solve_batch(currents, opts, reference_config())
     a_ref, flux_ref, res_ref = ol.newton_batch(model, currents, opts, threads=16)
     for b in range(len(currents)):
@@ -243,4 +243,28 @@ def test_postprocess_uniform_field_cancels():
     loads, radial = plan.postprocess_batch(model.node_xy[:, 0][None].copy())
     np.testing.assert_allclose(loads[0], 0.0, atol=1e-8)
     np.testing.assert_allclose(radial[0], [0.0, -1.0], atol=1e-12)
+    plan.close()
+
+
+def test_mixed_precision_form_matches_oracle():
+    """FP32 PCG inside FP64 refinement from the current state, same acceptance gates:
+    A within 1e-9 relative L2 of the FP64 oracle, flux within 1e-7."""
+    say("case newton motor_small mixed precision")
+    model, _ = meshes.motor_sector(angular_cells=160)
+    currents = np.vstack([meshes.motor_currents(3, seed=5), np.zeros((1, 3))])
+    plan = Plan(model)
+    opts = accepted_options()
+    out = plan.solve_batch(currents, opts, mixed_config())
+    a_ref, flux_ref, res_ref = ol.newton_batch(model, currents, opts, threads=16)
+    for b in range(len(currents)):
+        r = out["results"][b]
+        say(f"  item {b}: status {r['status']} newton {r['newton_iterations']} "
+            f"pcg {r['linear_iterations']} refinements {r['refinements']} "
+            f"eq {r['equation_residual']:.3e} | dA {_rel_l2(out['a'][b], a_ref[b]):.3e}")
+        assert r["status"] == 0
+        assert r["relative_step"] < 1e-6 and r["equation_residual"] <= 1e-8
+        assert _rel_l2(out["a"][b], a_ref[b]) <= 1e-9
+        assert np.abs(out["flux"][b] - flux_ref[b]).max() <= 1e-7 * np.abs(flux_ref[b]).max()
+    again = plan.solve_batch(currents, opts, mixed_config())
+    assert np.array_equal(again["a"], out["a"])
     plan.close()
