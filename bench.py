@@ -3,9 +3,19 @@
 Workload: 256 samples of a ~60k-node 12-slot / 4-pole anti-periodic motor sector
 (58,682 nodes, 116,160 P1 triangles, B-H iron, two NdFeB blocks, three phase coils),
 phase current densities ~ U(-8e6, 8e6) A/m^2 from a fixed seed.  One step = one batched
-Newton solve of all samples to the undamped 1e-6 step and the 1e-8 equation residual
-(assembly, PCG, state updates, equation-residual check and flux reductions all on the GPU).
+nonlinear solve of all samples to the undamped 1e-6 step and the 1e-8 equation residual
+(assembly, PCG, state updates, equation-residual checks and flux reductions all on the GPU).
 An item counts only if it passes both gates.  Inputs are resident in HBM before timing.
+
+Forms (both timed, same items, same gates):
+* plain_fp64   - the reference algorithm: FP64 Jacobi PCG from x0 = 0, Newton with the
+                 reference relaxation policy from the plain start.
+* learned_mixed (the reported value) - learned warm start (POD + linear path + BF16 MFMA
+                 MLP, predicted on the GPU inside the timed step), safeguarded Newton from
+                 it, FP32 PCG with FP64 reliable updates started from the current state,
+                 plain-start fallback for any item not accepted within the warm budget.
+                 The network is trained beforehand, untimed, on solver solutions of a
+                 disjoint seeded sample set; its cost is reported as `training`.
 
 Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs on one GPU.
 """
@@ -24,6 +34,8 @@ sys.path.insert(0, str(ROOT / "femm-gpu_amd"))
 
 HBM_PEAK_GBS = 8000.0
 RESULT_BYTES = 48
+TRAIN_SEED = 777
+WARM_BUDGET = 16
 
 
 def measured_traffic(kernel, workload):
@@ -39,57 +51,52 @@ def measured_traffic(kernel, workload):
     return rec.get("hbm_bytes_per_launch")
 
 
-def algorithmic_bytes(plan, results, B, precision):
-    """Algorithmic HBM bytes of one batched solve (per-item counters from the kernel).
+def algorithmic_bytes(plan, results, B, mixed):
+    """Algorithmic HBM bytes of one batched solve from the per-item kernel counters.
 
     PCG iteration (per item): s_v nnz + (4 nnz + 4 (n + 1)) / S + 13 s_x n with S = B (the
-    CSR pattern is shared by every item), s_v = s_x = 8 (FP64) or 4 (FP32 inner).
-    Newton iteration (per item): element pass (read A: 8N, write 12 doubles per triangle),
-    slot and row gathers (read 12 doubles per triangle, write 8 nnz + 16 n), step and
-    update passes (24 N).  Two extra assemblies + one SpMV per item for the equation
-    residual."""
+    CSR pattern is shared by every item); s_v = s_x = 8 (FP64) or 4 (FP32 iterations).
+    FP64 residual replacement (mixed form): 8 nnz + (4 nnz + 4 (n + 1)) / S + 76 n.
+    Newton iteration (per item): element pass (read A: 8 N, write 12 doubles per
+    triangle), slot and row gathers (read 12 doubles per triangle, write 8 nnz + 16 n),
+    step and update passes (24 N), equation-residual SpMV (8 nnz + 32 n)."""
     N, T, n, nnz = plan.N, plan.T, plan.n, plan.nnz
-    s = 8 if precision == 0 else 4
-    pcg_iter = s * nnz + (4 * nnz + 4 * (n + 1)) / B + 13 * s * n
+    s = 4 if mixed else 8
+    shared = (4 * nnz + 4 * (n + 1)) / B
+    pcg_iter = s * nnz + shared + 13 * s * n
+    replacement = 8 * nnz + shared + 76 * n
     assembly = 8 * N + 96 * T + 96 * T + 8 * nnz + 16 * n
-    newton_extra = 24 * N
-    eq = 2 * assembly + 8 * nnz + 24 * n
+    newton = assembly + 24 * N + 8 * nnz + 32 * n
     total = 0.0
     for r in results:
-        total += (r["linear_iterations"] * pcg_iter
-                  + r["newton_iterations"] * (assembly + newton_extra) + eq)
+        total += (r["linear_iterations"] * pcg_iter + r["newton_iterations"] * newton
+                  + (r["refinements"] * replacement if mixed else 0.0) + 2 * assembly)
     return total, pcg_iter
 
 
-def run_form(plan, currents_dev, B, steps, warmup, options, config, torch, warm_dev=None):
-    from femm_gpu_amd.solver import RESULT_DTYPE
-    a = torch.zeros((B, plan.N), dtype=torch.float64, device="cuda")
-    flux = torch.zeros((B, plan.C), dtype=torch.float64, device="cuda")
-    res = torch.zeros(B * RESULT_BYTES, dtype=torch.uint8, device="cuda")
-    stream = torch.cuda.current_stream()
+class Timed:
+    """HIP events on the stream the library launches on."""
 
-    def launch():
-        plan.solve_batch_device(currents_dev.data_ptr(), B, a.data_ptr(), flux.data_ptr(),
-                                res.data_ptr(), stream.cuda_stream, options, config,
-                                None if warm_dev is None else warm_dev.data_ptr())
+    def __init__(self, torch):
+        self.torch = torch
+        self.stream = torch.cuda.current_stream()
 
-    for _ in range(warmup):
-        launch()
-    torch.cuda.synchronize()
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    start.record(stream)
-    for _ in range(steps):
-        launch()
-    stop.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = start.elapsed_time(stop) / steps
-    results = np.frombuffer(res.cpu().numpy().tobytes(), dtype=RESULT_DTYPE)
-    return dict(ms=ms, wall=wall, a=a, flux=flux, results=results)
+    def run(self, launch, steps, warmup):
+        torch = self.torch
+        for _ in range(warmup):
+            launch()
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        start.record(self.stream)
+        for _ in range(steps):
+            launch()
+        stop.record(self.stream)
+        torch.cuda.synchronize()
+        return start.elapsed_time(stop) / steps, time.perf_counter() - t0
 
 
-def cpu_baseline(model, currents, options, seconds_hint):
+def cpu_baseline(model, currents, options):
     """The CPU oracle (an FP64 port of the reference's host assembler + Jacobi PCG + Newton),
     one item per host thread on a bounded sample of the same workload."""
     sys.path.insert(0, str(ROOT / "tests"))
@@ -108,14 +115,15 @@ def cpu_baseline(model, currents, options, seconds_hint):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--items", type=int, default=256)
     ap.add_argument("--seed", type=int, default=20251203)
+    ap.add_argument("--train-samples", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--angular-cells", type=int, default=480)
-    ap.add_argument("--forms", default="plain_fp64,mixed_warm",
-                    help="comma-separated subset of plain_fp64,mixed_warm (profiling runs)")
+    ap.add_argument("--forms", default="plain_fp64,learned_mixed",
+                    help="comma-separated subset of plain_fp64,learned_mixed,mixed_plain_start")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first, as the known "
                          "reference dispatch for FETCH_SIZE / WRITE_SIZE scaling")
@@ -126,8 +134,9 @@ def main():
 
     import torch
     from femm_gpu_amd import meshes
+    from femm_gpu_amd.learned import WarmStart, train_settings
     from femm_gpu_amd.model import accepted_options
-    from femm_gpu_amd.solver import Plan, device_info, mixed_config, reference_config
+    from femm_gpu_amd.solver import RESULT_DTYPE, Plan, device_info, mixed_config, reference_config
 
     torch.cuda.set_device(0)
     model, _ = meshes.motor_sector(angular_cells=args.angular_cells)
@@ -138,6 +147,7 @@ def main():
     setup_s = time.perf_counter() - t0
     currents_dev = torch.from_numpy(currents).to("cuda")
     options = accepted_options()
+    wanted = [f.strip() for f in args.forms.split(",") if f.strip()]
     print(f"# device {device_info()}; mesh {model.name}: N={plan.N} T={plan.T} n={plan.n} "
           f"nnz={plan.nnz}; plan setup {setup_s:.2f} s", file=sys.stderr, flush=True)
 
@@ -151,41 +161,96 @@ def main():
               f"{args.calibrate_copy_mib} MiB written", file=sys.stderr, flush=True)
         del src, dst
 
-    wanted = [f.strip() for f in args.forms.split(",") if f.strip()]
+    timer = Timed(torch)
+    stream = timer.stream.cuda_stream
+    a = torch.zeros((B, plan.N), dtype=torch.float64, device="cuda")
+    flux = torch.zeros((B, plan.C), dtype=torch.float64, device="cuda")
+    res = torch.zeros(B * RESULT_BYTES, dtype=torch.uint8, device="cuda")
+
+    def collect(ms, wall, mixed):
+        r = np.frombuffer(res.cpu().numpy().tobytes(), dtype=RESULT_DTYPE).copy()
+        passing = (r["status"] == 0) & (r["relative_step"] < 1e-6) & \
+                  (r["equation_residual"] <= 1e-8)
+        bytes_total, pcg_iter_bytes = algorithmic_bytes(plan, r, B, mixed)
+        newton = r["newton_iterations"]
+        hist = {int(q): int(v) for q, v in zip(*np.unique(newton, return_counts=True))}
+        return dict(ms=ms, wall=wall, passing=int(passing.sum()), newton_hist=hist,
+                    solves_per_s=float(passing.sum()) / (ms * 1e-3),
+                    newton_mean=float(r["newton_iterations"].mean()),
+                    pcg_mean=float(r["linear_iterations"].mean()),
+                    achieved_gbs=bytes_total / (ms * 1e-3) / 1e9, pcg_iter_bytes=pcg_iter_bytes,
+                    a=a.clone(), flux=flux.clone())
+
+    def report(name, f):
+        print(f"# {name}: {f['ms']:.1f} ms/step, pass {f['passing']}/{B}, newton "
+              f"{f['newton_mean']:.2f}, pcg/item {f['pcg_mean']:.0f}, "
+              f"{f['achieved_gbs']:.0f} GB/s algorithmic; newton counts {f['newton_hist']}",
+              file=sys.stderr, flush=True)
+
     forms = {}
-    for name, config in (("plain_fp64", reference_config()), ("mixed_warm", mixed_config())):
-        if name not in wanted:
-            continue
-        run = run_form(plan, currents_dev, B, args.steps, args.warmup, options, config, torch)
-        res = run["results"]
-        passing = (res["status"] == 0) & (res["relative_step"] < 1e-6) & \
-                  (res["equation_residual"] <= 1e-8)
-        precision = 0 if name == "plain_fp64" else 1
-        bytes_total, pcg_iter_bytes = algorithmic_bytes(plan, res, B, precision)
-        forms[name] = dict(
-            ms=run["ms"], passing=int(passing.sum()), solves_per_s=float(passing.sum()) /
-            (run["ms"] * 1e-3), newton_mean=float(res["newton_iterations"].mean()),
-            pcg_mean=float(res["linear_iterations"].mean()),
-            achieved_gbs=bytes_total / (run["ms"] * 1e-3) / 1e9, pcg_iter_bytes=pcg_iter_bytes,
-            a=run["a"])
-        print(f"# {name}: {run['ms']:.1f} ms/step, pass {int(passing.sum())}/{B}, newton "
-              f"{forms[name]['newton_mean']:.1f}, pcg/item {forms[name]['pcg_mean']:.0f}, "
-              f"{forms[name]['achieved_gbs']:.0f} GB/s algorithmic", file=sys.stderr, flush=True)
-    if len(forms) < 2:
+    training = None
+    if "plain_fp64" in wanted:
+        cfg = reference_config()
+        ms, wall = timer.run(lambda: plan.solve_batch_device(
+            currents_dev.data_ptr(), B, a.data_ptr(), flux.data_ptr(), res.data_ptr(), stream,
+            options, cfg), args.steps, args.warmup)
+        forms["plain_fp64"] = collect(ms, wall, False)
+        report("plain_fp64", forms["plain_fp64"])
+    if "mixed_plain_start" in wanted:
+        cfg = mixed_config()
+        ms, wall = timer.run(lambda: plan.solve_batch_device(
+            currents_dev.data_ptr(), B, a.data_ptr(), flux.data_ptr(), res.data_ptr(), stream,
+            options, cfg), args.steps, args.warmup)
+        forms["mixed_plain_start"] = collect(ms, wall, True)
+        report("mixed_plain_start", forms["mixed_plain_start"])
+    if "learned_mixed" in wanted:
+        # Untimed training: solver solutions of a disjoint seeded sample set.
+        t0 = time.perf_counter()
+        S = args.train_samples
+        train_cur = torch.from_numpy(meshes.motor_currents(S, seed=TRAIN_SEED)).to("cuda")
+        ta = torch.zeros((S, plan.N), dtype=torch.float64, device="cuda")
+        tf = torch.zeros((S, plan.C), dtype=torch.float64, device="cuda")
+        tr = torch.zeros(S * RESULT_BYTES, dtype=torch.uint8, device="cuda")
+        plan.solve_batch_device(train_cur.data_ptr(), S, ta.data_ptr(), tf.data_ptr(),
+                                tr.data_ptr(), stream, options, mixed_config())
+        torch.cuda.synchronize()
+        solve_s = time.perf_counter() - t0
+        ws = WarmStart(plan, inputs=3, modes=64, hidden=256, seed=12044)
+        fit = ws.fit_device(S, train_cur.data_ptr(), ta.data_ptr(), stream,
+                            train_settings(steps=4000, batch=128, learning_rate=2e-3))
+        torch.cuda.synchronize()
+        training = dict(samples=S, seed=TRAIN_SEED, solve_s=solve_s,
+                        fit_s=time.perf_counter() - t0 - solve_s, report=fit)
+        print(f"# training: {S} solves {solve_s:.1f} s, fit {training['fit_s']:.1f} s, "
+              f"POD energy {fit['pod_energy_fraction']:.8f}, loss {fit['first_loss']:.3e}"
+              f" -> {fit['final_loss']:.3e}", file=sys.stderr, flush=True)
+        del ta, tf, tr, train_cur
+        cfg = mixed_config()
+        ms, wall = timer.run(lambda: ws.solve_batch_device(
+            B, currents_dev.data_ptr(), currents_dev.data_ptr(), a.data_ptr(), flux.data_ptr(),
+            res.data_ptr(), stream, options, cfg, WARM_BUDGET), args.steps, args.warmup)
+        forms["learned_mixed"] = collect(ms, wall, True)
+        report("learned_mixed", forms["learned_mixed"])
+
+    if "plain_fp64" not in forms or "learned_mixed" not in forms:
         print(json.dumps({"profiling_run": True, "forms": {
-            k: {f: v for f, v in d.items() if f != "a"} for k, d in forms.items()}}), flush=True)
+            k: {f: v for f, v in d.items() if f not in ("a", "flux")} for k, d in forms.items()}}),
+            flush=True)
         return
-    plain, chosen = forms["plain_fp64"], forms["mixed_warm"]
-    diff = (chosen["a"] - plain["a"]).double()
-    rel_l2 = (diff.norm(dim=1) / plain["a"].double().norm(dim=1).clamp_min(1e-300)).max().item()
+    plain, chosen = forms["plain_fp64"], forms["learned_mixed"]
+    diff = chosen["a"] - plain["a"]
+    rel_l2 = (diff.norm(dim=1) / plain["a"].norm(dim=1).clamp_min(1e-300)).max().item()
+    fdiff = ((chosen["flux"] - plain["flux"]).abs().amax(dim=1)
+             / plain["flux"].abs().amax(dim=1).clamp_min(1e-300)).max().item()
     value = chosen["solves_per_s"]
     achieved = chosen["achieved_gbs"]
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(model, currents, options, 30)
+        cpu = cpu_baseline(model, currents, options)
         print(f"# cpu baseline {cpu}", file=sys.stderr, flush=True)
 
+    workload = f"motor_sector N={plan.N} T={plan.T} batch {B}"
     line = {
         "metric": "nonlinear solves/sec @1e-8 resid, 60k nodes batch 256: plain FP64 vs warm+mixed",
         "value": value,
@@ -197,13 +262,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 inner PCG / f64 refinement, assembly and residuals",
+        "dtype": "f32 PCG iterations with f64 residual replacement, f64 assembly / Newton / "
+                 "residuals / flux; bf16 MFMA warm-start network",
         "data": "synthetic: seeded 12-slot motor sector mesh and seeded phase currents",
         "config": {"workload": f"motor_sector 12-slot/4-pole anti-periodic, N={plan.N} nodes, "
                                f"T={plan.T} triangles, batch {B}, U(-8e6,8e6) A/m^2 phase currents",
-                   "form": "FP32 Jacobi PCG in FP64 refinement, PCG started from the current "
-                           "Newton state",
+                   "form": "learned warm start (POD 64 + linear + MLP 3-256-256-64 bf16) -> "
+                           "safeguarded Newton, FP32 PCG with FP64 reliable updates from the "
+                           f"current state, plain-start fallback after {WARM_BUDGET} iterations",
                    "items_passing": chosen["passing"],
+                   "newton_mean": chosen["newton_mean"], "pcg_iterations_mean": chosen["pcg_mean"],
                    "plain_fp64": {"solves_per_s": plain["solves_per_s"], "ms_per_step": plain["ms"],
                                   "items_passing": plain["passing"],
                                   "newton_mean": plain["newton_mean"],
@@ -211,13 +279,12 @@ def main():
                                   "achieved_GBps": plain["achieved_gbs"]},
                    "speedup_vs_plain_fp64": value / max(plain["solves_per_s"], 1e-30),
                    "max_A_rel_l2_vs_plain_fp64": rel_l2,
-                   "newton_mean": chosen["newton_mean"], "pcg_iterations_mean": chosen["pcg_mean"]},
+                   "max_flux_rel_vs_plain_fp64": fdiff,
+                   "training": training},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(
-                         "NewtonKernel<true>",
-                         f"motor_sector N={plan.N} T={plan.T} batch {B}"),
-                     "kernel": "NewtonKernel<mixed>",
+                     "traffic": measured_traffic("learned_mixed", workload),
+                     "kernel": "SafeguardedKernel<true> + NewtonKernel<true> (one learned step)",
                      "pcg_iteration_bytes_per_item": chosen["pcg_iter_bytes"]},
         "cpu_baseline": cpu,
     }

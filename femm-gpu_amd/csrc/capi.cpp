@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <thread>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -11,30 +13,9 @@
 
 #include "device.hpp"
 #include "femm_gpu_amd.h"
+#include "handle.hpp"
 #include "plan.hpp"
 #include "post.hpp"
-
-// Retained copy of the model arrays the postprocess plan needs.
-struct ModelCopy {
-  std::vector<double> node_xy, dirichlet_values;
-  std::vector<int32_t> tri_nodes, tri_mat, dirichlet_nodes, curve_offsets{0};
-  std::vector<femm_material_t> materials;
-  std::vector<femm_node_constraint_t> constraints;
-  femm_model_t view{};
-};
-
-struct femm_plan {
-  femm::HostPlan host;
-  femm::DeviceBuffers dev;
-  ModelCopy model;
-  femm::HostPostPlan post;
-  femm::DevicePostPlan post_dev{};
-  void *post_base = nullptr;
-  bool post_ready = false;
-  // staging for the host-pointer entry points
-  void *io = nullptr;
-  size_t io_bytes = 0;
-};
 
 namespace
 {
@@ -69,18 +50,6 @@ int EnsureIo(femm_plan *plan, size_t bytes)
 
 size_t Round(size_t bytes) { return (bytes + 255) / 256 * 256; }
 
-int EnsureDevice(femm_plan *plan)
-{
-  if (plan->dev.base != nullptr)
-    return FEMM_OK;
-  if (!GpuPresent())
-    return FEMM_GPU_UNAVAILABLE;
-  const int st = femm::UploadPlan(plan->host, &plan->dev);
-  if (st != FEMM_OK)
-    femm::ReleaseDevice(&plan->dev);
-  return st;
-}
-
 femm_newton_options_t DefaultOptions()
 {
   femm_newton_options_t o;
@@ -102,8 +71,65 @@ femm_solve_config_t DefaultConfig()
   c.pcg_warm_start = 0;
   c.verify_equation_residual = 1;
   c.max_refinements = 64;
-  c.inner_relative_tolerance = 1e-4;
+  c.inner_relative_tolerance = 0.1;
   return c;
+}
+
+}  // namespace
+
+namespace femm
+{
+
+int EnsurePlanDevice(femm_plan *plan)
+{
+  if (plan->dev.base != nullptr)
+    return FEMM_OK;
+  if (!GpuPresent())
+    return FEMM_GPU_UNAVAILABLE;
+  const int st = femm::UploadPlan(plan->host, &plan->dev);
+  if (st != FEMM_OK)
+    femm::ReleaseDevice(&plan->dev);
+  return st;
+}
+
+}  // namespace femm
+
+using femm::EnsurePlanDevice;
+
+namespace
+{
+
+// Batched Newton on device pointers, optionally per-item material variants, followed by the
+// bounded recovery when options ask for it.
+int SolveDevice(femm_plan *plan, int32_t items, const int32_t *variant_of_item,
+                const double *currents, const femm_newton_options_t *options,
+                const femm_solve_config_t *config, const double *warm_starts, double *a,
+                double *flux, femm_item_result_t *results, void *stream)
+{
+  if (plan == nullptr || items <= 0 || currents == nullptr || a == nullptr || flux == nullptr ||
+      results == nullptr)
+    return FEMM_INVALID_ARGUMENT;
+  const femm_newton_options_t o = options ? *options : DefaultOptions();
+  const femm_solve_config_t c = config ? *config : DefaultConfig();
+  if (c.precision != FEMM_PRECISION_FP64 && c.precision != FEMM_PRECISION_MIXED)
+    return FEMM_INVALID_ARGUMENT;
+  int st = EnsurePlanDevice(plan);
+  if (st == FEMM_OK)
+    st = femm::EnsureWork(plan->host, items, c.precision == FEMM_PRECISION_MIXED, &plan->dev);
+  if (st != FEMM_OK)
+    return st;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  femm::LaunchExtras x;
+  x.variant_of_item = variant_of_item;
+  st = femm::LaunchNewton(plan->dev, items, currents, o, c, warm_starts, a, flux, results, s, x);
+  if (st != FEMM_OK || !o.recover_nonconvergence || o.max_newton_iterations <= 0)
+    return st;
+  // Bounded recovery of exhausted items from the plain start
+  // (SolveSafeguarded, nonlinear_solver.cu:534-659).
+  x.prior = results;
+  x.prior_mode = femm::kPriorRedoUnconverged;
+  return femm::LaunchSafeguarded(plan->dev, items, currents, o, c, nullptr, a, flux, results, s,
+                                 x);
 }
 
 }  // namespace
@@ -134,6 +160,9 @@ int femm_plan_create(const femm_model_t *model, femm_plan **out)
   mc.dirichlet_values.assign(model->dirichlet_values,
                              model->dirichlet_values + model->dirichlet_count);
   mc.constraints.assign(model->constraints, model->constraints + model->constraint_count);
+  mc.curve_count = model->curve_count;
+  if (model->curve_count > 0)
+    mc.all_curve_offsets.assign(model->curve_offsets, model->curve_offsets + model->curve_count + 1);
   mc.view = *model;
   mc.view.node_xy = mc.node_xy.data();
   mc.view.triangle_nodes = mc.tri_nodes.data();
@@ -211,7 +240,7 @@ int femm_assemble_batch(femm_plan *plan, int32_t items, const double *states,
   if (plan == nullptr || items <= 0 || states == nullptr || currents == nullptr)
     return FEMM_INVALID_ARGUMENT;
   const femm::HostPlan &h = plan->host;
-  int st = EnsureDevice(plan);
+  int st = EnsurePlanDevice(plan);
   if (st == FEMM_OK)
     st = femm::EnsureWork(h, items, false, &plan->dev);
   if (st != FEMM_OK)
@@ -269,7 +298,7 @@ int femm_pcg_batch(femm_plan *plan, int32_t items, const double *values, const d
   if (plan == nullptr || items <= 0 || values == nullptr || rhs == nullptr)
     return FEMM_INVALID_ARGUMENT;
   const femm::HostPlan &h = plan->host;
-  int st = EnsureDevice(plan);
+  int st = EnsurePlanDevice(plan);
   if (st == FEMM_OK)
     st = femm::EnsureWork(h, items, false, &plan->dev);
   if (st != FEMM_OK)
@@ -305,26 +334,82 @@ int femm_newton_batch_device(femm_plan *plan, int32_t items, const double *curre
                              const femm_solve_config_t *config, const double *warm_starts,
                              double *a, double *flux, femm_item_result_t *results, void *stream)
 {
-  if (plan == nullptr || items <= 0 || currents == nullptr || a == nullptr || flux == nullptr ||
-      results == nullptr)
-    return FEMM_INVALID_ARGUMENT;
-  const femm_newton_options_t o = options ? *options : DefaultOptions();
-  const femm_solve_config_t c = config ? *config : DefaultConfig();
-  if (c.precision != FEMM_PRECISION_FP64 && c.precision != FEMM_PRECISION_MIXED)
-    return FEMM_INVALID_ARGUMENT;
-  int st = EnsureDevice(plan);
-  if (st == FEMM_OK)
-    st = femm::EnsureWork(plan->host, items, c.precision == FEMM_PRECISION_MIXED, &plan->dev);
-  if (st != FEMM_OK)
-    return st;
-  return femm::LaunchNewton(plan->dev, items, currents, o, c, warm_starts, a, flux, results,
-                            static_cast<hipStream_t>(stream));
+  return SolveDevice(plan, items, nullptr, currents, options, config, warm_starts, a, flux,
+                     results, stream);
 }
 
-int femm_newton_batch(femm_plan *plan, int32_t items, const double *currents,
-                      const femm_newton_options_t *options, const femm_solve_config_t *config,
-                      const double *warm_starts, double *a, double *bx, double *by, double *flux,
-                      femm_item_result_t *results)
+int femm_newton_batch_variants_device(femm_plan *plan, int32_t items,
+                                      const int32_t *variant_of_item, const double *currents,
+                                      const femm_newton_options_t *options,
+                                      const femm_solve_config_t *config,
+                                      const double *warm_starts, double *a, double *flux,
+                                      femm_item_result_t *results, void *stream)
+{
+  if (plan == nullptr || variant_of_item == nullptr || plan->dev.variants.count == 0)
+    return FEMM_INVALID_ARGUMENT;
+  return SolveDevice(plan, items, variant_of_item, currents, options, config, warm_starts, a,
+                     flux, results, stream);
+}
+
+int femm_plan_set_variants(femm_plan *plan, int32_t count, const femm_model_t *variants)
+{
+  if (plan == nullptr || count < 0 || (count > 0 && variants == nullptr))
+    return FEMM_INVALID_ARGUMENT;
+  int st = EnsurePlanDevice(plan);
+  if (st != FEMM_OK)
+    return st;
+  if ((st = femm::AllocateVariants(plan->host, 0, &plan->dev)) != FEMM_OK || count == 0)
+    return st;
+  femm_model_t base = plan->model.view;
+  base.curve_count = plan->model.curve_count;
+  base.curve_offsets = plan->model.all_curve_offsets.empty() ? nullptr
+                                                            : plan->model.all_curve_offsets.data();
+  if ((st = femm::AllocateVariants(plan->host, count, &plan->dev)) != FEMM_OK)
+    return st;
+  // Material parts are built on host threads in bounded chunks and copied as they finish.
+  const int workers = std::max(1, std::min<int>(16, static_cast<int>(std::thread::hardware_concurrency())));
+  const int chunk = 4 * workers;
+  std::vector<femm::HostPlan> parts(chunk);
+  std::vector<int> codes(chunk);
+  for (int first = 0; first < count; first += chunk) {
+    const int m = std::min(chunk, count - first);
+    std::atomic<int> next{0};
+    auto work = [&]() {
+      for (int k = next++; k < m; k = next++) {
+        const femm_model_t *v = variants + first + k;
+        codes[k] = femm::SameTopology(&base, v)
+                       ? femm::BuildHostPlan(v, &parts[k], nullptr, &plan->host)
+                       : FEMM_INVALID_ARGUMENT;
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < std::min(workers, m); ++t)
+      pool.emplace_back(work);
+    work();
+    for (std::thread &t : pool)
+      t.join();
+    for (int k = 0; k < m; ++k) {
+      if (codes[k] == FEMM_OK)
+        codes[k] = femm::CopyVariant(plan->host, first + k, parts[k], &plan->dev);
+      if (codes[k] != FEMM_OK) {
+        (void)femm::AllocateVariants(plan->host, 0, &plan->dev);
+        return codes[k];
+      }
+    }
+  }
+  return hipDeviceSynchronize() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
+}
+
+}  // extern "C"
+
+namespace
+{
+
+// Host-pointer batched solve: stage inputs, run on the device, copy results back.
+int SolveHost(femm_plan *plan, int32_t items, const int32_t *variant_of_item,
+              const double *currents, const femm_newton_options_t *options,
+              const femm_solve_config_t *config, const double *warm_starts, double *a,
+              double *bx, double *by, double *flux, femm_item_result_t *results)
 {
   if (plan == nullptr || items <= 0 || currents == nullptr || a == nullptr || flux == nullptr ||
       results == nullptr)
@@ -335,9 +420,10 @@ int femm_newton_batch(femm_plan *plan, int32_t items, const double *currents,
   const size_t sr = Round(sizeof(femm_item_result_t) * B);
   const size_t sb = (bx && by) ? Round(sizeof(double) * B * h.T) : 0;
   const size_t sw = warm_starts ? sa : 0;
-  int st = EnsureDevice(plan);
+  const size_t sv = variant_of_item ? Round(sizeof(int32_t) * B) : 0;
+  int st = EnsurePlanDevice(plan);
   if (st == FEMM_OK)
-    st = EnsureIo(plan, sa + 2 * sc + sr + 2 * sb + sw);
+    st = EnsureIo(plan, sa + 2 * sc + sr + 2 * sb + sw + sv);
   if (st != FEMM_OK)
     return st;
   char *io = static_cast<char *>(plan->io);
@@ -348,12 +434,14 @@ int femm_newton_batch(femm_plan *plan, int32_t items, const double *currents,
   double *d_bx = sb ? reinterpret_cast<double *>(io + sa + 2 * sc + sr) : nullptr;
   double *d_by = sb ? reinterpret_cast<double *>(io + sa + 2 * sc + sr + sb) : nullptr;
   double *d_warm = sw ? reinterpret_cast<double *>(io + sa + 2 * sc + sr + 2 * sb) : nullptr;
+  int32_t *d_var = sv ? reinterpret_cast<int32_t *>(io + sa + 2 * sc + sr + 2 * sb + sw) : nullptr;
   if (hipMemcpy(d_cur, currents, sizeof(double) * B * h.C, hipMemcpyHostToDevice) != hipSuccess ||
       (d_warm && hipMemcpy(d_warm, warm_starts, sizeof(double) * B * h.N,
-                           hipMemcpyHostToDevice) != hipSuccess))
+                           hipMemcpyHostToDevice) != hipSuccess) ||
+      (d_var && hipMemcpy(d_var, variant_of_item, sizeof(int32_t) * B, hipMemcpyHostToDevice) !=
+                    hipSuccess))
     return FEMM_INTERNAL_ERROR;
-  st = femm_newton_batch_device(plan, items, d_cur, options, config, d_warm, d_a, d_flux, d_res,
-                                nullptr);
+  st = SolveDevice(plan, items, d_var, d_cur, options, config, d_warm, d_a, d_flux, d_res, nullptr);
   if (st != FEMM_OK)
     return st;
   if (sb && (st = femm::LaunchFinalize(plan->dev, items, d_a, d_bx, d_by, nullptr, nullptr)) !=
@@ -370,6 +458,30 @@ int femm_newton_batch(femm_plan *plan, int32_t items, const double *currents,
   return FEMM_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int femm_newton_batch(femm_plan *plan, int32_t items, const double *currents,
+                      const femm_newton_options_t *options, const femm_solve_config_t *config,
+                      const double *warm_starts, double *a, double *bx, double *by, double *flux,
+                      femm_item_result_t *results)
+{
+  return SolveHost(plan, items, nullptr, currents, options, config, warm_starts, a, bx, by, flux,
+                   results);
+}
+
+int femm_newton_batch_variants(femm_plan *plan, int32_t items, const int32_t *variant_of_item,
+                               const double *currents, const femm_newton_options_t *options,
+                               const femm_solve_config_t *config, const double *warm_starts,
+                               double *a, double *flux, femm_item_result_t *results)
+{
+  if (plan == nullptr || variant_of_item == nullptr || plan->dev.variants.count == 0)
+    return FEMM_INVALID_ARGUMENT;
+  return SolveHost(plan, items, variant_of_item, currents, options, config, warm_starts, a,
+                   nullptr, nullptr, flux, results);
+}
+
 int femm_finalize_batch(femm_plan *plan, int32_t items, const double *states, double *bx,
                         double *by, double *flux)
 {
@@ -379,7 +491,7 @@ int femm_finalize_batch(femm_plan *plan, int32_t items, const double *states, do
   const size_t B = static_cast<size_t>(items);
   const size_t sa = Round(sizeof(double) * B * h.N), st_ = Round(sizeof(double) * B * h.T);
   const size_t sc = Round(sizeof(double) * B * std::max(h.C, 1));
-  int st = EnsureDevice(plan);
+  int st = EnsurePlanDevice(plan);
   if (st == FEMM_OK)
     st = EnsureIo(plan, sa + 2 * st_ + sc);
   if (st != FEMM_OK)
@@ -406,7 +518,7 @@ int femm_plan_set_postprocess(femm_plan *plan, const femm_postprocess_options_t 
 {
   if (plan == nullptr || options == nullptr)
     return FEMM_INVALID_ARGUMENT;
-  int st = EnsureDevice(plan);
+  int st = EnsurePlanDevice(plan);
   if (st != FEMM_OK)
     return st;
   if (plan->post_base) {

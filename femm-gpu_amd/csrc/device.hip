@@ -35,6 +35,15 @@ constexpr int kWaves = kBlockThreads / 64;
 
 // ----------------------------------------------------------------------------- reductions
 
+// The same value in every lane, made wave-uniform (held in scalar registers).
+__device__ inline double Uniform(double x)
+{
+  const uint64_t bits = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(bits));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(bits >> 32));
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
 struct BlockReducer {
   double *lds;   // [2][kWaves][4]
   int buf;
@@ -65,7 +74,7 @@ struct BlockReducer {
 #pragma unroll
       for (int w = 1; w < kWaves; ++w)
         s += b[w * 4 + k];
-      v[k] = s;
+      v[k] = Uniform(s);
     }
     buf ^= 1;
   }
@@ -85,7 +94,7 @@ struct BlockReducer {
     for (int w = 1; w < kWaves; ++w)
       s = fmax(s, b[w * 4]);
     buf ^= 1;
-    return s;
+    return Uniform(s);
   }
 
   __device__ int AnyTrue(bool local)
@@ -266,45 +275,113 @@ __device__ bool GatherRhs(const DevicePlan &P, const double *elem, const double 
 
 // y = A v over SELL-64 (lane = row of the slice); returns the local sum of
 // w[row] * y[row] when w is given.  A workgroup owns one item, so the memory
-// parallelism of one CU must come from within each wave: every slice of width
-// <= kUnrolledWidth issues all of its column and value loads before the first
-// gather, then all gathers before the first FMA.  The sum order over a row's
-// entries is the CSR order either way.
+// parallelism of one CU must come from within each wave.  Slices of width
+// <= kUnrolledWidth are software-pipelined: the gathers of slice s are issued
+// first, then the column and value loads of the wave's next slice, so waiting for
+// the gathers (vector loads complete in order) leaves the next slice's loads in
+// flight; every slice costs one memory round trip instead of two.  The sum order
+// over a row's entries is the CSR order either way.
 constexpr int kUnrolledWidth = 10;
 
-template <typename T, typename V>
+template <typename T>
+struct SliceLoads {
+  int c[kUnrolledWidth];
+  T a[kUnrolledWidth];
+  int width;
+};
+
+// Branch-free loads of one slice: entries past the slice width repeat the last column
+// (a valid address in the same cache line) with a zero value, so the whole slice is one
+// straight-line run of loads and later adds only +-0 terms.
+template <typename T>
+__device__ inline void LoadSlice(const DevicePlan &P, const T *__restrict__ vals, int s, int lane,
+                                 SliceLoads<T> &L)
+{
+  const int base = P.slice_ptr[s] + lane;
+  L.width = P.slice_width[s];
+  const int last = L.width - 1;   // every slice holds a diagonal, so width >= 1
+#pragma unroll
+  for (int k = 0; k < kUnrolledWidth; ++k) {
+    const int pos = base + max(min(k, last), 0) * kSliceRows;
+    L.c[k] = P.sell_cols[pos];
+    const T value = vals[pos];
+    L.a[k] = k <= last ? value : T(0);
+  }
+}
+
+// kPipeline = false: the plain two-round-trip loop, for rarely used products (FP64
+// residuals inside the mixed form, equation residuals) where registers matter more.
+template <typename T, typename V, bool kPipeline = true>
 __device__ double Spmv(const DevicePlan &P, const T *__restrict__ vals, const V *__restrict__ v,
                        V *__restrict__ y, const V *__restrict__ w)
 {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int *__restrict__ cols = P.sell_cols;
   double local = 0.0;
-  for (int s = wave; s < P.nslices; s += kWaves) {
-    const int base = P.slice_ptr[s] + lane, width = P.slice_width[s];
-    T sum = 0;
-    if (width <= kUnrolledWidth) {
-      int c[kUnrolledWidth];
-      T a[kUnrolledWidth];
+  if constexpr (!kPipeline) {
+    for (int s = wave; s < P.nslices; s += kWaves) {
+      const int base = P.slice_ptr[s] + lane, width = P.slice_width[s];
+      T sum = 0;
+      if (width <= kUnrolledWidth) {
+        int c[kUnrolledWidth];
+        T a[kUnrolledWidth];
 #pragma unroll
-      for (int k = 0; k < kUnrolledWidth; ++k)
-        if (k < width) {
-          c[k] = cols[base + k * kSliceRows];
-          a[k] = vals[base + k * kSliceRows];
+        for (int k = 0; k < kUnrolledWidth; ++k)
+          if (k < width) {
+            c[k] = P.sell_cols[base + k * kSliceRows];
+            a[k] = vals[base + k * kSliceRows];
+          }
+        V g[kUnrolledWidth];
+#pragma unroll
+        for (int k = 0; k < kUnrolledWidth; ++k)
+          if (k < width)
+            g[k] = v[c[k]];
+#pragma unroll
+        for (int k = 0; k < kUnrolledWidth; ++k)
+          if (k < width)
+            sum += a[k] * static_cast<T>(g[k]);
+      } else {
+        for (int k = 0; k < width; ++k) {
+          const int pos = base + k * kSliceRows;
+          sum += vals[pos] * static_cast<T>(v[P.sell_cols[pos]]);
         }
+      }
+      const int row = s * kSliceRows + lane;
+      if (row < P.n) {
+        y[row] = static_cast<V>(sum);
+        if (w)
+          local += static_cast<double>(w[row]) * static_cast<double>(sum);
+      }
+    }
+    return local;
+  }
+  SliceLoads<T> cur;
+  cur.width = 0;
+  if (wave < P.nslices)
+    LoadSlice(P, vals, wave, lane, cur);
+  for (int s = wave; s < P.nslices; s += kWaves) {
+    T sum = 0;
+    const int next = s + kWaves;
+    if (cur.width <= kUnrolledWidth) {
       V g[kUnrolledWidth];
 #pragma unroll
       for (int k = 0; k < kUnrolledWidth; ++k)
-        if (k < width)
-          g[k] = v[c[k]];
+        g[k] = v[cur.c[k]];
+      SliceLoads<T> nxt;
+      // The last slice prefetches itself again (in bounds, never used).
+      LoadSlice(P, vals, next < P.nslices ? next : s, lane, nxt);
 #pragma unroll
       for (int k = 0; k < kUnrolledWidth; ++k)
-        if (k < width)
-          sum += a[k] * static_cast<T>(g[k]);
+        sum += cur.a[k] * static_cast<T>(g[k]);
+      cur = nxt;
     } else {
-      for (int k = 0; k < width; ++k) {
+      const int base = P.slice_ptr[s] + lane;
+      for (int k = 0; k < cur.width; ++k) {
         const int pos = base + k * kSliceRows;
-        sum += vals[pos] * static_cast<T>(v[cols[pos]]);
+        sum += vals[pos] * static_cast<T>(v[P.sell_cols[pos]]);
       }
+      cur.width = 0;
+      if (next < P.nslices)
+        LoadSlice(P, vals, next, lane, cur);
     }
     const int row = s * kSliceRows + lane;
     if (row < P.n) {
@@ -333,6 +410,7 @@ struct LinearInfo {
 // the recomputed true residual with restart otherwise.  With warm != 0 the
 // iteration starts from the x already in `x` (unscaled) instead of zero; the
 // acceptance test ||b - A x|| <= tol ||b|| is unchanged.
+template <bool kPipeline = true>
 __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const double *inv,
                             const double *rhs, double *x, double *r, double *p, double *ap,
                             double tol, int maxit, bool warm, BlockReducer &R)
@@ -372,7 +450,7 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
     for (int i = threadIdx.x; i < n; i += kBlockThreads)
       x[i] = x[i] / scale;
     __syncthreads();
-    Spmv(P, vals, x, ap, static_cast<const double *>(nullptr));
+    Spmv<double, double, false>(P, vals, x, ap, static_cast<const double *>(nullptr));
     __syncthreads();
   }
   double s0[3] = {0.0, 0.0, 0.0};
@@ -406,7 +484,7 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
     return info;
   }
   for (int it = 0; it < maxit; ++it) {
-    double pap[1] = {Spmv(P, vals, p, ap, p)};
+    double pap[1] = {Spmv<double, double, kPipeline>(P, vals, p, ap, p)};
     R.Sum(pap);
     if (!isfinite(pap[0]) || !isfinite(rho)) {
       info.status = FEMM_NUMERICAL_NONFINITE;
@@ -452,7 +530,7 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
     info.residual = sqrt(s1[0]);
     if (info.residual <= threshold) {
       // True residual on the recomputed b/scale - A x (pcg_kernels.cu:192-270).
-      Spmv(P, vals, x, ap, static_cast<const double *>(nullptr));
+      Spmv<double, double, false>(P, vals, x, ap, static_cast<const double *>(nullptr));
       __syncthreads();
       double tr[1] = {0.0};
       for (int i = threadIdx.x; i < n; i += kBlockThreads) {
@@ -518,7 +596,7 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
     __syncthreads();
   }
   // Iteration cap: report the true residual (pcg_kernels.cu:302-343).
-  Spmv(P, vals, x, ap, static_cast<const double *>(nullptr));
+  Spmv<double, double, false>(P, vals, x, ap, static_cast<const double *>(nullptr));
   __syncthreads();
   double fr[1] = {0.0};
   for (int i = threadIdx.x; i < n; i += kBlockThreads) {
@@ -536,36 +614,119 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
 
 // ----------------------------------------------------------------------------- mixed precision
 
-// FP32 Jacobi PCG for A32 d = r (r given in FP64, scaled by rscale before the
-// FP32 conversion so small residuals do not underflow).  Dot products are
-// accumulated in FP64.  Stops when the recurrence residual drops below
-// eps * ||r|| or at maxit; a breakdown (p'Ap <= 0 from FP32 rounding) stops
-// early with the progress made so far.  Returns the iteration count; d32 holds
-// the correction in units of rscale.
-__device__ int PcgInner32(const DevicePlan &P, const float *vals32, const float *inv32,
-                          const double *r64, double rscale, float *d32, float *r32, float *p32,
-                          float *ap32, double eps, int maxit, BlockReducer &R)
+struct RefinedInfo {
+  LinearInfo linear;
+  int refinements;   // FP64 true-residual evaluations
+};
+
+// r = b - A64 x in FP64; returns ||r||_2 and max|r| through the reducer.
+__device__ void TrueResidual(const DevicePlan &P, const double *vals, const double *rhs,
+                             const double *x, double *r, double *ap, double *norm, double *amax,
+                             BlockReducer &R)
 {
-  const int n = P.n;
-  double s0[2] = {0.0, 0.0};
-  for (int i = threadIdx.x; i < n; i += kBlockThreads) {
-    const float ri = static_cast<float>(r64[i] / rscale);
-    const float zi = ri * inv32[i];
-    r32[i] = ri;
-    p32[i] = zi;
-    d32[i] = 0.0f;
-    s0[0] += static_cast<double>(ri) * ri;
-    s0[1] += static_cast<double>(ri) * zi;
+  Spmv<double, double, false>(P, vals, x, ap, static_cast<const double *>(nullptr));
+  __syncthreads();
+  double rr[1] = {0.0};
+  double rmax = 0.0;
+  for (int i = threadIdx.x; i < P.n; i += kBlockThreads) {
+    const double ri = rhs[i] - ap[i];
+    r[i] = ri;
+    rr[0] += ri * ri;
+    rmax = fmax(rmax, fabs(ri));
   }
-  R.Sum(s0);
-  double rho = s0[1];
-  const double stop2 = eps * eps * s0[0];
+  R.Sum(rr);
+  *amax = R.Max(rmax);
+  *norm = sqrt(rr[0]);
+}
+
+// Mixed-precision Jacobi PCG with reliable updates (Sleijpen and van der Vorst,
+// "Reliable updated residuals in hybrid Bi-CG methods", Computing 56, 1996): the
+// iteration runs in FP32 (values, SpMV, preconditioner, vectors; dot products
+// accumulated in FP64) on a correction d relative to the FP64 iterate x.  Whenever
+// the FP32 recurrence residual has dropped by `delta` since the last FP64 residual,
+// or claims convergence, x += sigma d and the FP64 true residual b - A64 x replaces
+// the recurrence residual (FP32 vectors are kept in units of sigma = max|r| at the last
+// replacement, so they never underflow); the search direction is kept, so the Krylov
+// space is not restarted.  Acceptance is the reference's test on the FP64 true residual,
+// ||b - A x|| <= tol ||b||.  A breakdown in FP32, a replacement that fails to halve the
+// true residual, or running out of replacements finishes the solve in FP64 PCG
+// from the current x.
+__device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, const double *inv,
+                                     const float *vals32, const float *inv32, const double *rhs,
+                                     double *x, double *r, double *p, double *ap, float *d32,
+                                     float *r32, float *p32, float *ap32, double tol, int maxit,
+                                     bool warm, int max_updates, double delta, BlockReducer &R)
+{
+  RefinedInfo out{{FEMM_INTERNAL_ERROR, 0, INFINITY}, 0};
+  const int n = P.n;
+  if (maxit < 0 || !(tol > 0.0) || !(tol < 1.0) || !isfinite(tol)) {
+    out.linear.status = FEMM_INVALID_ARGUMENT;
+    return out;
+  }
+  double b2[1] = {0.0};
+  bool finite = true;
+  for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+    b2[0] += rhs[i] * rhs[i];
+    finite = finite && isfinite(rhs[i]);
+    if (!warm)
+      x[i] = 0.0;
+  }
+  R.Sum(b2);
+  if (R.AnyTrue(!finite)) {
+    out.linear.status = FEMM_NUMERICAL_NONFINITE;
+    return out;
+  }
+  const double bnorm = sqrt(b2[0]);
+  if (bnorm == 0.0) {
+    for (int i = threadIdx.x; i < n; i += kBlockThreads)
+      x[i] = 0.0;
+    __syncthreads();
+    out.linear = {FEMM_OK, 0, 0.0};
+    return out;
+  }
+  const double threshold = tol * bnorm;
+  if (!(delta > 0.0) || !(delta < 1.0))
+    delta = 0.1;
+  double rnorm, rmax;
+  TrueResidual(P, vals, rhs, x, r, ap, &rnorm, &rmax, R);
+  out.refinements = 1;
+  out.linear.residual = rnorm;
+  if (!isfinite(rnorm)) {
+    out.linear.status = FEMM_NUMERICAL_NONFINITE;
+    return out;
+  }
+  if (rnorm <= threshold) {
+    out.linear.status = FEMM_OK;
+    return out;
+  }
+  double sigma = rmax;
+  double rho, rr;
+  {
+    double s0[2] = {0.0, 0.0};
+    for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+      const float ri = static_cast<float>(r[i] / sigma);
+      const float zi = ri * inv32[i];
+      r32[i] = ri;
+      p32[i] = zi;
+      d32[i] = 0.0f;
+      s0[0] += static_cast<double>(ri) * ri;
+      s0[1] += static_cast<double>(ri) * zi;
+    }
+    R.Sum(s0);
+    rr = s0[0];
+    rho = s0[1];
+  }
+  double rr_ref = rr;           // ||r32||^2 at the last replacement
+  double last_true = rnorm;     // FP64 true residual at the last replacement
+  bool fallback = false;
   int it = 0;
-  for (; it < maxit; ++it) {
+  while (it < maxit) {
     double pap[1] = {Spmv(P, vals32, p32, ap32, p32)};
     R.Sum(pap);
-    if (!(pap[0] > 0.0) || !(rho > 0.0) || !isfinite(pap[0]))
+    if (!(pap[0] > 0.0) || !(rho > 0.0) || !isfinite(pap[0]) || !isfinite(rho)) {
+      fallback = true;
       break;
+    }
     const float alpha = static_cast<float>(rho / pap[0]);
     double s1[2] = {0.0, 0.0};
     for (int i0 = threadIdx.x; i0 < n; i0 += kBlockThreads * kVecUnroll) {
@@ -594,11 +755,59 @@ __device__ int PcgInner32(const DevicePlan &P, const float *vals32, const float 
       }
     }
     R.Sum(s1);
-    if (!(s1[0] > stop2) || !(s1[1] > 0.0) || !isfinite(s1[1])) {
-      ++it;
+    ++it;
+    if (!isfinite(s1[0]) || !isfinite(s1[1])) {
+      fallback = true;
       break;
     }
-    const float beta = static_cast<float>(s1[1] / rho);
+    double rho_next = s1[1];
+    const bool claims = s1[0] * sigma * sigma <= threshold * threshold;
+    if (claims || s1[0] <= delta * delta * rr_ref) {
+      // Reliable update: fold the correction into x and replace the residual.
+      for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+        x[i] += sigma * static_cast<double>(d32[i]);
+        d32[i] = 0.0f;
+      }
+      __syncthreads();
+      TrueResidual(P, vals, rhs, x, r, ap, &rnorm, &rmax, R);
+      ++out.refinements;
+      out.linear.residual = rnorm;
+      if (!isfinite(rnorm)) {
+        out.linear.iterations = it;
+        out.linear.status = FEMM_NUMERICAL_NONFINITE;
+        return out;
+      }
+      if (rnorm <= threshold) {
+        out.linear.iterations = it;
+        out.linear.status = FEMM_OK;
+        return out;
+      }
+      if (rnorm > 0.5 * last_true || out.refinements > max_updates) {
+        fallback = true;
+        break;
+      }
+      last_true = rnorm;
+      // Rescale to the new residual unit; the direction carries over.
+      const double q = sigma / rmax;
+      sigma = rmax;
+      double s2[2] = {0.0, 0.0};
+      for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+        const float ri = static_cast<float>(r[i] / sigma);
+        r32[i] = ri;
+        p32[i] = static_cast<float>(q * static_cast<double>(p32[i]));
+        s2[0] += static_cast<double>(ri) * ri;
+        s2[1] += static_cast<double>(ri) * (ri * inv32[i]);
+      }
+      R.Sum(s2);
+      rho *= q * q;
+      rr_ref = s2[0];
+      rho_next = s2[1];
+    }
+    if (!(rho_next > 0.0)) {
+      fallback = true;
+      break;
+    }
+    const float beta = static_cast<float>(rho_next / rho);
     for (int i0 = threadIdx.x; i0 < n; i0 += kBlockThreads * kVecUnroll) {
       float rv[kVecUnroll], iv[kVecUnroll], pv[kVecUnroll];
 #pragma unroll
@@ -617,101 +826,21 @@ __device__ int PcgInner32(const DevicePlan &P, const float *vals32, const float 
           p32[i] = rv[u] * iv[u] + beta * pv[u];
       }
     }
-    rho = s1[1];
+    rho = rho_next;
     __syncthreads();
   }
+  // Fold any pending correction, then finish in FP64 from x (or report the cap).
+  for (int i = threadIdx.x; i < n; i += kBlockThreads)
+    x[i] += sigma * static_cast<double>(d32[i]);
   __syncthreads();
-  return it;
-}
-
-struct RefinedInfo {
-  LinearInfo linear;
-  int refinements;
-};
-
-// FP64 iterative refinement around FP32 PCG: r = b - A64 x in FP64 (the
-// acceptance test ||b - A x|| <= tol ||b|| is evaluated exactly as in the FP64
-// path), correction from the FP32 solve, x += d.  Starts from the x in `x`
-// (warm) or zero.  If a refinement fails to halve the FP64 residual, the solve
-// finishes in FP64 PCG from the current x.
-__device__ RefinedInfo RefinedSolve(const DevicePlan &P, const double *vals, const double *inv,
-                                    const float *vals32, const float *inv32, const double *rhs,
-                                    double *x, double *r, double *p, double *ap, float *d32,
-                                    float *r32, float *p32, float *ap32, double tol,
-                                    int maxit, bool warm, int max_refinements, double inner_eps,
-                                    BlockReducer &R)
-{
-  RefinedInfo out{{FEMM_INTERNAL_ERROR, 0, INFINITY}, 0};
-  const int n = P.n;
-  double b2[1] = {0.0};
-  bool finite = true;
-  for (int i = threadIdx.x; i < n; i += kBlockThreads) {
-    b2[0] += rhs[i] * rhs[i];
-    finite = finite && isfinite(rhs[i]);
-    if (!warm)
-      x[i] = 0.0;
-  }
-  R.Sum(b2);
-  if (R.AnyTrue(!finite)) {
-    out.linear.status = FEMM_NUMERICAL_NONFINITE;
+  out.linear.iterations = it;
+  const LinearInfo li = Pcg64<false>(P, vals, inv, rhs, x, r, p, ap, tol,
+                                     fallback ? max(0, maxit - it) : 0, true, R);
+  out.linear.iterations += li.iterations;
+  out.linear.residual = li.residual;
+  out.linear.status = li.status;
+  if (li.status == FEMM_OK || fallback)
     return out;
-  }
-  const double bnorm = sqrt(b2[0]);
-  if (bnorm == 0.0) {
-    for (int i = threadIdx.x; i < n; i += kBlockThreads)
-      x[i] = 0.0;
-    __syncthreads();
-    out.linear = {FEMM_OK, 0, 0.0};
-    return out;
-  }
-  double prev = INFINITY;
-  for (int k = 0; k <= max_refinements; ++k) {
-    Spmv(P, vals, x, ap, static_cast<const double *>(nullptr));
-    __syncthreads();
-    double rr[1] = {0.0};
-    double rmax = 0.0;
-    for (int i = threadIdx.x; i < n; i += kBlockThreads) {
-      const double ri = rhs[i] - ap[i];
-      r[i] = ri;
-      rr[0] += ri * ri;
-      rmax = fmax(rmax, fabs(ri));
-    }
-    R.Sum(rr);
-    const double rscale = R.Max(rmax);
-    const double rnorm = sqrt(rr[0]);
-    out.refinements = k + 1;
-    out.linear.residual = rnorm;
-    if (!isfinite(rnorm)) {
-      out.linear.status = FEMM_NUMERICAL_NONFINITE;
-      return out;
-    }
-    if (rnorm <= tol * bnorm) {
-      out.linear.status = FEMM_OK;
-      return out;
-    }
-    if (k == max_refinements || rnorm > 0.5 * prev) {
-      // Stalled in FP32 (or out of refinements): finish in FP64 from this x.
-      const LinearInfo li = Pcg64(P, vals, inv, rhs, x, r, p, ap, tol,
-                                  max(0, maxit - out.linear.iterations), true, R);
-      out.linear.status = li.status;
-      out.linear.iterations += li.iterations;
-      out.linear.residual = li.residual;
-      return out;
-    }
-    prev = rnorm;
-    // Aim a little below the remaining gap, never tighter than FP32 can deliver.
-    const double eps = fmax(inner_eps, 0.5 * tol * bnorm / rnorm);
-    const int iters = PcgInner32(P, vals32, inv32, r, rscale, d32, r32, p32, ap32, eps,
-                                 maxit - out.linear.iterations, R);
-    out.linear.iterations += iters;
-    for (int i = threadIdx.x; i < n; i += kBlockThreads)
-      x[i] += rscale * static_cast<double>(d32[i]);
-    __syncthreads();
-    if (out.linear.iterations >= maxit) {
-      out.linear.status = FEMM_LINEAR_SOLVE_NOT_CONVERGED;
-      return out;
-    }
-  }
   out.linear.status = FEMM_LINEAR_SOLVE_NOT_CONVERGED;
   return out;
 }
@@ -719,7 +848,7 @@ __device__ RefinedInfo RefinedSolve(const DevicePlan &P, const double *vals, con
 // ----------------------------------------------------------------------------- Newton
 
 struct ItemPointers {
-  double *elem, *vals, *rhs, *inv, *x, *r, *p, *ap;
+  double *elem, *vals, *rhs, *inv, *x, *r, *p, *ap, *trial;
   float *vals32, *r32, *p32, *ap32, *d32, *inv32;
 };
 
@@ -735,6 +864,7 @@ __device__ ItemPointers ItemWork(const DevicePlan &P, const DeviceWork &W, int i
   ip.r = W.r + item * n;
   ip.p = W.p + item * n;
   ip.ap = W.ap + item * n;
+  ip.trial = W.trial + static_cast<size_t>(item) * P.N;
   ip.vals32 = mixed ? W.vals32 + item * s : nullptr;
   ip.r32 = mixed ? W.r32 + item * n : nullptr;
   ip.p32 = mixed ? W.p32 + item * n : nullptr;
@@ -758,13 +888,14 @@ __device__ bool AssembleItem(const DevicePlan &P, const ItemPointers &w, const d
 
 // ||K(A) A - b(A)|| over the free rows, with A taken at each free root
 // (solvers/nonlinear_solver.cu:549-558).  Requires a fresh assembly at `a`.
+// `xf` receives A at the free roots (n values).
 __device__ double EquationNorm(const DevicePlan &P, const ItemPointers &w, const double *a,
-                               BlockReducer &R)
+                               BlockReducer &R, double *xf)
 {
   for (int i = threadIdx.x; i < P.n; i += kBlockThreads)
-    w.x[i] = a[P.free_nodes[i]];
+    xf[i] = a[P.free_nodes[i]];
   __syncthreads();
-  Spmv(P, w.vals, w.x, w.ap, static_cast<const double *>(nullptr));
+  Spmv<double, double, false>(P, w.vals, xf, w.ap, static_cast<const double *>(nullptr));
   __syncthreads();
   double s[1] = {0.0};
   for (int i = threadIdx.x; i < P.n; i += kBlockThreads) {
@@ -778,7 +909,46 @@ __device__ double EquationNorm(const DevicePlan &P, const ItemPointers &w, const
 struct NewtonArgs {
   femm_newton_options_t options;
   femm_solve_config_t config;
+  int32_t prior_mode;
+  femm_newton_options_t warm_options;   // kModeLearned: the warm attempt's options
 };
+
+// Points the item's plan at its material variant; false for an index outside the set.
+__device__ bool ApplyVariant(DevicePlan &P, const DeviceVariants &V, const int32_t *variant_of_item,
+                             int item)
+{
+  if (variant_of_item == nullptr)
+    return true;
+  const int v = variant_of_item[item];
+  if (v < 0 || v >= V.count)
+    return false;
+  const size_t T = P.T, M = V.mats, K = V.knots, n = P.n;
+  P.tri_mat = V.tri_mat + v * T;
+  P.tri_circuit = V.tri_circuit + v * T;
+  P.tri_flux = V.tri_flux + v * 3 * T;
+  P.mat_nu0 = V.mat_nu0 + v * M;
+  P.mat_alpha = V.mat_alpha + v * M;
+  P.mat_cold = V.mat_cold + v * M;
+  P.mat_curve_off = V.mat_curve_off + v * M;
+  P.mat_curve_cnt = V.mat_curve_cnt + v * M;
+  P.curve_b = V.curve_b + v * K;
+  P.curve_h = V.curve_h + v * K;
+  P.curve_s = V.curve_s + v * K;
+  P.pm_rhs = V.pm_rhs + v * n;
+  P.circuit_rhs = V.circuit_rhs + v * static_cast<size_t>(max(P.C, 1)) * n;
+  P.any_nonlinear = V.any_nonlinear[v];
+  return true;
+}
+
+// True when a launch with prior results leaves this item as the earlier attempt left it.
+__device__ bool KeepPrior(const femm_item_result_t &earlier, int32_t mode)
+{
+  if (mode == kPriorRedoFailed)
+    return earlier.status == FEMM_OK;
+  if (mode == kPriorRedoUnconverged)
+    return earlier.status != FEMM_NONLINEAR_SOLVE_NOT_CONVERGED;
+  return false;
+}
 
 // Inexact-Newton inner tolerance (physics/nonlinear_model.cpp:166-174).
 __device__ double LinearTolerance(const femm_newton_options_t &o, bool adaptive, double prev)
@@ -807,115 +977,157 @@ __device__ void WriteFlux(const DevicePlan &P, const double *a, double *flux, Bl
   }
 }
 
-template <bool kMixed>
-__global__ __launch_bounds__(kBlockThreads) void NewtonKernel(DevicePlan P, DeviceWork W,
-                                                              const double *currents,
-                                                              NewtonArgs args, const double *warm,
-                                                              double *A, double *flux_out,
-                                                              femm_item_result_t *results)
+// Result of an item before its solve starts.
+__device__ femm_item_result_t FreshResult()
 {
-  __shared__ double lds[2 * kWaves * 4];
-  BlockReducer R{lds, 0};
-  const int item = blockIdx.x;
-  const femm_newton_options_t &o = args.options;
-  const femm_solve_config_t &cfg = args.config;
-  constexpr bool mixed = kMixed;
-  const ItemPointers w = ItemWork(P, W, item, mixed);
-  double *a = A + static_cast<size_t>(item) * P.N;
-  const double *cur = currents + static_cast<size_t>(item) * P.C;
-  const bool has_warm = warm != nullptr;
-  const bool adaptive = o.adaptive_linear_tolerance && P.any_nonlinear;
-
   femm_item_result_t res{};
   res.status = FEMM_INVALID_ARGUMENT;
   res.relative_step = INFINITY;
   res.equation_residual = INFINITY;
   res.last_linear_tolerance = NAN;
   res.linear_residual = INFINITY;
-  bool inputs_ok = o.relative_tolerance > 0.0 && isfinite(o.relative_tolerance) &&
-                   o.max_newton_iterations >= 0;
-  for (int c = 0; c < P.C; ++c)
-    inputs_ok = inputs_ok && isfinite(cur[c]);
+  return res;
+}
 
-  // Start state: prescribed values on every fixed node (signed roots expanded),
-  // zero or the warm start elsewhere.
-  for (int v = threadIdx.x; v < P.N; v += kBlockThreads)
-    a[v] = P.node_to_free[v] < 0 ? P.boundary[v]
-                                 : (has_warm ? warm[static_cast<size_t>(item) * P.N + v] : 0.0);
+__device__ bool OptionsValid(const femm_newton_options_t &o)
+{
+  return o.relative_tolerance > 0.0 && isfinite(o.relative_tolerance) &&
+         o.max_newton_iterations >= 0;
+}
+
+// One linear solve of the current assembly into w.x (x0 = w.x when warm).
+template <bool kMixed>
+__device__ LinearInfo LinearSolve(const DevicePlan &P, const ItemPointers &w,
+                                  const femm_solve_config_t &cfg, double tol, int maxit,
+                                  femm_item_result_t &res, BlockReducer &R)
+{
+  LinearInfo li;
+  if constexpr (kMixed) {
+    const RefinedInfo ri = ReliableSolve(P, w.vals, w.inv, w.vals32, w.inv32, w.rhs, w.x, w.r, w.p,
+                                         w.ap, w.d32, w.r32, w.p32, w.ap32, tol, maxit,
+                                         cfg.pcg_warm_start != 0, cfg.max_refinements,
+                                         cfg.inner_relative_tolerance, R);
+    li = ri.linear;
+    res.refinements += ri.refinements;
+  } else {
+    li = Pcg64(P, w.vals, w.inv, w.rhs, w.x, w.r, w.p, w.ap, tol, maxit, cfg.pcg_warm_start != 0,
+               R);
+  }
   __syncthreads();
+  res.linear_iterations += li.iterations;
+  res.last_linear_tolerance = tol;
+  res.linear_residual = li.residual;
+  return li;
+}
 
-  double eq_scale = 1.0;
+// One item's nonlinear solve; leaves the state in `a`.
+//
+// safeguarded = false: the reference Newton driver (NonlinearP1Solver::Solve / SolveBatch,
+// nonlinear_solver.cu:183-530, 667-769): plain start (cold secant at iteration 0) or
+// `warm` (Newton terms from iteration 0), per-item inexact-Newton tolerance, fkn relaxation
+// after iteration 5, NewtonStepConverged, and with require_equation_residual the
+// equation-residual gate at the returned state.
+//
+// safeguarded = true: SolveSafeguarded (nonlinear_solver.cu:534-659).  Each iteration
+// solves for the Newton candidate at the current state with the exact linear tolerance;
+// the item is accepted when the candidate's undamped step is below 100x
+// relative_tolerance and the equation residual at the current state is within
+// relative_tolerance (the returned state is the current one).  Otherwise the first alpha
+// in 1, 1/2, ..., 2^-19 whose trial state either reduces the equation residual by the
+// Armijo factor (1 - 1e-4 alpha) or already passes the residual gate becomes the new
+// state; if none does, the item stops unconverged.  From `warm` (the reference's warm-start
+// hook, nonlinear_solver.cu:667-698) the iteration starts there.
+//
+// Both policies scale the equation residual by its norm at the Dirichlet-expanded plain
+// start (nonlinear_solver.cu:565-576), independent of any warm start.  The two policies
+// share one body so that a kernel running both (the learned mode) holds one copy of the
+// assembly and the linear solve.
+template <bool kMixed>
+__device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers &w,
+                                        const double *cur, double *a, bool inputs_ok,
+                                        const femm_newton_options_t &o,
+                                        const femm_solve_config_t &cfg, const double *warm,
+                                        bool safeguarded, BlockReducer &R)
+{
+  femm_item_result_t res = FreshResult();
+  inputs_ok = inputs_ok && OptionsValid(o);
+  const bool has_warm = warm != nullptr;
+  const bool adaptive = o.adaptive_linear_tolerance && P.any_nonlinear;
   const bool gate_eq = o.require_equation_residual != 0;
+
+  // Plain start: prescribed values on every fixed node (signed roots expanded), zero
+  // elsewhere; the residual scale is taken here, then the warm start replaces the free
+  // nodes.
+  for (int v = threadIdx.x; v < P.N; v += kBlockThreads)
+    a[v] = P.node_to_free[v] < 0 ? P.boundary[v] : 0.0;
+  __syncthreads();
   int status = inputs_ok ? FEMM_NONLINEAR_SOLVE_NOT_CONVERGED : FEMM_INVALID_ARGUMENT;
-  if (inputs_ok && (cfg.verify_equation_residual || gate_eq)) {
-    // Scale of the equation residual: the norm at the Dirichlet-expanded start
-    // (nonlinear_solver.cu:565-576), independent of any warm start.  Assemble at
-    // the boundary-only state held temporarily in `a`, then restore.
-    if (has_warm) {
-      for (int v = threadIdx.x; v < P.N; v += kBlockThreads)
-        if (P.node_to_free[v] >= 0)
-          a[v] = 0.0;
-      __syncthreads();
-    }
+  double scale = 1.0, norm = INFINITY;
+  if (inputs_ok && (safeguarded || cfg.verify_equation_residual || gate_eq)) {
     if (AssembleItem(P, w, a, cur, true, R)) {
       __syncthreads();
-      eq_scale = fmax(EquationNorm(P, w, a, R), DBL_MIN);
+      norm = EquationNorm(P, w, a, R, w.x);
+      scale = fmax(norm, DBL_MIN);
+    } else if (safeguarded) {
+      status = FEMM_ASSEMBLY_FAILED;
     }
-    if (has_warm) {
-      for (int v = threadIdx.x; v < P.N; v += kBlockThreads)
-        if (P.node_to_free[v] >= 0)
-          a[v] = warm[static_cast<size_t>(item) * P.N + v];
-    }
+  }
+  if (has_warm) {
+    for (int v = threadIdx.x; v < P.N; v += kBlockThreads)
+      if (P.node_to_free[v] >= 0)
+        a[v] = warm[v];
     __syncthreads();
+    if (safeguarded && status == FEMM_NONLINEAR_SOLVE_NOT_CONVERGED) {
+      if (AssembleItem(P, w, a, cur, true, R)) {
+        __syncthreads();
+        norm = EquationNorm(P, w, a, R, w.x);
+      } else {
+        status = FEMM_ASSEMBLY_FAILED;
+      }
+    }
   }
 
   double omega = 1.0, prev = INFINITY;
   bool pending = false;   // step gate passed; equation residual not yet checked
   bool eq_current = false;
-  for (int it = 0; inputs_ok; ++it) {
-    if (!pending && it >= o.max_newton_iterations)
-      break;
-    const bool newton = has_warm || it > 0;
-    if (!AssembleItem(P, w, a, cur, newton, R)) {
-      status = FEMM_ASSEMBLY_FAILED;
-      break;
-    }
-    __syncthreads();
-    if (pending) {
-      // The Newton assembly at the returned state also yields its equation
-      // residual (nonlinear_solver.cu:549-558); accept only if it passes too.
-      res.equation_residual = EquationNorm(P, w, a, R) / eq_scale;
-      eq_current = true;
-      pending = false;
-      if (res.equation_residual <= o.relative_tolerance) {
-        status = FEMM_OK;
-        break;
-      }
+  for (int it = 0; status == FEMM_NONLINEAR_SOLVE_NOT_CONVERGED; ++it) {
+    if (safeguarded) {
+      // The assembly at the current state is current (start or accepted trial).
       if (it >= o.max_newton_iterations)
         break;
+      if (!isfinite(norm)) {
+        status = FEMM_NUMERICAL_NONFINITE;
+        break;
+      }
+    } else {
+      if (!pending && it >= o.max_newton_iterations)
+        break;
+      if (!AssembleItem(P, w, a, cur, has_warm || it > 0, R)) {
+        status = FEMM_ASSEMBLY_FAILED;
+        break;
+      }
+      __syncthreads();
+      if (pending) {
+        // The Newton assembly at the returned state also yields its equation
+        // residual (nonlinear_solver.cu:549-558); accept only if it passes too.
+        res.equation_residual = EquationNorm(P, w, a, R, w.x) / scale;
+        eq_current = true;
+        pending = false;
+        if (res.equation_residual <= o.relative_tolerance) {
+          status = FEMM_OK;
+          break;
+        }
+        if (it >= o.max_newton_iterations)
+          break;
+      }
     }
-    const double tol = LinearTolerance(o, adaptive, prev);
+    const double tol = safeguarded ? o.linear_relative_tolerance : LinearTolerance(o, adaptive, prev);
     if (cfg.pcg_warm_start) {
       for (int i = threadIdx.x; i < P.n; i += kBlockThreads)
         w.x[i] = a[P.free_nodes[i]];
       __syncthreads();
     }
-    LinearInfo li;
-    if constexpr (kMixed) {
-      const RefinedInfo ri = RefinedSolve(
-          P, w.vals, w.inv, w.vals32, w.inv32, w.rhs, w.x, w.r, w.p, w.ap, w.d32, w.r32, w.p32,
-          w.ap32, tol, o.max_linear_iterations, cfg.pcg_warm_start != 0, cfg.max_refinements,
-          cfg.inner_relative_tolerance, R);
-      li = ri.linear;
-      res.refinements += ri.refinements;
-    } else {
-      li = Pcg64(P, w.vals, w.inv, w.rhs, w.x, w.r, w.p, w.ap, tol, o.max_linear_iterations,
-                 cfg.pcg_warm_start != 0, R);
-    }
-    __syncthreads();
-    res.linear_iterations += li.iterations;
-    res.last_linear_tolerance = tol;
-    res.linear_residual = li.residual;
+    const LinearInfo li = LinearSolve<kMixed>(P, w, cfg, tol, o.max_linear_iterations, res, R);
     if (li.status != FEMM_OK) {
       status = li.status;
       break;
@@ -936,6 +1148,41 @@ __global__ __launch_bounds__(kBlockThreads) void NewtonKernel(DevicePlan P, Devi
     if (!isfinite(step)) {
       status = FEMM_NUMERICAL_NONFINITE;
       break;
+    }
+    if (safeguarded) {
+      if (step < 100.0 * o.relative_tolerance && norm / scale <= o.relative_tolerance) {
+        status = FEMM_OK;
+        break;
+      }
+      bool accepted = false;
+      double alpha = 1.0;
+      for (int bt = 0; bt < 20; ++bt, alpha *= 0.5) {
+        for (int v = threadIdx.x; v < P.N; v += kBlockThreads) {
+          const int f = P.node_to_free[v];
+          const double cand =
+              f >= 0 ? static_cast<double>(P.node_sign[v]) * w.x[f] : P.boundary[v];
+          w.trial[v] = a[v] + alpha * (cand - a[v]);
+        }
+        __syncthreads();
+        if (!AssembleItem(P, w, w.trial, cur, true, R)) {
+          status = FEMM_ASSEMBLY_FAILED;
+          break;
+        }
+        __syncthreads();
+        const double tn = EquationNorm(P, w, w.trial, R, w.p);
+        if (isfinite(tn) &&
+            (tn <= (1.0 - 1e-4 * alpha) * norm || tn / scale <= o.relative_tolerance)) {
+          for (int v = threadIdx.x; v < P.N; v += kBlockThreads)
+            a[v] = w.trial[v];
+          __syncthreads();
+          norm = tn;
+          accepted = true;
+          break;
+        }
+      }
+      if (!accepted)
+        break;
+      continue;
     }
     // fkn-style relaxation after iteration 5 (nonlinear_solver.cu:344-351).
     if (it > 5) {
@@ -961,17 +1208,80 @@ __global__ __launch_bounds__(kBlockThreads) void NewtonKernel(DevicePlan P, Devi
     }
     prev = step;
   }
-  if (inputs_ok && cfg.verify_equation_residual && !eq_current &&
-      (status == FEMM_OK || status == FEMM_NONLINEAR_SOLVE_NOT_CONVERGED)) {
+  if (safeguarded) {
+    if (inputs_ok)
+      res.equation_residual = norm / scale;
+  } else if (inputs_ok && cfg.verify_equation_residual && !eq_current &&
+             (status == FEMM_OK || status == FEMM_NONLINEAR_SOLVE_NOT_CONVERGED)) {
     if (AssembleItem(P, w, a, cur, true, R)) {
       __syncthreads();
-      res.equation_residual = EquationNorm(P, w, a, R) / eq_scale;
+      res.equation_residual = EquationNorm(P, w, a, R, w.x) / scale;
     } else {
       res.equation_residual = INFINITY;
     }
   }
-  WriteFlux(P, a, flux_out + static_cast<size_t>(item) * P.C, R);
   res.status = status;
+  return res;
+}
+
+enum SolveMode : int32_t {
+  kModeNewton = 0,       // the reference driver
+  kModeSafeguarded = 1,  // SolveSafeguarded
+  kModeLearned = 2       // safeguarded from `warm` within a budget, then the driver from the
+                         // plain start in the same workgroup if that attempt is not accepted
+};
+
+// One workgroup per batch item runs the whole nonlinear solve of `kMode`.  With prior
+// results, an item the earlier launch kept (KeepPrior) is left untouched and the others
+// add their counters to the earlier ones.
+template <bool kMixed, int kMode>
+__global__ __launch_bounds__(kBlockThreads) void SolveKernel(
+    DevicePlan base, DeviceWork W, const double *currents, NewtonArgs args, const double *warm,
+    double *A, double *flux_out, femm_item_result_t *results, const femm_item_result_t *prior,
+    DeviceVariants VS, const int32_t *variant_of_item)
+{
+  __shared__ double lds[2 * kWaves * 4];
+  BlockReducer R{lds, 0};
+  const int item = blockIdx.x;
+  femm_item_result_t earlier{};
+  if (prior != nullptr) {
+    earlier = prior[item];
+    if (KeepPrior(earlier, args.prior_mode))
+      return;
+  }
+  DevicePlan P = base;
+  bool inputs_ok = ApplyVariant(P, VS, variant_of_item, item);
+  const ItemPointers w = ItemWork(P, W, item, kMixed);
+  double *a = A + static_cast<size_t>(item) * P.N;
+  const double *cur = currents + static_cast<size_t>(item) * P.C;
+  for (int c = 0; c < P.C; ++c)
+    inputs_ok = inputs_ok && isfinite(cur[c]);
+  const double *warm_item = warm ? warm + static_cast<size_t>(item) * P.N : nullptr;
+  // One call site of ItemSolve (the learned mode loops over its two attempts) keeps a
+  // single inlined copy of the assembly and the linear solve.
+  constexpr int kAttempts = kMode == kModeLearned ? 2 : 1;
+  femm_item_result_t res{}, spent{};
+#pragma clang loop unroll(disable)
+  for (int attempt = 0; attempt < kAttempts; ++attempt) {
+    const bool first = attempt == 0;
+    const bool safeguarded = kMode == kModeSafeguarded || (kMode == kModeLearned && first);
+    const femm_newton_options_t &o =
+        (kMode == kModeLearned && first) ? args.warm_options : args.options;
+    res = ItemSolve<kMixed>(P, w, cur, a, inputs_ok, o, args.config, first ? warm_item : nullptr,
+                            safeguarded, R);
+    res.newton_iterations += spent.newton_iterations;
+    res.linear_iterations += spent.linear_iterations;
+    res.refinements += spent.refinements;
+    if (res.status == FEMM_OK)
+      break;
+    spent = res;
+  }
+  WriteFlux(P, a, flux_out + static_cast<size_t>(item) * P.C, R);
+  if (prior != nullptr) {
+    res.newton_iterations += earlier.newton_iterations;
+    res.linear_iterations += earlier.linear_iterations;
+    res.refinements += earlier.refinements;
+  }
   if (threadIdx.x == 0)
     results[item] = res;
 }
@@ -1310,6 +1620,130 @@ int UploadPlan(const HostPlan &h, DeviceBuffers *dev)
   return hipDeviceSynchronize() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
 }
 
+namespace
+{
+
+// Visits the variant fields in layout order: fn(member pointer, per-variant element count).
+template <typename Fn>
+void ForEachVariantField(const HostPlan &topo, Fn &&fn)
+{
+  fn(&HostPlan::tri_mat, topo.tri_mat.size());
+  fn(&HostPlan::tri_circuit, topo.tri_circuit.size());
+  fn(&HostPlan::tri_flux, topo.tri_flux.size());
+  fn(&HostPlan::mat_nu0, topo.mat_nu0.size());
+  fn(&HostPlan::mat_alpha, topo.mat_alpha.size());
+  fn(&HostPlan::mat_cold, topo.mat_cold.size());
+  fn(&HostPlan::mat_curve_off, topo.mat_curve_off.size());
+  fn(&HostPlan::mat_curve_cnt, topo.mat_curve_cnt.size());
+  fn(&HostPlan::curve_b, topo.curve_b.size());
+  fn(&HostPlan::curve_h, topo.curve_h.size());
+  fn(&HostPlan::curve_s, topo.curve_s.size());
+  fn(&HostPlan::pm_rhs, topo.pm_rhs.size());
+  fn(&HostPlan::circuit_rhs, topo.circuit_rhs.size());
+}
+
+// Device array of a variant field.
+const int32_t *&FieldSlot(DeviceVariants &D, std::vector<int32_t> HostPlan::*m)
+{
+  if (m == &HostPlan::tri_mat)
+    return D.tri_mat;
+  if (m == &HostPlan::tri_circuit)
+    return D.tri_circuit;
+  if (m == &HostPlan::mat_curve_off)
+    return D.mat_curve_off;
+  return D.mat_curve_cnt;
+}
+
+const double *&FieldSlot(DeviceVariants &D, std::vector<double> HostPlan::*m)
+{
+  if (m == &HostPlan::tri_flux)
+    return D.tri_flux;
+  if (m == &HostPlan::mat_nu0)
+    return D.mat_nu0;
+  if (m == &HostPlan::mat_alpha)
+    return D.mat_alpha;
+  if (m == &HostPlan::mat_cold)
+    return D.mat_cold;
+  if (m == &HostPlan::curve_b)
+    return D.curve_b;
+  if (m == &HostPlan::curve_h)
+    return D.curve_h;
+  if (m == &HostPlan::curve_s)
+    return D.curve_s;
+  if (m == &HostPlan::pm_rhs)
+    return D.pm_rhs;
+  return D.circuit_rhs;
+}
+
+}  // namespace
+
+int AllocateVariants(const HostPlan &topo, int32_t count, DeviceBuffers *dev)
+{
+  if (dev->variants_base) {
+    if (hipDeviceSynchronize() != hipSuccess)
+      return FEMM_INTERNAL_ERROR;
+    (void)hipFree(dev->variants_base);
+    dev->variants_base = nullptr;
+  }
+  dev->variants = DeviceVariants{};
+  if (count <= 0)
+    return FEMM_OK;
+  const size_t V = static_cast<size_t>(count);
+  size_t bytes = 0;
+  ForEachVariantField(topo, [&](auto member, size_t stride) {
+    using T = typename std::decay_t<decltype(topo.*member)>::value_type;
+    bytes += (V * stride * sizeof(T) + 255) / 256 * 256;
+  });
+  bytes += (V * sizeof(int32_t) + 255) / 256 * 256;
+  void *mem = nullptr;
+  if (hipMalloc(&mem, bytes) != hipSuccess)
+    return FEMM_GPU_ALLOCATION_FAILED;
+  if (hipMemset(mem, 0, bytes) != hipSuccess) {
+    (void)hipFree(mem);
+    return FEMM_INTERNAL_ERROR;
+  }
+  dev->variants_base = mem;
+  DeviceVariants &D = dev->variants;
+  char *cursor = static_cast<char *>(mem);
+  ForEachVariantField(topo, [&](auto member, size_t stride) {
+    using T = typename std::decay_t<decltype(topo.*member)>::value_type;
+    FieldSlot(D, member) = reinterpret_cast<const T *>(cursor);
+    cursor += (V * stride * sizeof(T) + 255) / 256 * 256;
+  });
+  D.any_nonlinear = reinterpret_cast<const int32_t *>(cursor);
+  D.count = count;
+  D.mats = static_cast<int32_t>(topo.mat_nu0.size());
+  D.knots = static_cast<int32_t>(topo.curve_b.size());
+  return FEMM_OK;
+}
+
+int CopyVariant(const HostPlan &topo, int32_t index, const HostPlan &part, DeviceBuffers *dev)
+{
+  DeviceVariants &D = dev->variants;
+  if (index < 0 || index >= D.count)
+    return FEMM_INVALID_ARGUMENT;
+  bool ok = true;
+  ForEachVariantField(topo, [&](auto member, size_t stride) {
+    using T = typename std::decay_t<decltype(topo.*member)>::value_type;
+    const std::vector<T> &src = part.*member;
+    if (src.size() != stride) {
+      ok = false;
+      return;
+    }
+    T *dst = const_cast<T *>(FieldSlot(D, member)) + static_cast<size_t>(index) * stride;
+    if (ok && stride &&
+        hipMemcpy(dst, src.data(), stride * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+      ok = false;
+  });
+  if (!ok)
+    return FEMM_INVALID_ARGUMENT;
+  const int32_t flag = part.any_nonlinear ? 1 : 0;
+  if (hipMemcpy(const_cast<int32_t *>(D.any_nonlinear) + index, &flag, sizeof(int32_t),
+                hipMemcpyHostToDevice) != hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  return FEMM_OK;
+}
+
 int EnsureWork(const HostPlan &h, int32_t items, bool mixed, DeviceBuffers *dev)
 {
   const bool have_mixed = dev->work.vals32 != nullptr;
@@ -1325,7 +1759,7 @@ int EnsureWork(const HostPlan &h, int32_t items, bool mixed, DeviceBuffers *dev)
   }
   const size_t ci = static_cast<size_t>(cap);
   size_t bytes = Bytes<double>(ci * h.T * kElemStride) + Bytes<double>(ci * h.sell_size) +
-                 6 * Bytes<double>(ci * h.n);
+                 6 * Bytes<double>(ci * h.n) + Bytes<double>(ci * h.N);
   if (want_mixed)
     bytes += Bytes<float>(ci * h.sell_size) + 5 * Bytes<float>(ci * h.n);
   void *base = nullptr;
@@ -1348,6 +1782,7 @@ int EnsureWork(const HostPlan &h, int32_t items, bool mixed, DeviceBuffers *dev)
   w.r = Carve<double>(cur, ci * h.n);
   w.p = Carve<double>(cur, ci * h.n);
   w.ap = Carve<double>(cur, ci * h.n);
+  w.trial = Carve<double>(cur, ci * h.N);
   if (want_mixed) {
     w.vals32 = Carve<float>(cur, ci * h.sell_size);
     w.r32 = Carve<float>(cur, ci * h.n);
@@ -1363,6 +1798,10 @@ int EnsureWork(const HostPlan &h, int32_t items, bool mixed, DeviceBuffers *dev)
 
 void ReleaseDevice(DeviceBuffers *dev)
 {
+  if (dev->variants_base)
+    (void)hipFree(dev->variants_base);
+  dev->variants_base = nullptr;
+  dev->variants = DeviceVariants{};
   if (dev->work_base)
     (void)hipFree(dev->work_base);
   if (dev->base)
@@ -1371,21 +1810,58 @@ void ReleaseDevice(DeviceBuffers *dev)
   dev->base = nullptr;
 }
 
-int LaunchNewton(const DeviceBuffers &dev, int32_t items, const double *currents,
-                 const femm_newton_options_t &options, const femm_solve_config_t &config,
-                 const double *warm, double *a, double *flux, femm_item_result_t *results,
-                 hipStream_t stream)
+namespace
+{
+
+template <int kMode>
+int LaunchSolve(const DeviceBuffers &dev, int32_t items, const double *currents,
+                const NewtonArgs &args, const double *warm, double *a, double *flux,
+                femm_item_result_t *results, hipStream_t stream, const LaunchExtras &x)
 {
   if (items <= 0 || items > dev.work.capacity)
     return FEMM_INVALID_ARGUMENT;
-  NewtonArgs args{options, config};
-  if (config.precision == FEMM_PRECISION_MIXED)
-    hipLaunchKernelGGL(NewtonKernel<true>, dim3(items), dim3(kBlockThreads), 0, stream, dev.plan,
-                       dev.work, currents, args, warm, a, flux, results);
+  if (args.config.precision == FEMM_PRECISION_MIXED)
+    hipLaunchKernelGGL((SolveKernel<true, kMode>), dim3(items), dim3(kBlockThreads), 0, stream,
+                       dev.plan, dev.work, currents, args, warm, a, flux, results, x.prior,
+                       dev.variants, x.variant_of_item);
   else
-    hipLaunchKernelGGL(NewtonKernel<false>, dim3(items), dim3(kBlockThreads), 0, stream, dev.plan,
-                       dev.work, currents, args, warm, a, flux, results);
+    hipLaunchKernelGGL((SolveKernel<false, kMode>), dim3(items), dim3(kBlockThreads), 0, stream,
+                       dev.plan, dev.work, currents, args, warm, a, flux, results, x.prior,
+                       dev.variants, x.variant_of_item);
   return hipGetLastError() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
+}
+
+}  // namespace
+
+int LaunchNewton(const DeviceBuffers &dev, int32_t items, const double *currents,
+                 const femm_newton_options_t &options, const femm_solve_config_t &config,
+                 const double *warm, double *a, double *flux, femm_item_result_t *results,
+                 hipStream_t stream, const LaunchExtras &x)
+{
+  const NewtonArgs args{options, config, x.prior ? x.prior_mode : kPriorNone, options};
+  return LaunchSolve<kModeNewton>(dev, items, currents, args, warm, a, flux, results, stream, x);
+}
+
+int LaunchSafeguarded(const DeviceBuffers &dev, int32_t items, const double *currents,
+                      const femm_newton_options_t &options, const femm_solve_config_t &config,
+                      const double *warm, double *a, double *flux, femm_item_result_t *results,
+                      hipStream_t stream, const LaunchExtras &x)
+{
+  const NewtonArgs args{options, config, x.prior ? x.prior_mode : kPriorNone, options};
+  return LaunchSolve<kModeSafeguarded>(dev, items, currents, args, warm, a, flux, results, stream,
+                                       x);
+}
+
+int LaunchLearned(const DeviceBuffers &dev, int32_t items, const double *currents,
+                  const femm_newton_options_t &warm_options,
+                  const femm_newton_options_t &plain_options, const femm_solve_config_t &config,
+                  const double *warm, double *a, double *flux, femm_item_result_t *results,
+                  hipStream_t stream, const LaunchExtras &x)
+{
+  if (warm == nullptr)
+    return FEMM_INVALID_ARGUMENT;
+  const NewtonArgs args{plain_options, config, x.prior ? x.prior_mode : kPriorNone, warm_options};
+  return LaunchSolve<kModeLearned>(dev, items, currents, args, warm, a, flux, results, stream, x);
 }
 
 int LaunchAssemble(const DeviceBuffers &dev, int32_t items, const double *states,

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "femm_gpu_amd.h"
 #include "plan.hpp"
@@ -39,11 +40,27 @@ struct DeviceWork {
   double *vals;      // [items][sell_size]
   float *vals32;     // [items][sell_size]   (mixed precision)
   double *rhs, *inv, *x, *r, *p, *ap;   // [items][n]
+  double *trial;     // [items][N] Armijo trial state (safeguarded Newton)
   float *r32, *p32, *ap32, *d32, *inv32; // [items][n]   (mixed precision)
+};
+
+// Per-item material variants on the plan's mesh: every array is [count][stride] with the
+// plan's sizes (T triangles, `mats` materials, `knots` B-H knots, n free rows, C circuits).
+struct DeviceVariants {
+  int32_t count = 0, mats = 0, knots = 0;
+  const int32_t *tri_mat = nullptr, *tri_circuit = nullptr;
+  const double *tri_flux = nullptr;
+  const double *mat_nu0 = nullptr, *mat_alpha = nullptr, *mat_cold = nullptr;
+  const int32_t *mat_curve_off = nullptr, *mat_curve_cnt = nullptr;
+  const double *curve_b = nullptr, *curve_h = nullptr, *curve_s = nullptr;
+  const double *pm_rhs = nullptr, *circuit_rhs = nullptr;
+  const int32_t *any_nonlinear = nullptr;
 };
 
 struct DeviceBuffers {
   DevicePlan plan{};
+  DeviceVariants variants{};
+  void *variants_base = nullptr;
   void *base = nullptr;          // one allocation for the immutable plan
   DeviceWork work{};
   void *work_base = nullptr;
@@ -51,14 +68,50 @@ struct DeviceBuffers {
 };
 
 int UploadPlan(const HostPlan &host, DeviceBuffers *dev);
+// Replaces the variant set with `count` zeroed variants laid out for `topology`, then
+// fills variant `index` from a material part built with BuildHostPlan(..., &topology).
+int AllocateVariants(const HostPlan &topology, int32_t count, DeviceBuffers *dev);
+int CopyVariant(const HostPlan &topology, int32_t index, const HostPlan &part, DeviceBuffers *dev);
 int EnsureWork(const HostPlan &host, int32_t items, bool mixed, DeviceBuffers *dev);
 void ReleaseDevice(DeviceBuffers *dev);
 
-// Batched Newton solve on device pointers (enqueued on `stream`).
+// Which items a launch with prior results solves again (`prior` may alias `results`;
+// the others keep the earlier outputs untouched, and re-solved items add their
+// iteration counters to the earlier ones).
+enum PriorMode : int32_t {
+  kPriorNone = 0,          // no prior results: solve every item
+  kPriorRedoFailed = 1,    // items whose prior status is not FEMM_OK
+  kPriorRedoUnconverged = 2  // items whose prior status is NONLINEAR_SOLVE_NOT_CONVERGED
+};
+
+// Optional per-launch inputs: earlier results (see PriorMode) and the material variant
+// of each item (null = the plan's own materials).
+struct LaunchExtras {
+  const femm_item_result_t *prior = nullptr;
+  int32_t prior_mode = kPriorNone;
+  const int32_t *variant_of_item = nullptr;
+};
+
+// Batched Newton solve with the reference driver's policy (nonlinear_solver.cu:183-530)
+// on device pointers, enqueued on `stream`; `warm` null = the plain start.
 int LaunchNewton(const DeviceBuffers &dev, int32_t items, const double *currents,
                  const femm_newton_options_t &options, const femm_solve_config_t &config,
                  const double *warm, double *a, double *flux, femm_item_result_t *results,
-                 hipStream_t stream);
+                 hipStream_t stream, const LaunchExtras &extras = LaunchExtras{});
+// Learned-start solve in one launch: per item, the safeguarded Newton from `warm` with
+// `warm_options` (its iteration budget), and if that attempt is not accepted, the
+// reference driver from the plain start with `plain_options` in the same workgroup.
+int LaunchLearned(const DeviceBuffers &dev, int32_t items, const double *currents,
+                  const femm_newton_options_t &warm_options,
+                  const femm_newton_options_t &plain_options, const femm_solve_config_t &config,
+                  const double *warm, double *a, double *flux, femm_item_result_t *results,
+                  hipStream_t stream, const LaunchExtras &extras = LaunchExtras{});
+// Safeguarded Newton (full steps with Armijo backtracking on the equation residual,
+// nonlinear_solver.cu:534-659) from `warm` or the plain start.
+int LaunchSafeguarded(const DeviceBuffers &dev, int32_t items, const double *currents,
+                      const femm_newton_options_t &options, const femm_solve_config_t &config,
+                      const double *warm, double *a, double *flux, femm_item_result_t *results,
+                      hipStream_t stream, const LaunchExtras &extras = LaunchExtras{});
 // Assembly only (values in SELL layout into work.vals, rhs, inv) for `items`.
 int LaunchAssemble(const DeviceBuffers &dev, int32_t items, const double *states,
                    const double *currents, int32_t newton, int32_t *valid, hipStream_t stream);

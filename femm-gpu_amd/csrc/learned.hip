@@ -1,0 +1,753 @@
+// Learned Newton warm start: POD basis of solver-generated states, an FP64 linear
+// skip path and the BF16 MFMA MLP, giving A0 = mean + sum_k c_k phi_k with
+// c = L0 + L x_n + scale * MLP(x_n) for each item's features x (x_n standardised).
+//
+// The warm start feeds the reference's warm-start hook (NonlinearP1Solver::Solve with
+// warm_start, nonlinear_solver.cu:667-698: the start replaces the initial A with the
+// Dirichlet values re-imposed, and Newton terms are used from iteration 0).  From it the
+// safeguarded Newton of the reference (full steps with Armijo backtracking,
+// nonlinear_solver.cu:534-659) runs within a warm iteration budget; an item that does
+// not pass both acceptance gates there is solved again from the plain start by the
+// reference driver in a second, fallback launch.
+//
+// POD (method of snapshots): G = X~ X~^T over centred states, its leading eigenpairs
+// by subspace iteration (G Z on the GPU, orthonormalisation and the small Rayleigh-Ritz
+// problem on the host), modes phi_k = X~^T y_k / sigma_k and training coefficients
+// c_{s,k} = sigma_k y_{s,k}.  All POD arithmetic is FP64.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <new>
+#include <numeric>
+#include <random>
+#include <vector>
+
+#include "device.hpp"
+#include "femm_gpu_amd.h"
+#include "handle.hpp"
+#include "mlp.hpp"
+
+namespace femm
+{
+int TrainMlp(Mlp *net, int samples, const double *x, const float *t, int steps, int batch,
+             float lr, uint64_t seed, float *loss_history_dev, hipStream_t s);
+__global__ void SumToScalar(const float *v, int count, float scale, float *out);
+
+namespace
+{
+
+constexpr int kGramTile = 64;    // G tile edge per workgroup
+constexpr int kGramChunk = 32;   // nodes staged through LDS per step
+
+// mean[v] = (1/S) sum_s X[s][v], samples summed in order.
+__global__ void ColumnMeanKernel(const double *__restrict__ X, int S, int N,
+                                 double *__restrict__ mean)
+{
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= N)
+    return;
+  double s = 0.0;
+  for (int i = 0; i < S; ++i)
+    s += X[static_cast<size_t>(i) * N + v];
+  mean[v] = s / S;
+}
+
+// G[i][j] = sum_v (X[i][v] - m[v]) (X[j][v] - m[v]) for the upper tiles (bj >= bi),
+// mirrored.  256 threads compute a 64 x 64 tile, 4 x 4 outputs each, nodes in order.
+__global__ __launch_bounds__(256) void GramKernel(const double *__restrict__ X,
+                                                  const double *__restrict__ mean, int S, int N,
+                                                  double *__restrict__ G)
+{
+  const int bi = blockIdx.y, bj = blockIdx.x;
+  if (bj < bi)
+    return;
+  __shared__ double xi[kGramTile][kGramChunk + 1];
+  __shared__ double xj[kGramTile][kGramChunk + 1];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  double acc[4][4] = {};
+  for (int v0 = 0; v0 < N; v0 += kGramChunk) {
+    for (int e = threadIdx.x; e < kGramTile * kGramChunk; e += 256) {
+      const int r = e / kGramChunk, c = e % kGramChunk, v = v0 + c;
+      const int gi = bi * kGramTile + r, gj = bj * kGramTile + r;
+      const double m = v < N ? mean[v] : 0.0;
+      xi[r][c] = (gi < S && v < N) ? X[static_cast<size_t>(gi) * N + v] - m : 0.0;
+      xj[r][c] = (gj < S && v < N) ? X[static_cast<size_t>(gj) * N + v] - m : 0.0;
+    }
+    __syncthreads();
+    for (int c = 0; c < kGramChunk; ++c) {
+      double a[4], b[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        a[q] = xi[ty + 16 * q][c];
+        b[q] = xj[tx + 16 * q][c];
+      }
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          acc[p][q] += a[p] * b[q];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = bi * kGramTile + ty + 16 * p, j = bj * kGramTile + tx + 16 * q;
+      if (i < S && j < S) {
+        G[static_cast<size_t>(i) * S + j] = acc[p][q];
+        G[static_cast<size_t>(j) * S + i] = acc[p][q];
+      }
+    }
+}
+
+// out[i][c] = sum_j G[i][j] Z[j][c]  (G: S x S, Z: S x p, row-major), j in order.
+__global__ void SymmTimesBlock(const double *__restrict__ G, const double *__restrict__ Z, int S,
+                               int p, double *__restrict__ out)
+{
+  const size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= static_cast<size_t>(S) * p)
+    return;
+  const int i = static_cast<int>(e / p), c = static_cast<int>(e % p);
+  double s = 0.0;
+  for (int j = 0; j < S; ++j)
+    s += G[static_cast<size_t>(i) * S + j] * Z[static_cast<size_t>(j) * p + c];
+  out[e] = s;
+}
+
+// phi[k][v] = inv_sigma[k] * sum_s Y[s][k] (X[s][v] - m[v]); blockIdx.y picks 16 modes.
+__global__ void ModesKernel(const double *__restrict__ X, const double *__restrict__ mean,
+                            const double *__restrict__ Y, const double *__restrict__ inv_sigma,
+                            int S, int N, int K, double *__restrict__ phi)
+{
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k0 = blockIdx.y * 16;
+  if (v >= N)
+    return;
+  double acc[16] = {};
+  const double m = mean[v];
+  for (int s = 0; s < S; ++s) {
+    const double x = X[static_cast<size_t>(s) * N + v] - m;
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      if (k0 + q < K)
+        acc[q] += Y[static_cast<size_t>(s) * K + k0 + q] * x;
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q)
+    if (k0 + q < K)
+      phi[static_cast<size_t>(k0 + q) * N + v] = acc[q] * inv_sigma[k0 + q];
+}
+
+// One workgroup per item: c_k = L0_k + sum_j L_kj x_n,j + scale y_k (FP64), then
+// A0[item][v] = mean[v] + sum_k c_k phi[k][v].
+__global__ __launch_bounds__(256) void ExpandKernel(
+    const double *__restrict__ features, const double *__restrict__ f_mean,
+    const double *__restrict__ f_inv_std, const double *__restrict__ lin, const float *__restrict__ y,
+    int ld_y, double scale, const double *__restrict__ mean, const double *__restrict__ phi,
+    int inputs, int K, int N, double *__restrict__ a0)
+{
+  extern __shared__ double c[];
+  const int item = blockIdx.x;
+  const double *x = features + static_cast<size_t>(item) * inputs;
+  for (int k = threadIdx.x; k < K; k += blockDim.x) {
+    const double *row = lin + static_cast<size_t>(k) * (inputs + 1);
+    double s = row[0];
+    for (int j = 0; j < inputs; ++j)
+      s += row[1 + j] * ((x[j] - f_mean[j]) * f_inv_std[j]);
+    c[k] = s + scale * static_cast<double>(y ? y[static_cast<size_t>(item) * ld_y + k] : 0.0f);
+  }
+  __syncthreads();
+  double *out = a0 + static_cast<size_t>(item) * N;
+  for (int v = threadIdx.x; v < N; v += blockDim.x) {
+    double s = mean[v];
+    for (int k = 0; k < K; ++k)
+      s += c[k] * phi[static_cast<size_t>(k) * N + v];
+    out[v] = s;
+  }
+}
+
+unsigned Blocks(size_t count, int threads = 256)
+{
+  return static_cast<unsigned>((count + threads - 1) / threads);
+}
+
+// Modified Gram-Schmidt on the columns of Z (S x p, row-major), twice for stability.
+void Orthonormalise(std::vector<double> *Z, int S, int p)
+{
+  std::vector<double> &z = *Z;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int c = 0; c < p; ++c) {
+      for (int d = 0; d < c; ++d) {
+        double dot = 0.0;
+        for (int i = 0; i < S; ++i)
+          dot += z[static_cast<size_t>(i) * p + c] * z[static_cast<size_t>(i) * p + d];
+        for (int i = 0; i < S; ++i)
+          z[static_cast<size_t>(i) * p + c] -= dot * z[static_cast<size_t>(i) * p + d];
+      }
+      double nrm = 0.0;
+      for (int i = 0; i < S; ++i)
+        nrm += z[static_cast<size_t>(i) * p + c] * z[static_cast<size_t>(i) * p + c];
+      nrm = std::sqrt(nrm);
+      const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+      for (int i = 0; i < S; ++i)
+        z[static_cast<size_t>(i) * p + c] *= inv;
+    }
+}
+
+// Cyclic Jacobi eigen-decomposition of a small symmetric matrix H (p x p, row-major).
+// On return H's diagonal holds the eigenvalues and V (row-major) the eigenvectors as columns.
+void JacobiEigen(std::vector<double> *Hm, int p, std::vector<double> *Vm)
+{
+  std::vector<double> &H = *Hm, &V = *Vm;
+  V.assign(static_cast<size_t>(p) * p, 0.0);
+  for (int i = 0; i < p; ++i)
+    V[static_cast<size_t>(i) * p + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (int i = 0; i < p; ++i)
+      for (int j = 0; j < p; ++j)
+        (i == j ? diag : off) += H[static_cast<size_t>(i) * p + j] * H[static_cast<size_t>(i) * p + j];
+    if (off <= 1e-30 * diag)
+      break;
+    for (int r = 0; r < p - 1; ++r)
+      for (int q = r + 1; q < p; ++q) {
+        const double hrq = H[static_cast<size_t>(r) * p + q];
+        if (hrq == 0.0)
+          continue;
+        const double hrr = H[static_cast<size_t>(r) * p + r], hqq = H[static_cast<size_t>(q) * p + q];
+        const double theta = (hqq - hrr) / (2.0 * hrq);
+        const double t = (theta >= 0.0 ? 1.0 : -1.0) /
+                         (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
+        for (int k = 0; k < p; ++k) {   // columns r, q
+          const double hkr = H[static_cast<size_t>(k) * p + r], hkq = H[static_cast<size_t>(k) * p + q];
+          H[static_cast<size_t>(k) * p + r] = cs * hkr - sn * hkq;
+          H[static_cast<size_t>(k) * p + q] = sn * hkr + cs * hkq;
+        }
+        for (int k = 0; k < p; ++k) {   // rows r, q
+          const double hrk = H[static_cast<size_t>(r) * p + k], hqk = H[static_cast<size_t>(q) * p + k];
+          H[static_cast<size_t>(r) * p + k] = cs * hrk - sn * hqk;
+          H[static_cast<size_t>(q) * p + k] = sn * hrk + cs * hqk;
+        }
+        for (int k = 0; k < p; ++k) {
+          const double vkr = V[static_cast<size_t>(k) * p + r], vkq = V[static_cast<size_t>(k) * p + q];
+          V[static_cast<size_t>(k) * p + r] = cs * vkr - sn * vkq;
+          V[static_cast<size_t>(k) * p + q] = sn * vkr + cs * vkq;
+        }
+      }
+  }
+}
+
+// Symmetric positive (semi)definite solve by Cholesky with a relative ridge.
+bool CholeskySolve(std::vector<double> A, int n, std::vector<double> *B, int nrhs)
+{
+  double tr = 0.0;
+  for (int i = 0; i < n; ++i)
+    tr += A[static_cast<size_t>(i) * n + i];
+  for (int i = 0; i < n; ++i)
+    A[static_cast<size_t>(i) * n + i] += 1e-12 * tr / n;
+  for (int j = 0; j < n; ++j) {
+    double d = A[static_cast<size_t>(j) * n + j];
+    for (int k = 0; k < j; ++k)
+      d -= A[static_cast<size_t>(j) * n + k] * A[static_cast<size_t>(j) * n + k];
+    if (!(d > 0.0))
+      return false;
+    d = std::sqrt(d);
+    A[static_cast<size_t>(j) * n + j] = d;
+    for (int i = j + 1; i < n; ++i) {
+      double s = A[static_cast<size_t>(i) * n + j];
+      for (int k = 0; k < j; ++k)
+        s -= A[static_cast<size_t>(i) * n + k] * A[static_cast<size_t>(j) * n + k];
+      A[static_cast<size_t>(i) * n + j] = s / d;
+    }
+  }
+  std::vector<double> &b = *B;
+  for (int r = 0; r < nrhs; ++r) {
+    for (int i = 0; i < n; ++i) {   // L y = b
+      double s = b[static_cast<size_t>(i) * nrhs + r];
+      for (int k = 0; k < i; ++k)
+        s -= A[static_cast<size_t>(i) * n + k] * b[static_cast<size_t>(k) * nrhs + r];
+      b[static_cast<size_t>(i) * nrhs + r] = s / A[static_cast<size_t>(i) * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {   // L^T x = y
+      double s = b[static_cast<size_t>(i) * nrhs + r];
+      for (int k = i + 1; k < n; ++k)
+        s -= A[static_cast<size_t>(k) * n + i] * b[static_cast<size_t>(k) * nrhs + r];
+      b[static_cast<size_t>(i) * nrhs + r] = s / A[static_cast<size_t>(i) * n + i];
+    }
+  }
+  return true;
+}
+
+}  // namespace
+}  // namespace femm
+
+struct femm_warm_start {
+  femm_plan *plan = nullptr;     // not owned; must outlive the warm start
+  int inputs = 0, modes = 0, N = 0;
+  bool fitted = false;
+  femm::Mlp net;
+  femm::DeviceArray<double> mean, phi, lin, f_mean, f_inv_std;
+  double scale = 1.0;
+  femm::DeviceArray<double> a0;          // [capacity][N] predicted starts
+  femm::DeviceArray<double> features;    // staging for host-pointer calls
+  int capacity = 0;
+  femm_warm_start_report_t report{};
+};
+
+namespace
+{
+
+int FitDevice(femm_warm_start *ws, int S, const double *x_dev, const double *states_dev,
+              const femm_warm_start_train_t &tr, hipStream_t stream)
+{
+  using femm::DeviceArray;
+  const int N = ws->N, K = ws->modes, in = ws->inputs;
+  const int p = std::min(S, K + 16);
+  if (S < K + 1)
+    return FEMM_INVALID_ARGUMENT;
+  // Feature standardisation (host, FP64).
+  std::vector<double> xh(static_cast<size_t>(S) * in);
+  if (hipMemcpy(xh.data(), x_dev, sizeof(double) * xh.size(), hipMemcpyDeviceToHost) != hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  std::vector<double> fm(in, 0.0), fs(in, 0.0);
+  for (int s = 0; s < S; ++s)
+    for (int j = 0; j < in; ++j) {
+      const double v = xh[static_cast<size_t>(s) * in + j];
+      if (!std::isfinite(v))
+        return FEMM_INVALID_ARGUMENT;
+      fm[j] += v;
+    }
+  for (int j = 0; j < in; ++j)
+    fm[j] /= S;
+  for (int s = 0; s < S; ++s)
+    for (int j = 0; j < in; ++j) {
+      const double d = xh[static_cast<size_t>(s) * in + j] - fm[j];
+      fs[j] += d * d;
+    }
+  for (int j = 0; j < in; ++j) {
+    fs[j] = std::sqrt(fs[j] / S);
+    if (!(fs[j] > 0.0))
+      fs[j] = 1.0;   // a constant feature carries no information; keep it finite
+  }
+  // Mean state and Gram matrix of the centred snapshots.
+  DeviceArray<double> G, Z, GZ, Y, inv_sigma;
+  if (!ws->mean.Allocate(N) || !G.Allocate(static_cast<size_t>(S) * S) ||
+      !Z.Allocate(static_cast<size_t>(S) * p) || !GZ.Allocate(static_cast<size_t>(S) * p))
+    return FEMM_GPU_ALLOCATION_FAILED;
+  hipLaunchKernelGGL(femm::ColumnMeanKernel, dim3(femm::Blocks(N)), dim3(256), 0, stream,
+                     states_dev, S, N, ws->mean.get());
+  const int tiles = (S + femm::kGramTile - 1) / femm::kGramTile;
+  hipLaunchKernelGGL(femm::GramKernel, dim3(tiles, tiles), dim3(256), 0, stream, states_dev,
+                     ws->mean.get(), S, N, G.get());
+  // Subspace iteration on G with a seeded Gaussian start block.
+  std::vector<double> z(static_cast<size_t>(S) * p);
+  std::mt19937_64 rng(tr.seed ^ 0x9e3779b97f4a7c15ull);
+  std::normal_distribution<double> gauss(0.0, 1.0);
+  for (double &e : z)
+    e = gauss(rng);
+  femm::Orthonormalise(&z, S, p);
+  std::vector<double> prev_ritz(p, 0.0), H(static_cast<size_t>(p) * p), V;
+  const int max_iter = std::max(1, tr.pod_iterations);
+  for (int it = 0; it < max_iter; ++it) {
+    if (hipMemcpyAsync(Z.get(), z.data(), sizeof(double) * z.size(), hipMemcpyHostToDevice,
+                       stream) != hipSuccess)
+      return FEMM_INTERNAL_ERROR;
+    hipLaunchKernelGGL(femm::SymmTimesBlock, dim3(femm::Blocks(static_cast<size_t>(S) * p)),
+                       dim3(256), 0, stream, G.get(), Z.get(), S, p, GZ.get());
+    std::vector<double> gz(z.size());
+    if (hipMemcpyAsync(gz.data(), GZ.get(), sizeof(double) * gz.size(), hipMemcpyDeviceToHost,
+                       stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+      return FEMM_INTERNAL_ERROR;
+    // Rayleigh-Ritz on span(Z): H = Z^T G Z.
+    for (int a = 0; a < p; ++a)
+      for (int b = 0; b < p; ++b) {
+        double s = 0.0;
+        for (int i = 0; i < S; ++i)
+          s += z[static_cast<size_t>(i) * p + a] * gz[static_cast<size_t>(i) * p + b];
+        H[static_cast<size_t>(a) * p + b] = s;
+      }
+    for (int a = 0; a < p; ++a)
+      for (int b = a + 1; b < p; ++b) {
+        const double sym = 0.5 * (H[static_cast<size_t>(a) * p + b] + H[static_cast<size_t>(b) * p + a]);
+        H[static_cast<size_t>(a) * p + b] = H[static_cast<size_t>(b) * p + a] = sym;
+      }
+    std::vector<double> Hd = H;
+    femm::JacobiEigen(&Hd, p, &V);
+    std::vector<int> order(p);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int a, int b) {
+      return Hd[static_cast<size_t>(a) * p + a] > Hd[static_cast<size_t>(b) * p + b];
+    });
+    std::vector<double> ritz(p);
+    double change = 0.0;
+    for (int a = 0; a < p; ++a) {
+      ritz[a] = Hd[static_cast<size_t>(order[a]) * p + order[a]];
+      if (a < K)
+        change = std::max(change, std::fabs(ritz[a] - prev_ritz[a]) /
+                                      std::max(std::fabs(ritz[0]), 1e-300));
+    }
+    prev_ritz = ritz;
+    // Rotate the block onto the Ritz vectors: Z <- Z V (sorted).
+    std::vector<double> zr(z.size(), 0.0);
+    for (int i = 0; i < S; ++i)
+      for (int a = 0; a < p; ++a) {
+        double s = 0.0;
+        for (int b = 0; b < p; ++b)
+          s += z[static_cast<size_t>(i) * p + b] * V[static_cast<size_t>(b) * p + order[a]];
+        zr[static_cast<size_t>(i) * p + a] = s;
+      }
+    ws->report.pod_iterations = it + 1;
+    if (it > 0 && change < 1e-12) {
+      z = std::move(zr);
+      break;
+    }
+    if (it + 1 == max_iter) {
+      z = std::move(zr);
+      break;
+    }
+    // Next block: G Z V, orthonormalised.
+    std::vector<double> next(z.size(), 0.0);
+    for (int i = 0; i < S; ++i)
+      for (int a = 0; a < p; ++a) {
+        double s = 0.0;
+        for (int b = 0; b < p; ++b)
+          s += gz[static_cast<size_t>(i) * p + b] * V[static_cast<size_t>(b) * p + order[a]];
+        next[static_cast<size_t>(i) * p + a] = s;
+      }
+    femm::Orthonormalise(&next, S, p);
+    z = std::move(next);
+  }
+  // Leading K Ritz pairs: y_k (columns of z), lambda_k.
+  double total = 0.0, kept = 0.0;
+  {
+    // Trace of G = total centred energy.
+    std::vector<double> gdiag(S);
+    for (int i = 0; i < S; ++i)
+      if (hipMemcpy(&gdiag[i], G.get() + static_cast<size_t>(i) * S + i, sizeof(double),
+                    hipMemcpyDeviceToHost) != hipSuccess)
+        return FEMM_INTERNAL_ERROR;
+    for (double g : gdiag)
+      total += g;
+  }
+  std::vector<double> yk(static_cast<size_t>(S) * K), sig(K), isig(K);
+  for (int k = 0; k < K; ++k) {
+    const double lam = std::max(prev_ritz[k], 0.0);
+    sig[k] = std::sqrt(lam);
+    isig[k] = sig[k] > 0.0 ? 1.0 / sig[k] : 0.0;
+    kept += lam;
+    for (int i = 0; i < S; ++i)
+      yk[static_cast<size_t>(i) * K + k] = z[static_cast<size_t>(i) * p + k];
+  }
+  ws->report.pod_energy_fraction = total > 0.0 ? kept / total : 1.0;
+  ws->report.pod_leading_sigma = sig[0];
+  ws->report.pod_last_sigma = sig[K - 1];
+  if (!Y.Allocate(yk.size()) || !inv_sigma.Allocate(K) ||
+      !ws->phi.Allocate(static_cast<size_t>(K) * N))
+    return FEMM_GPU_ALLOCATION_FAILED;
+  if (hipMemcpy(Y.get(), yk.data(), sizeof(double) * yk.size(), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(inv_sigma.get(), isig.data(), sizeof(double) * K, hipMemcpyHostToDevice) !=
+          hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  hipLaunchKernelGGL(femm::ModesKernel, dim3(femm::Blocks(N), (K + 15) / 16), dim3(256), 0, stream,
+                     states_dev, ws->mean.get(), Y.get(), inv_sigma.get(), S, N, K,
+                     ws->phi.get());
+  // Training coefficients c_{s,k} = sigma_k y_{s,k}; FP64 linear fit c ~ L0 + L x_n.
+  std::vector<double> coef(static_cast<size_t>(S) * K);
+  for (int s = 0; s < S; ++s)
+    for (int k = 0; k < K; ++k)
+      coef[static_cast<size_t>(s) * K + k] = sig[k] * yk[static_cast<size_t>(s) * K + k];
+  const int q = in + 1;
+  std::vector<double> xtx(static_cast<size_t>(q) * q, 0.0), xtc(static_cast<size_t>(q) * K, 0.0);
+  std::vector<double> row(q);
+  for (int s = 0; s < S; ++s) {
+    row[0] = 1.0;
+    for (int j = 0; j < in; ++j)
+      row[1 + j] = (xh[static_cast<size_t>(s) * in + j] - fm[j]) / fs[j];
+    for (int a = 0; a < q; ++a) {
+      for (int b = 0; b < q; ++b)
+        xtx[static_cast<size_t>(a) * q + b] += row[a] * row[b];
+      for (int k = 0; k < K; ++k)
+        xtc[static_cast<size_t>(a) * K + k] += row[a] * coef[static_cast<size_t>(s) * K + k];
+    }
+  }
+  if (!tr.linear_skip)
+    std::fill(xtc.begin(), xtc.end(), 0.0);
+  else if (!femm::CholeskySolve(xtx, q, &xtc, K))
+    return FEMM_INTERNAL_ERROR;
+  std::vector<double> lin(static_cast<size_t>(K) * q);   // [K][q]
+  for (int k = 0; k < K; ++k)
+    for (int a = 0; a < q; ++a)
+      lin[static_cast<size_t>(k) * q + a] = xtc[static_cast<size_t>(a) * K + k];
+  // Residual coefficients -> MLP targets with one global scale (loss ~ ||A error||^2).
+  std::vector<double> resid(coef.size());
+  double ss = 0.0, cs = 0.0;
+  for (int s = 0; s < S; ++s) {
+    row[0] = 1.0;
+    for (int j = 0; j < in; ++j)
+      row[1 + j] = (xh[static_cast<size_t>(s) * in + j] - fm[j]) / fs[j];
+    for (int k = 0; k < K; ++k) {
+      double l = 0.0;
+      for (int a = 0; a < q; ++a)
+        l += lin[static_cast<size_t>(k) * q + a] * row[a];
+      const double r = coef[static_cast<size_t>(s) * K + k] - l;
+      resid[static_cast<size_t>(s) * K + k] = r;
+      ss += r * r;
+      cs += coef[static_cast<size_t>(s) * K + k] * coef[static_cast<size_t>(s) * K + k];
+    }
+  }
+  ws->scale = ss > 0.0 ? std::sqrt(ss / (static_cast<double>(S) * K)) : 1.0;
+  ws->report.linear_residual_fraction = cs > 0.0 ? std::sqrt(ss / cs) : 0.0;
+  std::vector<float> target(resid.size());
+  for (size_t i = 0; i < resid.size(); ++i)
+    target[i] = static_cast<float>(resid[i] / ws->scale);
+  std::vector<double> finv(in);
+  for (int j = 0; j < in; ++j)
+    finv[j] = 1.0 / fs[j];
+  DeviceArray<float> t_dev, hist;
+  if (!ws->lin.Allocate(lin.size()) || !ws->f_mean.Allocate(in) || !ws->f_inv_std.Allocate(in) ||
+      !t_dev.Allocate(target.size()) || !hist.Allocate(std::max(tr.steps, 1)))
+    return FEMM_GPU_ALLOCATION_FAILED;
+  if (hipMemcpy(ws->lin.get(), lin.data(), sizeof(double) * lin.size(), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(ws->f_mean.get(), fm.data(), sizeof(double) * in, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(ws->f_inv_std.get(), finv.data(), sizeof(double) * in, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(t_dev.get(), target.data(), sizeof(float) * target.size(),
+                hipMemcpyHostToDevice) != hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  int st = ws->net.SetStandardisation(fm.data(), fs.data());
+  if (st != FEMM_OK)
+    return st;
+  st = femm::TrainMlp(&ws->net, S, x_dev, t_dev.get(), tr.steps, tr.batch, tr.learning_rate,
+                      tr.seed, hist.get(), stream);
+  if (st != FEMM_OK)
+    return st;
+  std::vector<float> h(std::max(tr.steps, 1));
+  if (hipStreamSynchronize(stream) != hipSuccess ||
+      hipMemcpy(h.data(), hist.get(), sizeof(float) * h.size(), hipMemcpyDeviceToHost) !=
+          hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  ws->report.first_loss = tr.steps > 0 ? h.front() : 0.0f;
+  ws->report.final_loss = tr.steps > 0 ? h[tr.steps - 1] : 0.0f;
+  ws->report.coefficient_scale = ws->scale;
+  ws->report.samples = S;
+  ws->fitted = true;
+  return FEMM_OK;
+}
+
+int Predict(femm_warm_start *ws, int items, const double *features_dev, hipStream_t s)
+{
+  if (items > ws->capacity) {
+    if (!ws->a0.Allocate(static_cast<size_t>(items) * ws->N))
+      return FEMM_GPU_ALLOCATION_FAILED;
+    ws->capacity = items;
+  }
+  if (!ws->net.act.Ensure(ws->net.L, items))
+    return FEMM_GPU_ALLOCATION_FAILED;
+  ws->net.Pack(features_dev, nullptr, items, s);
+  ws->net.Forward(items, s);
+  hipLaunchKernelGGL(femm::ExpandKernel, dim3(items), dim3(256), sizeof(double) * ws->modes, s,
+                     features_dev, ws->f_mean.get(), ws->f_inv_std.get(), ws->lin.get(),
+                     ws->net.act.y.get(), ws->net.L.out_p, ws->scale, ws->mean.get(),
+                     ws->phi.get(), ws->inputs, ws->modes, ws->N, ws->a0.get());
+  return hipGetLastError() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
+}
+
+}  // namespace
+
+extern "C" {
+
+int femm_warm_start_create(femm_plan *plan, int32_t inputs, int32_t modes, int32_t hidden,
+                           uint64_t seed, femm_warm_start **out)
+{
+  if (out == nullptr || plan == nullptr || inputs <= 0 || modes <= 0 || modes > 1024)
+    return FEMM_INVALID_ARGUMENT;
+  *out = nullptr;
+  int st = femm::EnsurePlanDevice(plan);
+  if (st != FEMM_OK)
+    return st;
+  auto *ws = new (std::nothrow) femm_warm_start;
+  if (ws == nullptr)
+    return FEMM_INTERNAL_ERROR;
+  ws->plan = plan;
+  ws->inputs = inputs;
+  ws->modes = modes;
+  ws->N = plan->host.N;
+  st = ws->net.Init(inputs, hidden, modes, seed);
+  if (st != FEMM_OK) {
+    delete ws;
+    return st;
+  }
+  *out = ws;
+  return FEMM_OK;
+}
+
+void femm_warm_start_destroy(femm_warm_start *ws)
+{
+  if (ws == nullptr)
+    return;
+  (void)hipDeviceSynchronize();
+  delete ws;
+}
+
+int femm_warm_start_fit_device(femm_warm_start *ws, int32_t samples, const double *features,
+                               const double *states, const femm_warm_start_train_t *train,
+                               void *stream)
+{
+  if (ws == nullptr || samples <= 0 || features == nullptr || states == nullptr ||
+      train == nullptr || train->steps < 0 || train->batch <= 0 ||
+      !(train->learning_rate > 0.0f) || !std::isfinite(train->learning_rate))
+    return FEMM_INVALID_ARGUMENT;
+  ws->fitted = false;
+  ws->report = femm_warm_start_report_t{};
+  return FitDevice(ws, samples, features, states, *train, static_cast<hipStream_t>(stream));
+}
+
+int femm_warm_start_fit(femm_warm_start *ws, int32_t samples, const double *features,
+                        const double *states, const femm_warm_start_train_t *train)
+{
+  if (ws == nullptr || samples <= 0 || features == nullptr || states == nullptr)
+    return FEMM_INVALID_ARGUMENT;
+  femm::DeviceArray<double> x, a;
+  if (!x.Allocate(static_cast<size_t>(samples) * ws->inputs) ||
+      !a.Allocate(static_cast<size_t>(samples) * ws->N))
+    return FEMM_GPU_ALLOCATION_FAILED;
+  if (hipMemcpy(x.get(), features, sizeof(double) * samples * ws->inputs, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(a.get(), states, sizeof(double) * samples * static_cast<size_t>(ws->N),
+                hipMemcpyHostToDevice) != hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  const int st = femm_warm_start_fit_device(ws, samples, x.get(), a.get(), train, nullptr);
+  if (st == FEMM_OK && hipDeviceSynchronize() != hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  return st;
+}
+
+int femm_warm_start_report(const femm_warm_start *ws, femm_warm_start_report_t *report)
+{
+  if (ws == nullptr || report == nullptr)
+    return FEMM_INVALID_ARGUMENT;
+  *report = ws->report;
+  return ws->fitted ? FEMM_OK : FEMM_INVALID_ARGUMENT;
+}
+
+int femm_warm_start_predict(femm_warm_start *ws, int32_t items, const double *features,
+                            double *a0)
+{
+  if (ws == nullptr || items <= 0 || features == nullptr || a0 == nullptr)
+    return FEMM_INVALID_ARGUMENT;
+  if (!ws->fitted)
+    return FEMM_INVALID_ARGUMENT;
+  femm::DeviceArray<double> x;
+  if (!x.Allocate(static_cast<size_t>(items) * ws->inputs))
+    return FEMM_GPU_ALLOCATION_FAILED;
+  if (hipMemcpy(x.get(), features, sizeof(double) * items * ws->inputs, hipMemcpyHostToDevice) !=
+      hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  int st = Predict(ws, items, x.get(), nullptr);
+  if (st != FEMM_OK)
+    return st;
+  if (hipDeviceSynchronize() != hipSuccess ||
+      hipMemcpy(a0, ws->a0.get(), sizeof(double) * items * static_cast<size_t>(ws->N),
+                hipMemcpyDeviceToHost) != hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  return FEMM_OK;
+}
+
+int femm_newton_batch_learned_device(femm_plan *plan, femm_warm_start *ws, int32_t items,
+                                     const int32_t *variant_of_item, const double *features,
+                                     const double *currents, const femm_newton_options_t *options,
+                                     const femm_solve_config_t *config,
+                                     int32_t warm_newton_iterations, double *a, double *flux,
+                                     femm_item_result_t *results, void *stream)
+{
+  if (plan == nullptr || ws == nullptr || ws->plan != plan || items <= 0 || features == nullptr ||
+      currents == nullptr || options == nullptr || config == nullptr || a == nullptr ||
+      flux == nullptr || results == nullptr || warm_newton_iterations < 0)
+    return FEMM_INVALID_ARGUMENT;
+  if (!ws->fitted)
+    return FEMM_INVALID_ARGUMENT;
+  if (config->precision != FEMM_PRECISION_FP64 && config->precision != FEMM_PRECISION_MIXED)
+    return FEMM_INVALID_ARGUMENT;
+  int st = femm::EnsurePlanDevice(plan);
+  if (st == FEMM_OK)
+    st = femm::EnsureWork(plan->host, items, config->precision == FEMM_PRECISION_MIXED,
+                          &plan->dev);
+  if (st != FEMM_OK)
+    return st;
+  if (variant_of_item != nullptr && plan->dev.variants.count == 0)
+    return FEMM_INVALID_ARGUMENT;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if ((st = Predict(ws, items, features, s)) != FEMM_OK)
+    return st;
+  // One launch: per item, the safeguarded Newton (full steps, Armijo on the equation
+  // residual, both acceptance gates) from the predicted start within the warm budget, and
+  // if not accepted, the reference driver from the plain start in the same workgroup.
+  femm_newton_options_t warm_opts = *options;
+  warm_opts.require_equation_residual = 1;
+  warm_opts.max_newton_iterations = std::min(warm_newton_iterations, options->max_newton_iterations);
+  femm_newton_options_t plain_opts = *options;
+  plain_opts.require_equation_residual = 1;
+  femm::LaunchExtras x;
+  x.variant_of_item = variant_of_item;
+  if ((st = femm::LaunchLearned(plan->dev, items, currents, warm_opts, plain_opts, *config,
+                                ws->a0.get(), a, flux, results, s, x)) != FEMM_OK)
+    return st;
+  if (!options->recover_nonconvergence || options->max_newton_iterations <= 0)
+    return FEMM_OK;
+  // Bounded recovery of items the plain driver left unconverged.
+  x.prior = results;
+  x.prior_mode = femm::kPriorRedoUnconverged;
+  return femm::LaunchSafeguarded(plan->dev, items, currents, plain_opts, *config, nullptr, a, flux,
+                                 results, s, x);
+}
+
+int femm_newton_batch_learned(femm_plan *plan, femm_warm_start *ws, int32_t items,
+                              const int32_t *variant_of_item, const double *features,
+                              const double *currents, const femm_newton_options_t *options,
+                              const femm_solve_config_t *config, int32_t warm_newton_iterations,
+                              double *a, double *flux, femm_item_result_t *results)
+{
+  if (plan == nullptr || ws == nullptr || items <= 0 || features == nullptr ||
+      currents == nullptr || a == nullptr || flux == nullptr || results == nullptr)
+    return FEMM_INVALID_ARGUMENT;
+  const size_t B = static_cast<size_t>(items);
+  const int C = std::max(plan->host.C, 1);
+  femm::DeviceArray<double> df, dc, da, dflux;
+  femm::DeviceArray<femm_item_result_t> dres;
+  femm::DeviceArray<int32_t> dv;
+  if (!df.Allocate(B * ws->inputs) || !dc.Allocate(B * C) || !da.Allocate(B * plan->host.N) ||
+      !dflux.Allocate(B * C) || !dres.Allocate(B) || (variant_of_item && !dv.Allocate(B)))
+    return FEMM_GPU_ALLOCATION_FAILED;
+  if (hipMemcpy(df.get(), features, sizeof(double) * B * ws->inputs, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(dc.get(), currents, sizeof(double) * B * plan->host.C, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      (variant_of_item && hipMemcpy(dv.get(), variant_of_item, sizeof(int32_t) * B,
+                                    hipMemcpyHostToDevice) != hipSuccess))
+    return FEMM_INTERNAL_ERROR;
+  int st = femm_newton_batch_learned_device(plan, ws, items, variant_of_item ? dv.get() : nullptr,
+                                            df.get(), dc.get(), options, config,
+                                            warm_newton_iterations, da.get(), dflux.get(),
+                                            dres.get(), nullptr);
+  if (st != FEMM_OK)
+    return st;
+  if (hipDeviceSynchronize() != hipSuccess ||
+      hipMemcpy(a, da.get(), sizeof(double) * B * plan->host.N, hipMemcpyDeviceToHost) !=
+          hipSuccess ||
+      hipMemcpy(flux, dflux.get(), sizeof(double) * B * plan->host.C, hipMemcpyDeviceToHost) !=
+          hipSuccess ||
+      hipMemcpy(results, dres.get(), sizeof(femm_item_result_t) * B, hipMemcpyDeviceToHost) !=
+          hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  return FEMM_OK;
+}
+
+}  // extern "C"

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 
 namespace femm
@@ -208,7 +209,31 @@ bool BuildSpline(Spline *s)
   return true;
 }
 
-int BuildHostPlan(const femm_model_t *m, HostPlan *p, std::string *error)
+bool SameTopology(const femm_model_t *a, const femm_model_t *b)
+{
+  if (a == nullptr || b == nullptr)
+    return false;
+  if (a->problem_type != b->problem_type || a->node_count != b->node_count ||
+      a->triangle_count != b->triangle_count || a->material_count != b->material_count ||
+      a->curve_count != b->curve_count || a->dirichlet_count != b->dirichlet_count ||
+      a->constraint_count != b->constraint_count || a->circuit_count != b->circuit_count ||
+      !(a->depth_m == b->depth_m) || a->node_count <= 0 || a->triangle_count <= 0)
+    return false;
+  auto same = [](const void *x, const void *y, size_t bytes) {
+    return bytes == 0 || (x != nullptr && y != nullptr && std::memcmp(x, y, bytes) == 0);
+  };
+  const size_t N = static_cast<size_t>(a->node_count), T = static_cast<size_t>(a->triangle_count);
+  return same(a->node_xy, b->node_xy, sizeof(double) * 2 * N) &&
+         same(a->triangle_nodes, b->triangle_nodes, sizeof(int32_t) * 3 * T) &&
+         same(a->curve_offsets, b->curve_offsets,
+              a->curve_count > 0 ? sizeof(int32_t) * (a->curve_count + 1) : 0) &&
+         same(a->dirichlet_nodes, b->dirichlet_nodes, sizeof(int32_t) * a->dirichlet_count) &&
+         same(a->dirichlet_values, b->dirichlet_values, sizeof(double) * a->dirichlet_count) &&
+         same(a->constraints, b->constraints,
+              sizeof(femm_node_constraint_t) * a->constraint_count);
+}
+
+int BuildHostPlan(const femm_model_t *m, HostPlan *p, std::string *error, const HostPlan *topology)
 {
   auto fail = [error](int code, const char *why) {
     if (error)
@@ -280,58 +305,67 @@ int BuildHostPlan(const femm_model_t *m, HostPlan *p, std::string *error)
     if (!Basis(x, y, true, &basis[e]))
       return fail(FEMM_MESH_INVALID, "triangle basis");
   }
-  // Signed DOF map.
-  SignedUnionFind uf(N);
-  for (int k = 0; k < m->constraint_count; ++k) {
-    const femm_node_constraint_t &c = m->constraints[k];
-    if (c.node_a < 0 || c.node_b < 0 || c.node_a >= N || c.node_b >= N ||
-        (c.sign != 1 && c.sign != -1) || c.node_a == c.node_b)
-      return fail(FEMM_BOUNDARY_INVALID, "invalid node constraint");
-    if (!uf.Unite(c.node_a, c.node_b, static_cast<int8_t>(c.sign)))
-      return fail(FEMM_BOUNDARY_INVALID, "contradictory constraint cycle");
-  }
-  std::vector<int32_t> root(N);
-  p->node_sign.assign(N, 1);
-  for (int32_t v = 0; v < N; ++v) {
-    auto rs = uf.Find(v);
-    root[v] = rs.first;
-    p->node_sign[v] = rs.second;
-  }
-  std::vector<char> root_fixed(N, 0);
-  std::vector<double> root_value(N, 0.0);
-  for (int i = 0; i < m->dirichlet_count; ++i) {
-    const int32_t v = m->dirichlet_nodes[i];
-    const double rv = static_cast<double>(p->node_sign[v]) * m->dirichlet_values[i];
-    if (root_fixed[root[v]] && root_value[root[v]] != rv)
-      return fail(FEMM_BOUNDARY_INVALID, "inconsistent fixed value");
-    root_fixed[root[v]] = 1;
-    root_value[root[v]] = rv;
-  }
-  std::vector<int32_t> root_to_free(N, -1);
-  for (int32_t v = 0; v < N; ++v)
-    if (root[v] == v && !root_fixed[v]) {
-      root_to_free[v] = static_cast<int32_t>(p->free_nodes.size());
-      p->free_nodes.push_back(v);
+  if (topology != nullptr) {
+    // Variant of an existing plan: same constraints and Dirichlet data, same DOF map.
+    p->node_sign = topology->node_sign;
+    p->node_to_free = topology->node_to_free;
+    p->free_nodes = topology->free_nodes;
+    p->boundary = topology->boundary;
+    p->any_boundary_value = topology->any_boundary_value;
+  } else {
+    // Signed DOF map.
+    SignedUnionFind uf(N);
+    for (int k = 0; k < m->constraint_count; ++k) {
+      const femm_node_constraint_t &c = m->constraints[k];
+      if (c.node_a < 0 || c.node_b < 0 || c.node_a >= N || c.node_b >= N ||
+          (c.sign != 1 && c.sign != -1) || c.node_a == c.node_b)
+        return fail(FEMM_BOUNDARY_INVALID, "invalid node constraint");
+      if (!uf.Unite(c.node_a, c.node_b, static_cast<int8_t>(c.sign)))
+        return fail(FEMM_BOUNDARY_INVALID, "contradictory constraint cycle");
     }
-  if (p->free_nodes.empty())
-    return fail(FEMM_BOUNDARY_INVALID, "no free DOF");
-  p->node_to_free.assign(N, -1);
-  p->boundary.assign(N, 0.0);
-  for (int32_t v = 0; v < N; ++v) {
-    p->node_to_free[v] = root_to_free[root[v]];
-    if (p->node_to_free[v] < 0) {
-      p->boundary[v] = static_cast<double>(p->node_sign[v]) * root_value[root[v]];
-      p->any_boundary_value = p->any_boundary_value || p->boundary[v] != 0.0;
+    std::vector<int32_t> root(N);
+    p->node_sign.assign(N, 1);
+    for (int32_t v = 0; v < N; ++v) {
+      auto rs = uf.Find(v);
+      root[v] = rs.first;
+      p->node_sign[v] = rs.second;
     }
-  }
-  if (!planar) {
-    double scale = 1e-3;
+    std::vector<char> root_fixed(N, 0);
+    std::vector<double> root_value(N, 0.0);
+    for (int i = 0; i < m->dirichlet_count; ++i) {
+      const int32_t v = m->dirichlet_nodes[i];
+      const double rv = static_cast<double>(p->node_sign[v]) * m->dirichlet_values[i];
+      if (root_fixed[root[v]] && root_value[root[v]] != rv)
+        return fail(FEMM_BOUNDARY_INVALID, "inconsistent fixed value");
+      root_fixed[root[v]] = 1;
+      root_value[root[v]] = rv;
+    }
+    std::vector<int32_t> root_to_free(N, -1);
     for (int32_t v = 0; v < N; ++v)
-      scale = std::max(scale, m->node_xy[2 * v]);
-    for (int32_t v = 0; v < N; ++v)
-      if (m->node_xy[2 * v] <= 1e-12 * scale &&
-          (p->node_to_free[v] >= 0 || p->boundary[v] != 0.0))
-        return fail(FEMM_BOUNDARY_INVALID, "axis node not fixed to zero");
+      if (root[v] == v && !root_fixed[v]) {
+        root_to_free[v] = static_cast<int32_t>(p->free_nodes.size());
+        p->free_nodes.push_back(v);
+      }
+    if (p->free_nodes.empty())
+      return fail(FEMM_BOUNDARY_INVALID, "no free DOF");
+    p->node_to_free.assign(N, -1);
+    p->boundary.assign(N, 0.0);
+    for (int32_t v = 0; v < N; ++v) {
+      p->node_to_free[v] = root_to_free[root[v]];
+      if (p->node_to_free[v] < 0) {
+        p->boundary[v] = static_cast<double>(p->node_sign[v]) * root_value[root[v]];
+        p->any_boundary_value = p->any_boundary_value || p->boundary[v] != 0.0;
+      }
+    }
+    if (!planar) {
+      double scale = 1e-3;
+      for (int32_t v = 0; v < N; ++v)
+        scale = std::max(scale, m->node_xy[2 * v]);
+      for (int32_t v = 0; v < N; ++v)
+        if (m->node_xy[2 * v] <= 1e-12 * scale &&
+            (p->node_to_free[v] >= 0 || p->boundary[v] != 0.0))
+          return fail(FEMM_BOUNDARY_INVALID, "axis node not fixed to zero");
+    }
   }
   const int32_t n = static_cast<int32_t>(p->free_nodes.size());
   p->N = N;
@@ -424,6 +458,9 @@ int BuildHostPlan(const femm_model_t *m, HostPlan *p, std::string *error)
       }
     }
   }
+
+  if (topology != nullptr)
+    return FEMM_OK;
 
   // Symbolic CSR: sorted unique (row, col) among free roots of every triangle
   // coupling (the rule of nonlinear_topology.cpp:37-85), built by a stable sort of

@@ -60,7 +60,17 @@ struct HostPlan {
 };
 
 // Validates like ValidateNonlinearModel (reference physics/nonlinear_model.cpp:223-319) and
-// builds the plan.  Returns a femm status code.
-int BuildHostPlan(const femm_model_t *model, HostPlan *plan, std::string *error);
+// builds the plan.  Returns a femm status code.  With `topology` (a plan built from a
+// model for which SameTopology holds), only the material-dependent parts are built
+// (materials, splines, per-triangle material / circuit / flux weights, PM and circuit
+// right-hand sides); the DOF map is taken from `topology` and the symbolic CSR and SELL
+// layout are left empty.
+int BuildHostPlan(const femm_model_t *model, HostPlan *plan, std::string *error,
+                  const HostPlan *topology = nullptr);
+
+// True when `variant` shares `base`'s mesh, Dirichlet nodes and values, node constraints,
+// problem type, depth, circuit count, material count and B-H knot layout, so that only
+// material values, the triangle-to-material map and B-H knot values may differ.
+bool SameTopology(const femm_model_t *base, const femm_model_t *variant);
 
 }  // namespace femm

@@ -8,7 +8,9 @@ meshes with Triangle outside the solve); node counts and seeds are recorded in
 * :func:`square_coil_ring`  -- planar linear air box with a square coil and an
   iron ring (structured grid, jittered interior).
 * :func:`motor_sector`      -- planar 12-slot / 4-pole surface-PM motor, one pole
-  pitch (90 deg) with anti-periodic sector boundaries, B-H iron, 3 phase coils.
+  pitch (90 deg) with anti-periodic sector boundaries, B-H iron, 3 phase coils
+  (:class:`MotorFamily` meshes once and varies materials / rotor position per sample;
+  :func:`study_family` / :func:`study_samples` define the 11-input learned study).
 * :func:`c_core`            -- planar C-core with an air gap, an NdFeB block and two
   coils, Delaunay-meshed from seeded points.
 * :func:`solenoid_rz`       -- axisymmetric solenoid with a plunger.
@@ -104,101 +106,184 @@ _MOTOR_BANDS = [  # (r_start, r_end, intervals)
 ]
 
 
-def motor_sector(angular_cells=480, seed=20251203, ms=1.9, hc=(9e5, 9e5), fill=(1.0, 1.0, 1.0),
-                 jitter=0.15, radial_scale=1):
-    """One pole pitch (90 deg) of a 12-slot / 4-pole surface-PM machine, anti-periodic.
+class MotorFamily:
+    """One pole pitch (90 deg) of a 12-slot / 4-pole surface-PM machine, anti-periodic,
+    meshed once; :meth:`model` assigns materials for one sample.
 
     Structured polar grid: radial bands of _MOTOR_BANDS (x radial_scale) by
-    `angular_cells` sectors (must be a multiple of 160 so every slot / magnet edge lies
-    on a grid line); interior nodes jittered by U(-jitter, jitter) of the local spacing.
-    The default (480) gives 122 x 481 = 58,682 nodes and 116,160 triangles.
+    `angular_cells` sectors (a multiple of 160 so every slot / magnet edge lies on a grid
+    line); interior nodes jittered by U(-jitter, jitter) of the local spacing, except on
+    material-interface lines and, with `structured_magnet_band`, everywhere inside the
+    magnet band (so the magnets can be moved by whole cells on the same mesh).
 
-    Materials: rotor and stator iron with B = Ms tanh(H/300) + mu0 H (40 knots), two
-    parallel-magnetised NdFeB blocks (mu_r 1.05, Hc from `hc`, angles 27 and 63 deg),
-    air, and three slot coils carrying phases A, -C, B (turns = +-area times the slot
-    fill factor, so the phase current is the current density in A/m^2).  A = 0 on the
-    shaft (r = 10 mm) and outer (r = 75 mm) arcs; nodes on theta = 0 and theta = 90 deg
-    are anti-periodic pairs.  Returns (model, nominal currents)."""
-    if angular_cells % 160:
-        raise ValueError("angular_cells must be a multiple of 160")
-    radii = [np.array([_MOTOR_BANDS[0][0]])]
-    for r0, r1, k in _MOTOR_BANDS:
-        radii.append(np.linspace(r0, r1, k * radial_scale + 1)[1:])
-    r_nodes = np.concatenate(radii) * 1e-3
-    nr, nt = len(r_nodes), angular_cells + 1
-    theta = np.linspace(0.0, 0.5 * math.pi, nt)
-    R, TH = np.meshgrid(r_nodes, theta, indexing="ij")       # node (i, j) = i * nt + j
+    Slots of half-width 6 deg (openings 1.5 deg) at `slot_centres_deg` carry the coils of
+    `slot_phases` (circuit, sign) pairs.  The default (15, 45, 75) gives three full slots
+    with phases A, -C, B; (0, 30, 60, 90) cuts the sector through slot centres, giving two
+    half slots (A, and its anti-periodic image -A) and two full slots (-C, B), i.e. four
+    coil regions.  Rotor: iron below r = 30 mm and two parallel-magnetised NdFeB blocks
+    (mu_r 1.05) between 30 and 34 mm spanning rotor angles 9..45 and 45..81 deg with
+    magnetisation angles 27 and 63 deg (both turning with the rotor).  A = 0 on the shaft
+    (r = 10 mm) and outer (r = 75 mm) arcs; nodes on theta = 0 and 90 deg are anti-periodic
+    pairs."""
+
+    def __init__(self, angular_cells=480, seed=20251203, jitter=0.15, radial_scale=1,
+                 slot_centres_deg=(15.0, 45.0, 75.0), slot_phases=((0, 1.0), (1, -1.0), (2, 1.0)),
+                 structured_magnet_band=False):
+        if angular_cells % 160:
+            raise ValueError("angular_cells must be a multiple of 160")
+        if len(slot_phases) != len(slot_centres_deg):
+            raise ValueError("one (circuit, sign) per slot")
+        self.angular_cells, self.seed = angular_cells, seed
+        self.slot_centres = tuple(float(c) for c in slot_centres_deg)
+        self.slot_phases = tuple((int(c), float(g)) for c, g in slot_phases)
+        radii = [np.array([_MOTOR_BANDS[0][0]])]
+        for r0, r1, k in _MOTOR_BANDS:
+            radii.append(np.linspace(r0, r1, k * radial_scale + 1)[1:])
+        r_nodes = np.concatenate(radii) * 1e-3
+        nr, nt = len(r_nodes), angular_cells + 1
+        theta = np.linspace(0.0, 0.5 * math.pi, nt)
+        R, TH = np.meshgrid(r_nodes, theta, indexing="ij")       # node (i, j) = i * nt + j
+        rng = np.random.default_rng(seed)
+        dr = np.gradient(r_nodes)[:, None] * np.ones((1, nt))
+        dth = (theta[1] - theta[0]) * np.ones((nr, nt))
+        interior = np.zeros((nr, nt), bool)
+        interior[1:-1, 1:-1] = True
+        # Keep every material-interface grid line straight so labels follow the geometry.
+        band_edges_mm = {b[0] for b in _MOTOR_BANDS} | {b[1] for b in _MOTOR_BANDS} | {38.0, 57.0}
+        r_mm = r_nodes * 1e3
+        on_radial_edge = np.array([any(abs(r - e) < 1e-9 for e in band_edges_mm) for r in r_mm])
+        interior[on_radial_edge, :] = False
+        if structured_magnet_band:
+            interior[(r_mm > 30.0 - 1e-9) & (r_mm < 34.0 + 1e-9), :] = False
+        cell_deg = 90.0 / angular_cells
+        self.cell_deg = cell_deg
+        edge_deg = {9.0, 45.0, 81.0}
+        for c in self.slot_centres:
+            edge_deg |= {c - 6.0, c + 6.0, c - 1.5, c + 1.5}
+        for d in sorted(edge_deg):
+            if 0.0 < d < 90.0:
+                interior[:, int(round(d / cell_deg))] = False
+        Rj = R + np.where(interior, rng.uniform(-jitter, jitter, R.shape) * dr, 0.0)
+        THj = TH + np.where(interior, rng.uniform(-jitter, jitter, R.shape) * dth, 0.0)
+        self.xy = np.stack([Rj * np.cos(THj), Rj * np.sin(THj)], -1).reshape(-1, 2)
+
+        # cells (i, j): radial interval i between r_nodes[i], r_nodes[i+1]; angular j.
+        rc = 0.5 * (r_mm[:-1] + r_mm[1:])[:, None]
+        tc = ((np.arange(angular_cells) + 0.5) * cell_deg)[None, :]
+        self.rc, self.tc = np.broadcast_arrays(rc, tc)
+        tri = []
+        cell = []
+        for i in range(nr - 1):
+            for j in range(angular_cells):
+                a, b = i * nt + j, i * nt + j + 1
+                c, d = (i + 1) * nt + j, (i + 1) * nt + j + 1
+                # alternate the diagonal so the grid has no preferred direction
+                if (i + j) % 2 == 0:
+                    tri += [(a, c, d), (a, d, b)]
+                else:
+                    tri += [(a, c, b), (b, c, d)]
+                cell += [i * angular_cells + j] * 2
+        self.tri = _ccw(self.xy, np.asarray(tri, np.int32))
+        self.cell_of_tri = np.asarray(cell, np.int64)
+        self.area = _area(self.xy, self.tri)
+        self.outer = np.concatenate([np.arange(nt), (nr - 1) * nt + np.arange(nt)]).astype(np.int32)
+        self.pairs = np.asarray([(i * nt, i * nt + nt - 1, -1) for i in range(1, nr - 1)], np.int32)
+        self.nr, self.nt = nr, nt
+
+    AIR, ROTOR, STATOR, MAG1, MAG2, COIL0 = range(6)
+
+    @property
+    def coil_regions(self):
+        return len(self.slot_centres)
+
+    def labels(self, rotor_steps=0):
+        """Per-triangle material labels with the rotor turned by `rotor_steps` cells."""
+        rc, tc = self.rc, self.tc
+        phi = tc - rotor_steps * self.cell_deg          # rotor-frame angle
+        label = np.full(rc.shape, self.AIR, np.int32)
+        label[rc < 30] = self.ROTOR
+        band = (rc > 30) & (rc < 34)
+        label[band & (phi > 9) & (phi < 45)] = self.MAG1
+        label[band & (phi > 45) & (phi < 81)] = self.MAG2
+        stator = rc > 35
+        in_slot = np.zeros(rc.shape, bool)
+        in_opening = np.zeros(rc.shape, bool)
+        for s in self.slot_centres:
+            in_slot |= (rc > 37) & (rc < 58) & (np.abs(tc - s) < 6)
+            in_opening |= (rc > 35) & (rc < 37) & (np.abs(tc - s) < 1.5)
+        label[stator & ~in_slot & ~in_opening] = self.STATOR
+        for k, s in enumerate(self.slot_centres):
+            coil = (rc > 38) & (rc < 57) & (np.abs(tc - s) < 6)
+            label[coil] = self.COIL0 + k
+        return label.reshape(-1)[self.cell_of_tri].astype(np.int32)
+
+    def model(self, hc=(9e5, 9e5), ms=1.9, fill=None, rotor_steps=0, name=None):
+        """The sample's model: magnet coercivities `hc`, B-H saturation `ms` (both irons),
+        coil-region fill factors `fill` (turns = sign * fill * region area, so a phase
+        current in A/m^2 times the fill is the region's current density) and the rotor
+        turned by `rotor_steps` angular cells (magnetisation turning with it)."""
+        fill = tuple(fill) if fill is not None else (1.0,) * self.coil_regions
+        if len(fill) != self.coil_regions:
+            raise ValueError(f"{self.coil_regions} fill factors expected")
+        tlabel = self.labels(rotor_steps)
+        b, h = tanh_bh_curve(ms=ms)
+        turn = rotor_steps * self.cell_deg
+        mats = [Material(1.0 / MU0, group=0),
+                Material(1.0, bh_curve=0, group=1),
+                Material(1.0, bh_curve=0, group=2),
+                Material(1.0 / (MU0 * 1.05), hc=hc[0], magnetization_deg=27.0 + turn, group=3),
+                Material(1.0 / (MU0 * 1.05), hc=hc[1], magnetization_deg=63.0 + turn, group=4)]
+        for k, (circuit, sign) in enumerate(self.slot_phases):
+            coil_area = float(self.area[tlabel == self.COIL0 + k].sum())
+            turns = sign * fill[k] * coil_area
+            mats.append(Material(1.0 / MU0, source_j_per_a=turns / coil_area, circuit=circuit,
+                                 group=5 + k))
+        circuits = 1 + max(c for c, _ in self.slot_phases)
+        return Model(self.xy, self.tri, tlabel, mats, [(b, h)], self.outer,
+                     np.zeros(len(self.outer)), self.pairs, circuits, 0.05, PLANAR,
+                     name=name or f"motor_sector_12s4p_{self.nr}x{self.nt}_seed{self.seed}")
+
+
+def motor_sector(angular_cells=480, seed=20251203, ms=1.9, hc=(9e5, 9e5), fill=(1.0, 1.0, 1.0),
+                 jitter=0.15, radial_scale=1):
+    """The benchmark motor: :class:`MotorFamily` with three full slots (phases A, -C, B),
+    the rotor at its reference position.  The default (480 cells) gives 122 x 481 = 58,682
+    nodes and 116,160 triangles.  Returns (model, nominal currents)."""
+    family = MotorFamily(angular_cells=angular_cells, seed=seed, jitter=jitter,
+                         radial_scale=radial_scale)
+    return family.model(hc=hc, ms=ms, fill=fill), np.array([5e6, -3e6, -2e6])
+
+
+STUDY_SLOTS = (0.0, 30.0, 60.0, 90.0)
+STUDY_PHASES = ((0, 1.0), (1, -1.0), (2, 1.0), (0, -1.0))
+STUDY_ROTOR_STEPS = 15          # rotor index k in [-15, 15] turns the rotor by 3 k cells
+
+
+def study_family(angular_cells=480, seed=20251203):
+    """Motor family of the learned warm-start study: sector cut through slot centres (four
+    coil regions), structured magnet band (rotor positions on one mesh)."""
+    return MotorFamily(angular_cells=angular_cells, seed=seed, slot_centres_deg=STUDY_SLOTS,
+                       slot_phases=STUDY_PHASES, structured_magnet_band=True)
+
+
+def study_samples(count, seed):
+    """Seeded per-sample inputs (11 columns): 3 phase current densities U(-8e6, 8e6) A/m^2,
+    2 magnet coercivities U(7e5, 1e6) A/m, B-H saturation scale U(0.9, 1.1), 4 coil-region
+    fill factors U(0.4, 0.6), rotor index integer U[-15, 15] (rotor turned by 3 index cells)."""
     rng = np.random.default_rng(seed)
-    dr = np.gradient(r_nodes)[:, None] * np.ones((1, nt))
-    dth = (theta[1] - theta[0]) * np.ones((nr, nt))
-    interior = np.zeros((nr, nt), bool)
-    interior[1:-1, 1:-1] = True
-    # Keep every material-interface grid line straight so labels follow the geometry.
-    band_edges_mm = {b[0] for b in _MOTOR_BANDS} | {b[1] for b in _MOTOR_BANDS} | {38.0, 57.0}
-    r_mm = r_nodes * 1e3
-    on_radial_edge = np.array([any(abs(r - e) < 1e-9 for e in band_edges_mm) for r in r_mm])
-    interior[on_radial_edge, :] = False
-    cell_deg = 90.0 / angular_cells
-    edge_deg = [9, 13.5, 16.5, 21, 39, 43.5, 46.5, 51, 69, 73.5, 76.5, 81, 45]
-    for d in edge_deg:
-        interior[:, int(round(d / cell_deg))] = False
-    Rj = R + np.where(interior, rng.uniform(-jitter, jitter, R.shape) * dr, 0.0)
-    THj = TH + np.where(interior, rng.uniform(-jitter, jitter, R.shape) * dth, 0.0)
-    xy = np.stack([Rj * np.cos(THj), Rj * np.sin(THj)], -1).reshape(-1, 2)
+    x = np.empty((count, 11))
+    x[:, 0:3] = rng.uniform(-8e6, 8e6, (count, 3))
+    x[:, 3:5] = rng.uniform(7e5, 1e6, (count, 2))
+    x[:, 5] = rng.uniform(0.9, 1.1, count)
+    x[:, 6:10] = rng.uniform(0.4, 0.6, (count, 4))
+    x[:, 10] = rng.integers(-STUDY_ROTOR_STEPS, STUDY_ROTOR_STEPS + 1, count)
+    return x
 
-    # cells (i, j): radial interval i between r_nodes[i], r_nodes[i+1]; angular j.
-    rc = 0.5 * (r_mm[:-1] + r_mm[1:])[:, None]
-    tc = ((np.arange(angular_cells) + 0.5) * cell_deg)[None, :]
-    rc, tc = np.broadcast_arrays(rc, tc)
-    AIR, ROTOR, STATOR, MAG1, MAG2, COIL1, COIL2, COIL3 = range(8)
-    label = np.full(rc.shape, AIR, np.int32)
-    label[rc < 30] = ROTOR
-    label[(rc > 30) & (rc < 34) & (tc > 9) & (tc < 45)] = MAG1
-    label[(rc > 30) & (rc < 34) & (tc > 45) & (tc < 81)] = MAG2
-    stator = rc > 35
-    slot_centres = (15.0, 45.0, 75.0)
-    in_slot = np.zeros(rc.shape, bool)
-    in_opening = np.zeros(rc.shape, bool)
-    for s in slot_centres:
-        in_slot |= (rc > 37) & (rc < 58) & (np.abs(tc - s) < 6)
-        in_opening |= (rc > 35) & (rc < 37) & (np.abs(tc - s) < 1.5)
-    label[stator & ~in_slot & ~in_opening] = STATOR
-    for k, s in enumerate(slot_centres):
-        coil = (rc > 38) & (rc < 57) & (np.abs(tc - s) < 6)
-        label[coil] = COIL1 + k
-    tri = []
-    tlabel = []
-    for i in range(nr - 1):
-        for j in range(angular_cells):
-            a, b = i * nt + j, i * nt + j + 1
-            c, d = (i + 1) * nt + j, (i + 1) * nt + j + 1
-            # alternate the diagonal so the grid has no preferred direction
-            if (i + j) % 2 == 0:
-                tri += [(a, c, d), (a, d, b)]
-            else:
-                tri += [(a, c, b), (b, c, d)]
-            tlabel += [label[i, j]] * 2
-    tri = _ccw(xy, np.asarray(tri, np.int32))
-    tlabel = np.asarray(tlabel, np.int32)
-    area = _area(xy, tri)
-    b, h = tanh_bh_curve(ms=ms)
-    mats = [Material(1.0 / MU0, group=0),
-            Material(1.0, bh_curve=0, group=1),
-            Material(1.0, bh_curve=0, group=2),
-            Material(1.0 / (MU0 * 1.05), hc=hc[0], magnetization_deg=27.0, group=3),
-            Material(1.0 / (MU0 * 1.05), hc=hc[1], magnetization_deg=63.0, group=4)]
-    signs = (1.0, -1.0, 1.0)   # phases A, -C, B in the three slots of one pole pitch
-    for k in range(3):
-        coil_area = float(area[tlabel == COIL1 + k].sum())
-        turns = signs[k] * fill[k] * coil_area
-        mats.append(Material(1.0 / MU0, source_j_per_a=turns / coil_area, circuit=k,
-                             group=5 + k))
-    outer = np.concatenate([np.arange(nt), (nr - 1) * nt + np.arange(nt)]).astype(np.int32)
-    pairs = [(i * nt, i * nt + nt - 1, -1) for i in range(1, nr - 1)]
-    model = Model(xy, tri, tlabel, mats, [(b, h)], outer, np.zeros(len(outer)),
-                  np.asarray(pairs, np.int32), 3, 0.05, PLANAR,
-                  name=f"motor_sector_12s4p_{nr}x{nt}_seed{seed}")
-    return model, np.array([5e6, -3e6, -2e6])
+
+def study_model(family, sample):
+    """Model of one study sample (row of :func:`study_samples`)."""
+    return family.model(hc=(sample[3], sample[4]), ms=1.9 * sample[5], fill=tuple(sample[6:10]),
+                        rotor_steps=3 * int(round(sample[10])))
 
 
 def motor_currents(count, seed, low=-8e6, high=8e6):

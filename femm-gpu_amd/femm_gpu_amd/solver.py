@@ -64,9 +64,10 @@ def reference_config(**kw) -> SolveConfig:
 
 
 def mixed_config(**kw) -> SolveConfig:
-    """FP32 Jacobi PCG inside FP64 iterative refinement, starting from the current state."""
+    """FP32 Jacobi PCG with FP64 reliable updates (true residual replaced whenever the FP32
+    residual has dropped 10x), each Newton correction starting from the current state."""
     values = dict(precision=MIXED, pcg_warm_start=1, verify_equation_residual=1,
-                  max_refinements=64, inner_relative_tolerance=1e-4)
+                  max_refinements=64, inner_relative_tolerance=0.1)
     values.update(kw)
     return SolveConfig(**values)
 
@@ -202,6 +203,47 @@ class Plan:
             ctypes.byref(config), None if warm_ptr is None else ctypes.c_void_p(warm_ptr),
             ctypes.c_void_p(a_ptr), ctypes.c_void_p(flux_ptr), ctypes.c_void_p(results_ptr),
             ctypes.c_void_p(stream_ptr)), "femm_newton_batch_device")
+
+    def set_variants(self, models):
+        """Per-item material variants on this plan's mesh (femm_plan_set_variants)."""
+        models = list(models)
+        structs = (CModel * max(1, len(models)))()
+        for i, m in enumerate(models):
+            structs[i] = m.c_struct()
+        self._variant_models = models          # keep the arrays alive while the call runs
+        _check(self._lib.femm_plan_set_variants(self._handle, len(models), structs),
+               "femm_plan_set_variants")
+        self.variant_count = len(models)
+        self._variant_models = None
+
+    def solve_batch_variants(self, variant_of_item, currents, options: CNewtonOptions = None,
+                             config: SolveConfig = None, warm_starts=None):
+        """Batched Newton with item b on variant variant_of_item[b]."""
+        currents = np.ascontiguousarray(currents, np.float64).reshape(-1, self.C)
+        B = currents.shape[0]
+        index = np.ascontiguousarray(variant_of_item, np.int32).reshape(B)
+        options = options or production_options()
+        config = config or reference_config()
+        a, flux = np.zeros((B, self.N)), np.zeros((B, self.C))
+        results = np.zeros(B, RESULT_DTYPE)
+        warm = None if warm_starts is None else np.ascontiguousarray(warm_starts, np.float64)
+        _check(self._lib.femm_newton_batch_variants(
+            self._handle, B, _ip(index), _dp(currents), ctypes.byref(options),
+            ctypes.byref(config), None if warm is None else _dp(warm), _dp(a), _dp(flux),
+            results.ctypes.data_as(ctypes.c_void_p)), "femm_newton_batch_variants")
+        return dict(a=a, flux=flux, results=results)
+
+    def solve_batch_variants_device(self, variant_ptr, currents_ptr, items, a_ptr, flux_ptr,
+                                    results_ptr, stream_ptr, options: CNewtonOptions = None,
+                                    config: SolveConfig = None, warm_ptr=None):
+        options = options or production_options()
+        config = config or reference_config()
+        _check(self._lib.femm_newton_batch_variants_device(
+            self._handle, int(items), ctypes.c_void_p(variant_ptr), ctypes.c_void_p(currents_ptr),
+            ctypes.byref(options), ctypes.byref(config),
+            None if warm_ptr is None else ctypes.c_void_p(warm_ptr), ctypes.c_void_p(a_ptr),
+            ctypes.c_void_p(flux_ptr), ctypes.c_void_p(results_ptr), ctypes.c_void_p(stream_ptr)),
+            "femm_newton_batch_variants_device")
 
     def set_postprocess(self, selected_label=-1, air_label=-1, selected_group=-1, air_group=-1,
                         radius_m=0.0, angles_rad=(), max_mask_iterations=256,

@@ -27,12 +27,14 @@ enum { FEMM_PRECISION_FP64 = 0, FEMM_PRECISION_MIXED = 1 };
  * is the reference algorithm (x0 = 0 in every Newton correction). */
 typedef struct femm_solve_config {
   int32_t precision;           /* FEMM_PRECISION_FP64: FP64 Jacobi PCG.
-                                  FEMM_PRECISION_MIXED: FP32 values / SpMV / Jacobi inside FP64
-                                  iterative refinement (FP64 residual b - A x). */
+                                  FEMM_PRECISION_MIXED: FP32 values / SpMV / Jacobi / vectors with
+                                  FP64 reliable updates (the FP64 residual b - A x replaces the
+                                  FP32 one periodically; acceptance on the FP64 residual). */
   int32_t pcg_warm_start;      /* 1: each Newton correction starts PCG from the current state */
   int32_t verify_equation_residual; /* 1: compute ||K(A)A - b(A)|| / ||b(A0)|| per item */
-  int32_t max_refinements;     /* FP64 refinement cap in the mixed mode */
-  double inner_relative_tolerance; /* FP32 inner solve target relative to its right-hand side */
+  int32_t max_refinements;     /* cap on FP64 residual replacements per solve (mixed mode) */
+  double inner_relative_tolerance; /* mixed mode: replace the residual once the FP32 one has
+                                      dropped by this factor since the last replacement */
 } femm_solve_config_t;
 
 /* Per-item outcome (reference SolveInfo plus the Newton counters of
@@ -98,6 +100,29 @@ int femm_newton_batch_device(femm_plan *plan, int32_t items, const double *curre
                              const femm_solve_config_t *config, const double *warm_starts,
                              double *a, double *flux, femm_item_result_t *results, void *stream);
 
+/* Per-item material variants on the plan's mesh.  Each variant is a full model with the
+ * plan's nodes, triangles, Dirichlet nodes and values, node constraints, problem type,
+ * depth, circuit count, material count and B-H knot layout; its material values, the
+ * triangle-to-material map and the B-H knot values may differ (per-sample coercivity, B-H
+ * scale, coil fill factors, a rotor position on a fixed mesh).  The reference solves one
+ * model per NonlinearP1Solver (nonlinear_solver.cuh:32-38), so a variant item equals a
+ * reference solve of that variant's model.  count = 0 clears the set; a variant that fails
+ * validation leaves the set cleared and returns that variant's status. */
+int femm_plan_set_variants(femm_plan *plan, int32_t count, const femm_model_t *variants);
+
+/* femm_newton_batch(_device) with item b solved on variant variant_of_item[b] (an index
+ * outside the set gives that item INVALID_ARGUMENT). */
+int femm_newton_batch_variants(femm_plan *plan, int32_t items, const int32_t *variant_of_item,
+                               const double *currents, const femm_newton_options_t *options,
+                               const femm_solve_config_t *config, const double *warm_starts,
+                               double *a, double *flux, femm_item_result_t *results);
+int femm_newton_batch_variants_device(femm_plan *plan, int32_t items,
+                                      const int32_t *variant_of_item, const double *currents,
+                                      const femm_newton_options_t *options,
+                                      const femm_solve_config_t *config,
+                                      const double *warm_starts, double *a, double *flux,
+                                      femm_item_result_t *results, void *stream);
+
 /* Element B and circuit flux for given states (FinalizeResult,
  * solvers/nonlinear_solver.cu:16-83) on the device. */
 int femm_finalize_batch(femm_plan *plan, int32_t items, const double *states, double *bx,
@@ -141,6 +166,104 @@ const char *femm_status_name(int32_t status);
 
 /* Library / device description: writes a NUL-terminated string. */
 int femm_device_info(char *text, size_t capacity);
+
+/* ---------------------------------------------------------------------------
+ * Learned warm start.  The reference has a warm-start hook but no learned part
+ * (NonlinearP1Solver::Solve(..., warm_start), nonlinear_solver.cu:667-698: the start
+ * replaces the initial A, Dirichlet values are re-imposed and Newton terms are used
+ * from iteration 0).  The functions below have no counterpart in the reference; they
+ * produce starts for that hook.
+ * ------------------------------------------------------------------------- */
+
+/* MLP inputs -> hidden -> hidden -> outputs (SiLU), products in BF16 MFMA with FP32
+ * accumulation, FP32 master weights trained by Adam.  Parameters are one FP32 array:
+ * W1 [in_p][hidden], b1 [hidden], W2 [hidden][hidden], b2 [hidden], W3 [hidden][out_p],
+ * b3 [out_p], where in_p / out_p round inputs / outputs up to a multiple of 16 (padded
+ * entries are zero and never updated); y = x W + b per layer. */
+typedef struct femm_mlp femm_mlp;
+
+int femm_mlp_create(int32_t inputs, int32_t hidden, int32_t outputs, uint64_t seed,
+                    femm_mlp **out);
+void femm_mlp_destroy(femm_mlp *mlp);
+/* layout[10] = {inputs, hidden, outputs, in_p, out_p, parameter count, offset of b1,
+ * offset of W2, offset of W3, offset of b3} (W1 at 0, b2 right after W2). */
+int femm_mlp_layout(const femm_mlp *mlp, int64_t layout[10]);
+int femm_mlp_get_parameters(const femm_mlp *mlp, float *params);
+/* Sets FP32 master weights (BF16 copies re-rounded) and clears the Adam state. */
+int femm_mlp_set_parameters(femm_mlp *mlp, const float *params);
+/* Inputs are standardised as (x - mean) / stddev in FP64 before rounding to BF16. */
+int femm_mlp_set_standardization(femm_mlp *mlp, const double *mean, const double *stddev);
+/* y[rows][outputs] = MLP(x[rows][inputs]). */
+int femm_mlp_forward(femm_mlp *mlp, int32_t rows, const double *x, float *y);
+/* One forward + backward pass: loss = mean over rows x outputs of (y - target)^2 and its
+ * gradient with respect to every parameter (same layout as the parameters). */
+int femm_mlp_gradients(femm_mlp *mlp, int32_t rows, const double *x, const float *target,
+                       float *loss, float *gradients);
+/* Minibatch Adam (beta 0.9 / 0.999, eps 1e-8), learning rate cosine-decayed to 1 %,
+ * epoch-wise shuffles from `seed`; loss_history[steps] (optional) gets each step's loss. */
+int femm_mlp_train(femm_mlp *mlp, int32_t samples, const double *x, const float *target,
+                   int32_t steps, int32_t batch, float learning_rate, uint64_t seed,
+                   float *loss_history);
+
+/* Warm-start model for one plan: POD basis of `modes` modes from solver states, an FP64
+ * least-squares linear path and the MLP (inputs -> hidden -> hidden -> modes) fitted to
+ * the remaining coefficients.  A0 = mean + sum_k c_k phi_k. */
+typedef struct femm_warm_start femm_warm_start;
+
+typedef struct femm_warm_start_train {
+  int32_t steps;            /* Adam steps */
+  int32_t batch;            /* minibatch rows */
+  float learning_rate;      /* initial rate, cosine-decayed to 1 % */
+  int32_t pod_iterations;   /* cap on subspace iterations for the POD eigenproblem */
+  int32_t linear_skip;      /* 1: FP64 linear path, the MLP fits what it leaves */
+  int32_t reserved;
+  uint64_t seed;
+} femm_warm_start_train_t;
+
+typedef struct femm_warm_start_report {
+  int32_t samples;
+  int32_t pod_iterations;
+  double pod_energy_fraction;       /* retained share of the centred snapshot energy */
+  double pod_leading_sigma;
+  double pod_last_sigma;
+  double linear_residual_fraction;  /* ||c - linear(c)|| / ||c|| over the training set */
+  double coefficient_scale;         /* MLP output unit */
+  float first_loss;
+  float final_loss;
+} femm_warm_start_report_t;
+
+/* The plan must outlive the warm start. */
+int femm_warm_start_create(femm_plan *plan, int32_t inputs, int32_t modes, int32_t hidden,
+                           uint64_t seed, femm_warm_start **out);
+void femm_warm_start_destroy(femm_warm_start *ws);
+/* features[samples][inputs] and states[samples][N] (converged solutions). */
+int femm_warm_start_fit(femm_warm_start *ws, int32_t samples, const double *features,
+                        const double *states, const femm_warm_start_train_t *train);
+int femm_warm_start_fit_device(femm_warm_start *ws, int32_t samples, const double *features,
+                               const double *states, const femm_warm_start_train_t *train,
+                               void *stream);
+int femm_warm_start_report(const femm_warm_start *ws, femm_warm_start_report_t *report);
+/* a0[items][N] predicted starts. */
+int femm_warm_start_predict(femm_warm_start *ws, int32_t items, const double *features,
+                            double *a0);
+/* Batched solve from the learned start: the reference's safeguarded Newton (full steps,
+ * Armijo backtracking on the equation residual, SolveSafeguarded,
+ * nonlinear_solver.cu:534-659) with at most warm_newton_iterations iterations and both
+ * acceptance gates; every item not accepted there is solved again from the plain start by
+ * the reference driver (NonlinearP1Solver::SolveBatch policy) in the same call, followed by
+ * the bounded recovery when options->recover_nonconvergence is set.  Counters of the
+ * attempts add up per item. */
+int femm_newton_batch_learned(femm_plan *plan, femm_warm_start *ws, int32_t items,
+                              const int32_t *variant_of_item, const double *features,
+                              const double *currents, const femm_newton_options_t *options,
+                              const femm_solve_config_t *config, int32_t warm_newton_iterations,
+                              double *a, double *flux, femm_item_result_t *results);
+int femm_newton_batch_learned_device(femm_plan *plan, femm_warm_start *ws, int32_t items,
+                                     const int32_t *variant_of_item, const double *features,
+                                     const double *currents, const femm_newton_options_t *options,
+                                     const femm_solve_config_t *config,
+                                     int32_t warm_newton_iterations, double *a, double *flux,
+                                     femm_item_result_t *results, void *stream);
 
 #ifdef __cplusplus
 }

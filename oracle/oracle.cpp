@@ -968,8 +968,11 @@ double EquationResidual(const Model &m, const std::vector<double> &a,
 
 // nonlinear_solver.cu:534-659: host-assembled Newton with Armijo backtracking
 // on the equation residual.
+// With `warm`, the iteration starts from the warm state (free nodes taken from it, the
+// prescribed values kept) while the residual scale stays the norm at the plain start:
+// the warm-start hook of Solve (nonlinear_solver.cu:667-698) combined with this loop.
 int Safeguarded(const Model &m, const std::vector<double> &cur, const femm_newton_options_t &o,
-                std::vector<double> *state_out, oracle_result_t *res)
+                std::vector<double> *state_out, oracle_result_t *res, const double *warm = nullptr)
 {
   std::vector<double> state;
   if (!StartState(m, &state))
@@ -980,6 +983,15 @@ int Safeguarded(const Model &m, const std::vector<double> &cur, const femm_newto
     return st;
   double norm = EquationNorm(as, state);
   const double scale = std::max(norm, std::numeric_limits<double>::min());
+  if (warm) {
+    for (size_t v = 0; v < state.size(); ++v)
+      if (as.node_to_free[v] >= 0)
+        state[v] = warm[v];
+    st = Assemble(m, state, cur, true, &as);
+    if (st != FEMM_OK)
+      return st;
+    norm = EquationNorm(as, state);
+  }
   int status = FEMM_NONLINEAR_SOLVE_NOT_CONVERGED;
   for (int it = 0; it < o.max_newton_iterations; ++it) {
     if (!std::isfinite(norm)) {
@@ -1617,6 +1629,35 @@ int oracle_newton_solve(const femm_model_t *model, const double *currents,
       return result->status = fs;
   }
   return st;
+}
+
+int oracle_safeguarded_solve(const femm_model_t *model, const double *currents,
+                             const femm_newton_options_t *options, const double *warm_start,
+                             double *a, oracle_result_t *result)
+{
+  *result = oracle_result_t{};
+  result->status = FEMM_INVALID_ARGUMENT;
+  result->relative_step = std::numeric_limits<double>::infinity();
+  result->equation_residual = std::numeric_limits<double>::infinity();
+  Model m;
+  if (!LoadModel(model, &m))
+    return result->status;
+  const int v = Validate(m);
+  if (v != FEMM_OK)
+    return result->status = v;
+  const femm_newton_options_t &o = *options;
+  std::vector<double> cur(currents, currents + m.circuits);
+  if (!(o.relative_tolerance > 0.0) || !std::isfinite(o.relative_tolerance) ||
+      o.max_newton_iterations < 0)
+    return result->status;
+  for (double c : cur)
+    if (!std::isfinite(c))
+      return result->status;
+  std::vector<double> state;
+  const int st = Safeguarded(m, cur, o, &state, result, warm_start);
+  if (state.size() == m.nodes.size())
+    std::copy(state.begin(), state.end(), a);
+  return result->status = st;
 }
 
 int oracle_newton_batch(const femm_model_t *model, int items, const double *currents,

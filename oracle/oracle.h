@@ -83,6 +83,14 @@ int oracle_newton_solve(const femm_model_t *model, const double *currents,
                         double *a, double *bx, double *by, double *flux,
                         oracle_result_t *result);
 
+/* Safeguarded Newton alone (SolveSafeguarded, nonlinear_solver.cu:534-659: full
+ * steps with Armijo backtracking on the equation residual, exact linear tolerance).
+ * warm_start (may be NULL) replaces the free nodes of the plain start, as the Solve
+ * hook does (:667-698); the residual scale stays the plain start's. */
+int oracle_safeguarded_solve(const femm_model_t *model, const double *currents,
+                             const femm_newton_options_t *options, const double *warm_start,
+                             double *a, oracle_result_t *result);
+
 /* Batch of independent items on `threads` host threads (one item per thread at a
  * time, as the reference's host-assembly pool, nonlinear_solver.cu:371-396). */
 int oracle_newton_batch(const femm_model_t *model, int items, const double *currents,

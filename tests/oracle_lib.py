@@ -172,6 +172,20 @@ def newton_solve(model, currents, options, warm_start=None):
                 eq=res.equation_residual)
 
 
+def safeguarded_solve(model, currents, options, warm_start=None):
+    """SolveSafeguarded alone (optionally from a warm start)."""
+    a = np.zeros(model.node_count)
+    res = OracleResult()
+    currents = np.ascontiguousarray(currents, np.float64)
+    warm = None if warm_start is None else np.ascontiguousarray(warm_start, np.float64)
+    st = oracle().oracle_safeguarded_solve(ctypes.byref(model.c_struct()), _dp(currents),
+                                           ctypes.byref(options),
+                                           None if warm is None else _dp(warm), _dp(a),
+                                           ctypes.byref(res))
+    return dict(status=st, a=a, newton=res.newton_iterations, linear=res.linear_iterations,
+                step=res.relative_step, eq=res.equation_residual)
+
+
 def newton_batch(model, currents, options, threads):
     currents = np.ascontiguousarray(currents, np.float64)
     B = currents.shape[0]

@@ -1,0 +1,255 @@
+"""GPU checks of the safeguarded Newton kernel and the learned warm start (C ABI).
+
+* MLP forward / gradients: the device's BF16 MFMA kernels against the CPU restatement
+  with the same BF16 rounding points (oracle/mlp_oracle.py, itself pinned by finite
+  differences); FP32 accumulation order differs, so the bar is 2e-3 relative L2 per
+  parameter block (a BF16 rounding of an intermediate may flip by one unit).
+* Safeguarded Newton (nonlinear_solver.cu:534-659) and the recovery path: A within 1e-9
+  relative L2 of the oracle, same iteration counts.
+* Learned start: every item passes the 1e-6 step and 1e-8 equation-residual gates, A and
+  flux agree with the plain FP64 solve within the gates' own resolution (5e-8 / 1e-7),
+  the mean Newton iteration count drops, and a zero warm budget falls back to the plain
+  start for every item.
+"""
+
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import femm_fixtures as ff
+import oracle_lib as ol
+from femm_gpu_amd import meshes
+from femm_gpu_amd.learned import Mlp, WarmStart, train_settings
+from femm_gpu_amd.model import accepted_options, plain_options
+from femm_gpu_amd.solver import Plan, mixed_config, reference_config
+
+ROOT = Path(__file__).resolve().parents[1]
+sys.path.insert(0, str(ROOT / "oracle"))
+import mlp_oracle as mo  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*args):
+    print(*args, flush=True)
+
+
+def _rel_l2(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _network(inputs, hidden, outputs, seed):
+    net = Mlp(inputs, hidden, outputs, seed=seed)
+    layout = mo.Layout(inputs, hidden, outputs)
+    assert (net.count, net.in_p, net.out_p) == (layout.count, layout.in_p, layout.out_p)
+    assert (net.off_b1, net.off_w2, net.off_w3, net.off_b3) == (layout.b1, layout.w2, layout.w3,
+                                                                layout.b3)
+    return net, layout
+
+
+def test_mlp_forward_matches_cpu_restatement():
+    say("case mlp forward 11->256->256->64, 37 rows")
+    net, layout = _network(11, 256, 64, seed=5)
+    rng = np.random.default_rng(1)
+    x = rng.normal(3.0, 2.0, (37, 11))
+    mean, std = x.mean(0), x.std(0)
+    net.set_standardization(mean, std)
+    flat = net.parameters().astype(np.float64)
+    y = net.forward(x)
+    ref = mo.forward(flat, x, mean, std, layout, quantized=True)
+    say(f"  rel L2 {_rel_l2(y, ref):.2e}")
+    assert _rel_l2(y, ref) <= 2e-3
+    assert np.array_equal(net.forward(x), y)
+
+
+def test_mlp_gradients_match_cpu_restatement():
+    say("case mlp gradients 11->256->256->64, 53 rows")
+    net, layout = _network(11, 256, 64, seed=7)
+    rng = np.random.default_rng(2)
+    flat = net.parameters().astype(np.float64)
+    W1, b1, W2, b2, W3, b3 = layout.split(flat)
+    b1[:] = rng.normal(0, 0.2, b1.shape)        # non-zero biases exercise the bias paths
+    b2[:] = rng.normal(0, 0.2, b2.shape)
+    b3[: layout.outputs] = rng.normal(0, 0.2, layout.outputs)
+    net.set_parameters(flat.astype(np.float32))
+    flat = net.parameters().astype(np.float64)
+    x = rng.normal(size=(53, 11))
+    t = rng.normal(size=(53, 64)).astype(np.float32)
+    mean, std = np.zeros(11), np.ones(11)
+    loss, grad = net.gradients(x, t)
+    ref_loss, ref_grad = mo.loss_and_gradients(flat, x, t.astype(np.float64), mean, std, layout,
+                                               quantized=True)
+    say(f"  loss {loss:.6e} vs {ref_loss:.6e}")
+    assert loss == pytest.approx(ref_loss, rel=1e-4)
+    names = ["W1", "b1", "W2", "b2", "W3", "b3"]
+    for name, g, r in zip(names, layout.split(grad.astype(np.float64)), layout.split(ref_grad)):
+        err = _rel_l2(g, r)
+        say(f"  {name}: rel L2 {err:.2e}")
+        assert err <= 2e-3, name
+    # padded weights never receive gradient
+    gW1 = layout.split(grad)[0]
+    assert not gW1[11:].any()
+    _, again = net.gradients(x, t)
+    assert np.array_equal(again, grad)
+
+
+def test_mlp_training_fits_a_smooth_map():
+    say("case mlp training 3->64->64->8")
+    net, _ = _network(3, 64, 8, seed=3)
+    rng = np.random.default_rng(6)
+    x = rng.uniform(-1, 1, (512, 3))
+    t = np.stack([np.sin(x[:, 0] + k * x[:, 1]) * np.cos(x[:, 2]) for k in range(8)], 1)
+    net.set_standardization(np.zeros(3), np.ones(3))
+    hist = net.train(x, t, steps=1500, batch=64, learning_rate=3e-3, seed=1)
+    say(f"  loss {hist[0]:.3e} -> {hist[-100:].mean():.3e}")
+    assert np.all(np.isfinite(hist))
+    assert hist[-100:].mean() <= 0.02 * hist[:10].mean()
+    y = net.forward(x[:64])
+    assert np.mean((y - t[:64]) ** 2) <= 5e-3
+
+
+def test_safeguarded_kernel_matches_oracle():
+    """Recovery path (SolveSafeguarded after an exhausted reference driver) against the
+    oracle: an exhausted recovery on the PM/coil model (cap 3) and the small motor with a
+    cap of 16, where item 0 needs the recovery and item 1 does not."""
+    say("case safeguarded pm_coil + motor_small vs oracle")
+    model, _, current, _ = ff.nonlinear_pm_coil()
+    o = plain_options(relative_tolerance=1e-8, max_newton_iterations=3,
+                      recover_nonconvergence=1)
+    ref = ol.newton_solve(model, [current], o)
+    plan = Plan(model)
+    out = plan.solve_batch(np.array([[current]]), o, reference_config())
+    r = out["results"][0]
+    say(f"  pm_coil: status {r['status']} newton {r['newton_iterations']} "
+        f"(oracle {ref['status']} / {ref['newton']}) pcg {r['linear_iterations']} "
+        f"(oracle {ref['linear']})")
+    assert r["status"] == ref["status"] == 14
+    assert r["newton_iterations"] == ref["newton"] == 6
+    assert r["linear_iterations"] == ref["linear"]
+    assert _rel_l2(out["a"][0], ref["a"]) <= 1e-9
+    plan.close()
+
+    model, _ = meshes.motor_sector(angular_cells=160)
+    currents = meshes.motor_currents(2, seed=9)
+    o = accepted_options(max_newton_iterations=16, recover_nonconvergence=1)
+    plan = Plan(model)
+    out = plan.solve_batch(currents, o, reference_config())
+    for b in range(2):
+        ref = ol.newton_solve(model, currents[b], o)
+        r = out["results"][b]
+        say(f"  motor item {b}: status {r['status']} newton {r['newton_iterations']} "
+            f"(oracle {ref['newton']}) dA {_rel_l2(out['a'][b], ref['a']):.2e}")
+        assert r["status"] == ref["status"] == 0
+        assert r["newton_iterations"] == ref["newton"]
+        assert _rel_l2(out["a"][b], ref["a"]) <= 1e-9
+        assert r["equation_residual"] <= 1e-8 and r["relative_step"] < 1e-6
+    assert out["results"][0]["newton_iterations"] > 16     # recovered
+    assert out["results"][1]["newton_iterations"] <= 16
+    plan.close()
+
+
+def _small_motor_data(plan, count, seed):
+    currents = meshes.motor_currents(count, seed=seed)
+    out = plan.solve_batch(currents, accepted_options(), mixed_config())
+    assert np.all(out["results"]["status"] == 0)
+    return currents, out
+
+
+def test_learned_warm_start_gains_and_matches_plain():
+    say("case learned warm start motor_small: 160 training solves, 16 held out")
+    model, _ = meshes.motor_sector(angular_cells=160)
+    plan = Plan(model)
+    train_cur, train = _small_motor_data(plan, 160, seed=777)
+    ws = WarmStart(plan, inputs=3, modes=32, hidden=256, seed=2)
+    rep = ws.fit(train_cur, train["a"], train_settings(steps=3000, batch=64, learning_rate=2e-3))
+    say(f"  fit: {rep}")
+    assert rep["pod_energy_fraction"] > 0.999
+    assert rep["final_loss"] < rep["first_loss"]
+    test_cur = meshes.motor_currents(16, seed=20251203)
+    plain = plan.solve_batch(test_cur, accepted_options(), reference_config())
+    a0 = ws.predict(test_cur)
+    err = np.array([_rel_l2(a0[b], plain["a"][b]) for b in range(16)])
+    say(f"  prediction rel error: mean {err.mean():.2e} max {err.max():.2e}")
+    assert err.max() < 1e-2
+    learned = ws.solve_batch(test_cur, test_cur, accepted_options(), mixed_config(),
+                             warm_iterations=12)
+    res, pres = learned["results"], plain["results"]
+    for b in range(16):
+        assert res[b]["status"] == 0
+        assert res[b]["relative_step"] < 1e-6 and res[b]["equation_residual"] <= 1e-8
+        assert _rel_l2(learned["a"][b], plain["a"][b]) <= 5e-8
+        assert np.abs(learned["flux"][b] - plain["flux"][b]).max() <= \
+            1e-7 * np.abs(plain["flux"][b]).max()
+    say(f"  newton mean: learned {res['newton_iterations'].mean():.2f} plain "
+        f"{pres['newton_iterations'].mean():.2f}; pcg mean learned "
+        f"{res['linear_iterations'].mean():.0f} plain {pres['linear_iterations'].mean():.0f}")
+    assert res["newton_iterations"].mean() <= 0.75 * pres["newton_iterations"].mean()
+    again = ws.solve_batch(test_cur, test_cur, accepted_options(), mixed_config(),
+                           warm_iterations=12)
+    assert np.array_equal(again["a"], learned["a"])
+    # A zero warm budget: every item is solved from the plain start by the fallback.
+    fb = ws.solve_batch(test_cur[:4], test_cur[:4], accepted_options(), reference_config(),
+                        warm_iterations=0)
+    for b in range(4):
+        assert fb["results"][b]["status"] == 0
+        assert fb["results"][b]["newton_iterations"] == pres[b]["newton_iterations"]
+        assert np.array_equal(fb["a"][b], plain["a"][b])
+    ws.close()
+    plan.close()
+
+
+def test_warm_safeguarded_kernel_matches_oracle_from_the_same_start():
+    say("case warm safeguarded motor_small vs oracle")
+    model, _ = meshes.motor_sector(angular_cells=160)
+    plan = Plan(model)
+    train_cur, train = _small_motor_data(plan, 64, seed=31)
+    ws = WarmStart(plan, inputs=3, modes=16, hidden=64, seed=4)
+    ws.fit(train_cur, train["a"], train_settings(steps=800, batch=32))
+    cur = meshes.motor_currents(2, seed=41)
+    a0 = ws.predict(cur)
+    out = ws.solve_batch(cur, cur, accepted_options(), reference_config(), warm_iterations=32)
+    for b in range(2):
+        ref = ol.safeguarded_solve(model, cur[b], accepted_options(), warm_start=a0[b])
+        r = out["results"][b]
+        say(f"  item {b}: newton {r['newton_iterations']} (oracle {ref['newton']}) "
+            f"dA {_rel_l2(out['a'][b], ref['a']):.2e}")
+        assert r["status"] == ref["status"] == 0
+        assert r["newton_iterations"] == ref["newton"]
+        assert _rel_l2(out["a"][b], ref["a"]) <= 1e-9
+    ws.close()
+    plan.close()
+
+
+def test_variants_match_per_item_oracle_models():
+    """Per-item materials (coercivities, B-H scale, fill factors, rotor position) on one
+    mesh: each item equals the oracle's solve of its own model (A 1e-9 rel L2, flux 1e-7,
+    same Newton count); an index outside the set fails only that item."""
+    say("case variants: study family 160 cells, 4 samples")
+    fam = meshes.study_family(angular_cells=160)
+    x = meshes.study_samples(4, seed=3)
+    models = [meshes.study_model(fam, s) for s in x]
+    plan = Plan(models[0])
+    plan.set_variants(models)
+    opts = accepted_options()
+    index = np.array([2, 0, 3, 1, 7], np.int32)
+    cur = np.vstack([x[i, :3] for i in (2, 0, 3, 1, 1)])
+    out = plan.solve_batch_variants(index, cur, opts, reference_config())
+    for b in range(4):
+        ref = ol.newton_solve(models[index[b]], cur[b], opts)
+        r = out["results"][b]
+        say(f"  item {b} (variant {index[b]}, rotor {int(x[index[b], 10])}): status {r['status']} "
+            f"newton {r['newton_iterations']} (oracle {ref['newton']}) "
+            f"dA {_rel_l2(out['a'][b], ref['a']):.2e}")
+        assert r["status"] == ref["status"] == 0
+        assert r["newton_iterations"] == ref["newton"]
+        assert _rel_l2(out["a"][b], ref["a"]) <= 1e-9
+        assert np.abs(out["flux"][b] - ref["flux"]).max() <= 1e-7 * np.abs(ref["flux"]).max()
+    assert out["results"][4]["status"] == 2
+    # a variant set built from the plan's own model reproduces the plain solve bitwise
+    plan.set_variants([models[0]])
+    v = plan.solve_batch_variants(np.zeros(1, np.int32), x[:1, :3], opts, reference_config())
+    p = plan.solve_batch(x[:1, :3], opts, reference_config())
+    assert np.array_equal(v["a"], p["a"])
+    plan.close()
