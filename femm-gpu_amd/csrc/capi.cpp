@@ -72,6 +72,8 @@ femm_solve_config_t DefaultConfig()
   c.verify_equation_residual = 1;
   c.max_refinements = 64;
   c.inner_relative_tolerance = 0.1;
+  c.partitions = 1;
+  c.reserved = 0;
   return c;
 }
 
@@ -349,6 +351,86 @@ int femm_newton_batch_variants_device(femm_plan *plan, int32_t items,
     return FEMM_INVALID_ARGUMENT;
   return SolveDevice(plan, items, variant_of_item, currents, options, config, warm_starts, a,
                      flux, results, stream);
+}
+
+int femm_plan_autotune(femm_plan *plan, int32_t items, const femm_solve_config_t *config,
+                       int32_t *partitions)
+{
+  if (plan == nullptr || items <= 0 || partitions == nullptr)
+    return FEMM_INVALID_ARGUMENT;
+  femm_solve_config_t c = config ? *config : DefaultConfig();
+  if (c.precision != FEMM_PRECISION_FP64 && c.precision != FEMM_PRECISION_MIXED)
+    return FEMM_INVALID_ARGUMENT;
+  int st = EnsurePlanDevice(plan);
+  if (st == FEMM_OK)
+    st = femm::EnsureWork(plan->host, items, c.precision == FEMM_PRECISION_MIXED, &plan->dev);
+  if (st != FEMM_OK)
+    return st;
+  const femm::HostPlan &h = plan->host;
+  int device = 0, cus = 0;
+  if (hipGetDevice(&device) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+    return FEMM_GPU_UNAVAILABLE;
+  // Probe: two Newton iterations of 200 PCG iterations each (a tolerance no solve reaches)
+  // at unit circuit currents, timed per candidate partition count.
+  femm_newton_options_t o = DefaultOptions();
+  o.relative_tolerance = 1e-300;
+  o.linear_relative_tolerance = 1e-300;
+  o.max_newton_iterations = 2;
+  o.max_linear_iterations = 200;
+  o.adaptive_linear_tolerance = 0;
+  c.verify_equation_residual = 0;
+  const size_t B = static_cast<size_t>(items);
+  const size_t sa = Round(sizeof(double) * B * h.N), sc = Round(sizeof(double) * B * std::max(h.C, 1));
+  const size_t sr = Round(sizeof(femm_item_result_t) * B);
+  if ((st = EnsureIo(plan, sa + 2 * sc + sr)) != FEMM_OK)
+    return st;
+  char *io = static_cast<char *>(plan->io);
+  double *d_a = reinterpret_cast<double *>(io);
+  double *d_cur = reinterpret_cast<double *>(io + sa);
+  double *d_flux = reinterpret_cast<double *>(io + sa + sc);
+  auto *d_res = reinterpret_cast<femm_item_result_t *>(io + sa + 2 * sc);
+  std::vector<double> ones(B * std::max(h.C, 1), 1.0);
+  if (hipMemcpy(d_cur, ones.data(), sizeof(double) * B * h.C, hipMemcpyHostToDevice) != hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  int best = 1;
+  float best_ms = -1.0f;
+  for (int p = 1; p <= femm::kMaxPartitions; p *= 2) {
+    if (B * p > static_cast<size_t>(cus) || (p > 1 && p > h.nslices))
+      break;
+    if (p == 1 && B * 4 < static_cast<size_t>(cus))
+      continue;   // a lone workgroup per item cannot fill the chip
+    c.partitions = p;
+    float ms = 0.0f;
+    for (int rep = 0; rep < 2 && st == FEMM_OK; ++rep) {
+      (void)hipEventRecord(e0, nullptr);
+      st = femm::LaunchNewton(plan->dev, items, d_cur, o, c, nullptr, d_a, d_flux, d_res, nullptr);
+      (void)hipEventRecord(e1, nullptr);
+      if (st == FEMM_OK && (hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        st = FEMM_INTERNAL_ERROR;
+    }
+    if (st != FEMM_OK)
+      break;
+    if (best_ms < 0.0f || ms < best_ms) {
+      best_ms = ms;
+      best = p;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st != FEMM_OK)
+    return st;
+  auto &tuned = plan->dev.tuned;
+  tuned.erase(std::remove_if(tuned.begin(), tuned.end(),
+                             [items](const std::pair<int32_t, int32_t> &t) { return t.first == items; }),
+              tuned.end());
+  tuned.emplace_back(items, best);
+  *partitions = best;
+  return FEMM_OK;
 }
 
 int femm_plan_set_variants(femm_plan *plan, int32_t count, const femm_model_t *variants)

@@ -104,6 +104,152 @@ struct BlockReducer {
   }
 };
 
+// Index ranges one workgroup covers for one item: free rows (slice-aligned), SELL
+// slices, CSR slots of those rows, nodes and triangles.
+struct Span {
+  int row0, row1, slice0, slice1, slot0, slot1, node0, node1, tri0, tri1;
+};
+
+// The whole item (one workgroup per item).
+__device__ inline Span WholeItem(const DevicePlan &P)
+{
+  return Span{0, P.n, 0, P.nslices, 0, P.nnz, 0, P.N, 0, P.T};
+}
+
+// The workgroups that solve one item together, as seen by one of them: its ranges,
+// team-wide reductions (identical in every lane and every workgroup) and a team-wide
+// barrier.  A single workgroup owns the whole item here.
+struct BlockTeam {
+  BlockReducer R;
+  Span S;
+  template <int K>
+  __device__ void Sum(double (&v)[K])
+  {
+    R.Sum(v);
+  }
+  __device__ double Max(double v) { return R.Max(v); }
+  __device__ int AnyTrue(bool local) { return R.AnyTrue(local); }
+  __device__ void Sync() { __syncthreads(); }
+};
+
+
+// Several workgroups solving one item together (a partitioned solve).  Workgroup g of G
+// owns a slice-aligned share of the free rows (with their CSR slots) and equal shares of
+// the nodes and triangles.  A reduction turns each workgroup's block sum into a partial in
+// global memory; after a team barrier every workgroup adds the G partials in workgroup
+// order, so the result is identical in every workgroup and repeatable for a fixed G.  The
+// barrier is sense-reversing on two words per item (arrivals, generation) with agent-scope
+// release / acquire, so each workgroup's global writes are visible to the others after
+// it.  All workgroups of a launch are co-resident (cooperative launch); a wait that still
+// exceeds kMaxBarrierPolls (seconds) marks the team aborted, and every later barrier of
+// the team returns at once, so no workgroup can wait forever.
+constexpr long kMaxBarrierPolls = 1L << 27;
+
+// Thread 0's part of the team barrier; one out-of-line copy keeps the spin loop from
+// being expanded at every barrier of the solver.
+__device__ __noinline__ void TeamArrive(uint32_t *bar, int32_t *abort, int G)
+{
+  __threadfence();   // the workgroup's writes (ordered before by the block barrier)
+  if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+    return;
+  const uint32_t gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t arrived =
+      __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  if (arrived == static_cast<uint32_t>(G - 1)) {
+    __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    // Relaxed polls (no cache invalidation per poll); the fence below acquires once.
+    for (long polls = 0;
+         __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen; ++polls) {
+      if (polls > kMaxBarrierPolls ||
+          __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  __threadfence();   // acquire: the other workgroups' writes, for the loads after the barrier
+}
+
+struct GridTeam {
+  BlockReducer R;
+  Span S;
+  double *partials;   // [2][G][4]
+  uint32_t *bar;      // [2]: arrivals, generation
+  int32_t *abort;
+  int G, g, buf;
+
+  __device__ void Sync()
+  {
+    __syncthreads();
+    if (threadIdx.x == 0)
+      TeamArrive(bar, abort, G);
+    __syncthreads();
+  }
+
+  template <int K>
+  __device__ void Sum(double (&v)[K])
+  {
+    R.Sum(v);
+    double *mine = partials + (static_cast<size_t>(buf) * G + g) * 4;
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        mine[k] = v[k];
+    }
+    Sync();
+    const double *all = partials + static_cast<size_t>(buf) * G * 4;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double s = 0.0;
+      for (int h = 0; h < G; ++h)
+        s += all[h * 4 + k];
+      v[k] = Uniform(s);
+    }
+    buf ^= 1;
+  }
+
+  __device__ double Max(double v)
+  {
+    v = R.Max(v);
+    double *mine = partials + (static_cast<size_t>(buf) * G + g) * 4;
+    if (threadIdx.x == 0)
+      mine[0] = v;
+    Sync();
+    const double *all = partials + static_cast<size_t>(buf) * G * 4;
+    double m = all[0];
+    for (int h = 1; h < G; ++h)
+      m = fmax(m, all[h * 4]);
+    buf ^= 1;
+    return Uniform(m);
+  }
+
+  __device__ int AnyTrue(bool local) { return Max(local ? 1.0 : 0.0) > 0.0; }
+};
+
+__device__ inline int UniformInt(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Workgroup g's share of an item split over G workgroups (wave-uniform values).
+__device__ inline Span Partition(const DevicePlan &P, int G, int g)
+{
+  Span s;
+  s.slice0 = static_cast<int>(static_cast<long>(P.nslices) * g / G);
+  s.slice1 = static_cast<int>(static_cast<long>(P.nslices) * (g + 1) / G);
+  s.row0 = min(s.slice0 * kSliceRows, P.n);
+  s.row1 = min(s.slice1 * kSliceRows, P.n);
+  s.slot0 = P.row_offsets[s.row0];
+  s.slot1 = P.row_offsets[s.row1];
+  s.node0 = static_cast<int>(static_cast<long>(P.N) * g / G);
+  s.node1 = static_cast<int>(static_cast<long>(P.N) * (g + 1) / G);
+  s.tri0 = static_cast<int>(static_cast<long>(P.T) * g / G);
+  s.tri1 = static_cast<int>(static_cast<long>(P.T) * (g + 1) / G);
+  return Span{UniformInt(s.row0),  UniformInt(s.row1),  UniformInt(s.slice0), UniformInt(s.slice1),
+              UniformInt(s.slot0), UniformInt(s.slot1), UniformInt(s.node0),  UniformInt(s.node1),
+              UniformInt(s.tri0),  UniformInt(s.tri1)};
+}
+
 // ----------------------------------------------------------------------------- B-H curve
 
 // Hermite B-H evaluation of the reference (physics/nonlinear_model.cpp:115-148,
@@ -152,10 +298,11 @@ __device__ bool EvalBh(const DevicePlan &P, int off, int cnt, double field, doub
 // dnu/dB^2, then the 3x3 Newton block K + C and the per-node rhs terms
 // C a_old - (K + C) A_boundary (physics/nonlinear_assembly.cu:56-98 and the
 // term-by-term formulas of :100-186).  Returns true if every element is valid.
-__device__ bool ElementPass(const DevicePlan &P, const double *a, double *elem, bool newton)
+__device__ bool ElementPass(const DevicePlan &P, const Span &S, const double *a, double *elem,
+                            bool newton)
 {
   bool ok = true;
-  for (int t = threadIdx.x; t < P.T; t += kBlockThreads) {
+  for (int t = S.tri0 + threadIdx.x; t < S.tri1; t += kBlockThreads) {
     const int n0 = P.tri_nodes[3 * t], n1 = P.tri_nodes[3 * t + 1], n2 = P.tri_nodes[3 * t + 2];
     const double av[3] = {a[n0], a[n1], a[n2]};
     double bv[3] = {0.0, 0.0, 0.0};
@@ -226,9 +373,10 @@ __device__ bool ElementPass(const DevicePlan &P, const double *a, double *elem, 
 // Slot-owned ordered gather into SELL positions (physics/nonlinear_assembly.cu:100-140):
 // contributor order is the host element/i/j emission order; one lane owns a slot.
 template <typename Real32>
-__device__ void GatherValues(const DevicePlan &P, const double *elem, double *vals, Real32 *vals32)
+__device__ void GatherValues(const DevicePlan &P, const Span &S, const double *elem, double *vals,
+                             Real32 *vals32)
 {
-  for (int slot = threadIdx.x; slot < P.nnz; slot += kBlockThreads) {
+  for (int slot = S.slot0 + threadIdx.x; slot < S.slot1; slot += kBlockThreads) {
     double sum = 0.0;
     const int end = P.slot_off[slot + 1];
     for (int k = P.slot_off[slot]; k < end; ++k) {
@@ -246,11 +394,12 @@ __device__ void GatherValues(const DevicePlan &P, const double *elem, double *va
 
 // Row-owned rhs (physics/nonlinear_assembly.cu:142-186): PM + circuit sources, then the
 // ordered element terms.  Also the Jacobi inverse (:188-206).
-__device__ bool GatherRhs(const DevicePlan &P, const double *elem, const double *vals,
-                          const double *cur, double *rhs, double *inv, float *inv32)
+__device__ bool GatherRhs(const DevicePlan &P, const Span &S, const double *elem,
+                          const double *vals, const double *cur, double *rhs, double *inv,
+                          float *inv32)
 {
   bool ok = true;
-  for (int row = threadIdx.x; row < P.n; row += kBlockThreads) {
+  for (int row = S.row0 + threadIdx.x; row < S.row1; row += kBlockThreads) {
     double s = P.pm_rhs[row];
     for (int c = 0; c < P.C; ++c)
       s += cur[c] * P.circuit_rhs[static_cast<size_t>(c) * P.n + row];
@@ -312,13 +461,13 @@ __device__ inline void LoadSlice(const DevicePlan &P, const T *__restrict__ vals
 // kPipeline = false: the plain two-round-trip loop, for rarely used products (FP64
 // residuals inside the mixed form, equation residuals) where registers matter more.
 template <typename T, typename V, bool kPipeline = true>
-__device__ double Spmv(const DevicePlan &P, const T *__restrict__ vals, const V *__restrict__ v,
-                       V *__restrict__ y, const V *__restrict__ w)
+__device__ double Spmv(const DevicePlan &P, const Span &S, const T *__restrict__ vals,
+                       const V *__restrict__ v, V *__restrict__ y, const V *__restrict__ w)
 {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   double local = 0.0;
   if constexpr (!kPipeline) {
-    for (int s = wave; s < P.nslices; s += kWaves) {
+    for (int s = S.slice0 + wave; s < S.slice1; s += kWaves) {
       const int base = P.slice_ptr[s] + lane, width = P.slice_width[s];
       T sum = 0;
       if (width <= kUnrolledWidth) {
@@ -356,9 +505,9 @@ __device__ double Spmv(const DevicePlan &P, const T *__restrict__ vals, const V 
   }
   SliceLoads<T> cur;
   cur.width = 0;
-  if (wave < P.nslices)
-    LoadSlice(P, vals, wave, lane, cur);
-  for (int s = wave; s < P.nslices; s += kWaves) {
+  if (S.slice0 + wave < S.slice1)
+    LoadSlice(P, vals, S.slice0 + wave, lane, cur);
+  for (int s = S.slice0 + wave; s < S.slice1; s += kWaves) {
     T sum = 0;
     const int next = s + kWaves;
     if (cur.width <= kUnrolledWidth) {
@@ -368,7 +517,7 @@ __device__ double Spmv(const DevicePlan &P, const T *__restrict__ vals, const V 
         g[k] = v[cur.c[k]];
       SliceLoads<T> nxt;
       // The last slice prefetches itself again (in bounds, never used).
-      LoadSlice(P, vals, next < P.nslices ? next : s, lane, nxt);
+      LoadSlice(P, vals, next < S.slice1 ? next : s, lane, nxt);
 #pragma unroll
       for (int k = 0; k < kUnrolledWidth; ++k)
         sum += cur.a[k] * static_cast<T>(g[k]);
@@ -380,7 +529,7 @@ __device__ double Spmv(const DevicePlan &P, const T *__restrict__ vals, const V 
         sum += vals[pos] * static_cast<T>(v[P.sell_cols[pos]]);
       }
       cur.width = 0;
-      if (next < P.nslices)
+      if (next < S.slice1)
         LoadSlice(P, vals, next, lane, cur);
     }
     const int row = s * kSliceRows + lane;
@@ -410,10 +559,10 @@ struct LinearInfo {
 // the recomputed true residual with restart otherwise.  With warm != 0 the
 // iteration starts from the x already in `x` (unscaled) instead of zero; the
 // acceptance test ||b - A x|| <= tol ||b|| is unchanged.
-template <bool kPipeline = true>
+template <bool kPipeline = true, class Team = BlockTeam>
 __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const double *inv,
                             const double *rhs, double *x, double *r, double *p, double *ap,
-                            double tol, int maxit, bool warm, BlockReducer &R)
+                            double tol, int maxit, bool warm, Team &R)
 {
   LinearInfo info{FEMM_INTERNAL_ERROR, 0, INFINITY};
   const int n = P.n;
@@ -423,7 +572,7 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
   }
   // First-invalid-row precedence of the reference's validation scan.
   double bad = 0.0, bmax = 0.0;
-  for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+  for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
     const double d = vals[P.diag_sell[i]];
     if (!isfinite(rhs[i]) || !isfinite(d) || !(d > 0.0))
       bad = fmax(bad, static_cast<double>(n - i));
@@ -439,22 +588,22 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
   }
   const double scale = R.Max(bmax);
   if (scale == 0.0) {
-    for (int i = threadIdx.x; i < n; i += kBlockThreads)
+    for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
       x[i] = 0.0;
-    __syncthreads();
+    R.Sync();
     info.status = FEMM_OK;
     info.residual = 0.0;
     return info;
   }
   if (warm) {
-    for (int i = threadIdx.x; i < n; i += kBlockThreads)
+    for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
       x[i] = x[i] / scale;
-    __syncthreads();
-    Spmv<double, double, false>(P, vals, x, ap, static_cast<const double *>(nullptr));
-    __syncthreads();
+    R.Sync();
+    Spmv<double, double, false>(P, R.S, vals, x, ap, static_cast<const double *>(nullptr));
+    R.Sync();
   }
   double s0[3] = {0.0, 0.0, 0.0};
-  for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+  for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
     const double b = rhs[i] / scale;
     const double ri = warm ? b - ap[i] : b;
     if (!warm)
@@ -476,15 +625,15 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
   const double threshold = tol * sqrt(s0[2]);
   if (warm && info.residual <= threshold) {
     // The start already satisfies the acceptance test.
-    for (int i = threadIdx.x; i < n; i += kBlockThreads)
+    for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
       x[i] *= scale;
-    __syncthreads();
+    R.Sync();
     info.status = FEMM_OK;
     info.residual *= scale;
     return info;
   }
   for (int it = 0; it < maxit; ++it) {
-    double pap[1] = {Spmv<double, double, kPipeline>(P, vals, p, ap, p)};
+    double pap[1] = {Spmv<double, double, kPipeline>(P, R.S, vals, p, ap, p)};
     R.Sum(pap);
     if (!isfinite(pap[0]) || !isfinite(rho)) {
       info.status = FEMM_NUMERICAL_NONFINITE;
@@ -496,12 +645,12 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
     }
     const double alpha = rho / pap[0];
     double s1[2] = {0.0, 0.0};
-    for (int i0 = threadIdx.x; i0 < n; i0 += kBlockThreads * kVecUnroll) {
+    for (int i0 = R.S.row0 + threadIdx.x; i0 < R.S.row1; i0 += kBlockThreads * kVecUnroll) {
       double xv[kVecUnroll], pv[kVecUnroll], rv[kVecUnroll], av[kVecUnroll], iv[kVecUnroll];
 #pragma unroll
       for (int u = 0; u < kVecUnroll; ++u) {
         const int i = i0 + u * kBlockThreads;
-        if (i < n) {
+        if (i < R.S.row1) {
           xv[u] = x[i];
           pv[u] = p[i];
           rv[u] = r[i];
@@ -512,7 +661,7 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
 #pragma unroll
       for (int u = 0; u < kVecUnroll; ++u) {
         const int i = i0 + u * kBlockThreads;
-        if (i < n) {
+        if (i < R.S.row1) {
           x[i] = xv[u] + alpha * pv[u];
           const double ri = rv[u] - alpha * av[u];
           r[i] = ri;
@@ -530,10 +679,10 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
     info.residual = sqrt(s1[0]);
     if (info.residual <= threshold) {
       // True residual on the recomputed b/scale - A x (pcg_kernels.cu:192-270).
-      Spmv<double, double, false>(P, vals, x, ap, static_cast<const double *>(nullptr));
-      __syncthreads();
+      Spmv<double, double, false>(P, R.S, vals, x, ap, static_cast<const double *>(nullptr));
+      R.Sync();
       double tr[1] = {0.0};
-      for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+      for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
         const double ri = rhs[i] / scale - ap[i];
         r[i] = ri;
         tr[0] += ri * ri;
@@ -546,7 +695,7 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
       info.residual = sqrt(tr[0]);
       if (info.residual <= threshold) {
         bool finite = true;
-        for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+        for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
           x[i] *= scale;
           finite = finite && isfinite(x[i]);
         }
@@ -556,7 +705,7 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
         return info;
       }
       double rs[1] = {0.0};
-      for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+      for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
         const double zi = r[i] * inv[i];
         p[i] = zi;
         rs[0] += r[i] * zi;
@@ -574,12 +723,12 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
       return info;
     }
     const double beta = s1[1] / rho;
-    for (int i0 = threadIdx.x; i0 < n; i0 += kBlockThreads * kVecUnroll) {
+    for (int i0 = R.S.row0 + threadIdx.x; i0 < R.S.row1; i0 += kBlockThreads * kVecUnroll) {
       double rv[kVecUnroll], iv[kVecUnroll], pv[kVecUnroll];
 #pragma unroll
       for (int u = 0; u < kVecUnroll; ++u) {
         const int i = i0 + u * kBlockThreads;
-        if (i < n) {
+        if (i < R.S.row1) {
           rv[u] = r[i];
           iv[u] = inv[i];
           pv[u] = p[i];
@@ -588,25 +737,25 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
 #pragma unroll
       for (int u = 0; u < kVecUnroll; ++u) {
         const int i = i0 + u * kBlockThreads;
-        if (i < n)
+        if (i < R.S.row1)
           p[i] = rv[u] * iv[u] + beta * pv[u];
       }
     }
     rho = s1[1];
-    __syncthreads();
+    R.Sync();
   }
   // Iteration cap: report the true residual (pcg_kernels.cu:302-343).
-  Spmv<double, double, false>(P, vals, x, ap, static_cast<const double *>(nullptr));
-  __syncthreads();
+  Spmv<double, double, false>(P, R.S, vals, x, ap, static_cast<const double *>(nullptr));
+  R.Sync();
   double fr[1] = {0.0};
-  for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+  for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
     const double ri = rhs[i] / scale - ap[i];
     fr[0] += ri * ri;
   }
   R.Sum(fr);
-  for (int i = threadIdx.x; i < n; i += kBlockThreads)
+  for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
     x[i] *= scale;
-  __syncthreads();
+  R.Sync();
   info.residual = sqrt(fr[0]) * scale;
   info.status = isfinite(info.residual) ? FEMM_LINEAR_SOLVE_NOT_CONVERGED : FEMM_NUMERICAL_NONFINITE;
   return info;
@@ -620,15 +769,16 @@ struct RefinedInfo {
 };
 
 // r = b - A64 x in FP64; returns ||r||_2 and max|r| through the reducer.
+template <class Team>
 __device__ void TrueResidual(const DevicePlan &P, const double *vals, const double *rhs,
                              const double *x, double *r, double *ap, double *norm, double *amax,
-                             BlockReducer &R)
+                             Team &R)
 {
-  Spmv<double, double, false>(P, vals, x, ap, static_cast<const double *>(nullptr));
-  __syncthreads();
+  Spmv<double, double, false>(P, R.S, vals, x, ap, static_cast<const double *>(nullptr));
+  R.Sync();
   double rr[1] = {0.0};
   double rmax = 0.0;
-  for (int i = threadIdx.x; i < P.n; i += kBlockThreads) {
+  for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
     const double ri = rhs[i] - ap[i];
     r[i] = ri;
     rr[0] += ri * ri;
@@ -651,11 +801,12 @@ __device__ void TrueResidual(const DevicePlan &P, const double *vals, const doub
 // ||b - A x|| <= tol ||b||.  A breakdown in FP32, a replacement that fails to halve the
 // true residual, or running out of replacements finishes the solve in FP64 PCG
 // from the current x.
+template <class Team>
 __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, const double *inv,
                                      const float *vals32, const float *inv32, const double *rhs,
                                      double *x, double *r, double *p, double *ap, float *d32,
                                      float *r32, float *p32, float *ap32, double tol, int maxit,
-                                     bool warm, int max_updates, double delta, BlockReducer &R)
+                                     bool warm, int max_updates, double delta, Team &R)
 {
   RefinedInfo out{{FEMM_INTERNAL_ERROR, 0, INFINITY}, 0};
   const int n = P.n;
@@ -665,7 +816,7 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
   }
   double b2[1] = {0.0};
   bool finite = true;
-  for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+  for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
     b2[0] += rhs[i] * rhs[i];
     finite = finite && isfinite(rhs[i]);
     if (!warm)
@@ -678,9 +829,9 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
   }
   const double bnorm = sqrt(b2[0]);
   if (bnorm == 0.0) {
-    for (int i = threadIdx.x; i < n; i += kBlockThreads)
+    for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
       x[i] = 0.0;
-    __syncthreads();
+    R.Sync();
     out.linear = {FEMM_OK, 0, 0.0};
     return out;
   }
@@ -703,7 +854,7 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
   double rho, rr;
   {
     double s0[2] = {0.0, 0.0};
-    for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+    for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
       const float ri = static_cast<float>(r[i] / sigma);
       const float zi = ri * inv32[i];
       r32[i] = ri;
@@ -721,7 +872,7 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
   bool fallback = false;
   int it = 0;
   while (it < maxit) {
-    double pap[1] = {Spmv(P, vals32, p32, ap32, p32)};
+    double pap[1] = {Spmv(P, R.S, vals32, p32, ap32, p32)};
     R.Sum(pap);
     if (!(pap[0] > 0.0) || !(rho > 0.0) || !isfinite(pap[0]) || !isfinite(rho)) {
       fallback = true;
@@ -729,12 +880,12 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
     }
     const float alpha = static_cast<float>(rho / pap[0]);
     double s1[2] = {0.0, 0.0};
-    for (int i0 = threadIdx.x; i0 < n; i0 += kBlockThreads * kVecUnroll) {
+    for (int i0 = R.S.row0 + threadIdx.x; i0 < R.S.row1; i0 += kBlockThreads * kVecUnroll) {
       float dv[kVecUnroll], pv[kVecUnroll], rv[kVecUnroll], av[kVecUnroll], iv[kVecUnroll];
 #pragma unroll
       for (int u = 0; u < kVecUnroll; ++u) {
         const int i = i0 + u * kBlockThreads;
-        if (i < n) {
+        if (i < R.S.row1) {
           dv[u] = d32[i];
           pv[u] = p32[i];
           rv[u] = r32[i];
@@ -745,7 +896,7 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
 #pragma unroll
       for (int u = 0; u < kVecUnroll; ++u) {
         const int i = i0 + u * kBlockThreads;
-        if (i < n) {
+        if (i < R.S.row1) {
           d32[i] = dv[u] + alpha * pv[u];
           const float ri = rv[u] - alpha * av[u];
           r32[i] = ri;
@@ -764,11 +915,11 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
     const bool claims = s1[0] * sigma * sigma <= threshold * threshold;
     if (claims || s1[0] <= delta * delta * rr_ref) {
       // Reliable update: fold the correction into x and replace the residual.
-      for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+      for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
         x[i] += sigma * static_cast<double>(d32[i]);
         d32[i] = 0.0f;
       }
-      __syncthreads();
+      R.Sync();
       TrueResidual(P, vals, rhs, x, r, ap, &rnorm, &rmax, R);
       ++out.refinements;
       out.linear.residual = rnorm;
@@ -791,7 +942,7 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
       const double q = sigma / rmax;
       sigma = rmax;
       double s2[2] = {0.0, 0.0};
-      for (int i = threadIdx.x; i < n; i += kBlockThreads) {
+      for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
         const float ri = static_cast<float>(r[i] / sigma);
         r32[i] = ri;
         p32[i] = static_cast<float>(q * static_cast<double>(p32[i]));
@@ -808,12 +959,12 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
       break;
     }
     const float beta = static_cast<float>(rho_next / rho);
-    for (int i0 = threadIdx.x; i0 < n; i0 += kBlockThreads * kVecUnroll) {
+    for (int i0 = R.S.row0 + threadIdx.x; i0 < R.S.row1; i0 += kBlockThreads * kVecUnroll) {
       float rv[kVecUnroll], iv[kVecUnroll], pv[kVecUnroll];
 #pragma unroll
       for (int u = 0; u < kVecUnroll; ++u) {
         const int i = i0 + u * kBlockThreads;
-        if (i < n) {
+        if (i < R.S.row1) {
           rv[u] = r32[i];
           iv[u] = inv32[i];
           pv[u] = p32[i];
@@ -822,17 +973,17 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
 #pragma unroll
       for (int u = 0; u < kVecUnroll; ++u) {
         const int i = i0 + u * kBlockThreads;
-        if (i < n)
+        if (i < R.S.row1)
           p32[i] = rv[u] * iv[u] + beta * pv[u];
       }
     }
     rho = rho_next;
-    __syncthreads();
+    R.Sync();
   }
   // Fold any pending correction, then finish in FP64 from x (or report the cap).
-  for (int i = threadIdx.x; i < n; i += kBlockThreads)
+  for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
     x[i] += sigma * static_cast<double>(d32[i]);
-  __syncthreads();
+  R.Sync();
   out.linear.iterations = it;
   const LinearInfo li = Pcg64<false>(P, vals, inv, rhs, x, r, p, ap, tol,
                                      fallback ? max(0, maxit - it) : 0, true, R);
@@ -875,30 +1026,32 @@ __device__ ItemPointers ItemWork(const DevicePlan &P, const DeviceWork &W, int i
 }
 
 // Full device assembly for one item; returns false if any element or diagonal is invalid.
+template <class Team>
 __device__ bool AssembleItem(const DevicePlan &P, const ItemPointers &w, const double *a,
-                             const double *cur, bool newton, BlockReducer &R)
+                             const double *cur, bool newton, Team &R)
 {
-  const bool e_ok = ElementPass(P, a, w.elem, newton);
-  __syncthreads();
-  GatherValues(P, w.elem, w.vals, w.vals32);
-  __syncthreads();
-  const bool r_ok = GatherRhs(P, w.elem, w.vals, cur, w.rhs, w.inv, w.inv32);
+  const bool e_ok = ElementPass(P, R.S, a, w.elem, newton);
+  R.Sync();
+  GatherValues(P, R.S, w.elem, w.vals, w.vals32);
+  __syncthreads();   // GatherRhs reads the diagonal of this workgroup's own rows
+  const bool r_ok = GatherRhs(P, R.S, w.elem, w.vals, cur, w.rhs, w.inv, w.inv32);
   return !R.AnyTrue(!(e_ok && r_ok));
 }
 
 // ||K(A) A - b(A)|| over the free rows, with A taken at each free root
 // (solvers/nonlinear_solver.cu:549-558).  Requires a fresh assembly at `a`.
 // `xf` receives A at the free roots (n values).
+template <class Team>
 __device__ double EquationNorm(const DevicePlan &P, const ItemPointers &w, const double *a,
-                               BlockReducer &R, double *xf)
+                               Team &R, double *xf)
 {
-  for (int i = threadIdx.x; i < P.n; i += kBlockThreads)
+  for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
     xf[i] = a[P.free_nodes[i]];
-  __syncthreads();
-  Spmv<double, double, false>(P, w.vals, xf, w.ap, static_cast<const double *>(nullptr));
-  __syncthreads();
+  R.Sync();
+  Spmv<double, double, false>(P, R.S, w.vals, xf, w.ap, static_cast<const double *>(nullptr));
+  R.Sync();
   double s[1] = {0.0};
-  for (int i = threadIdx.x; i < P.n; i += kBlockThreads) {
+  for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
     const double v = w.ap[i] - w.rhs[i];
     s[0] += v * v;
   }
@@ -960,11 +1113,13 @@ __device__ double LinearTolerance(const femm_newton_options_t &o, bool adaptive,
   return fmax(exact, 1e-8 * fmin(1.0, prev));
 }
 
-__device__ void WriteFlux(const DevicePlan &P, const double *a, double *flux, BlockReducer &R)
+template <class Team>
+__device__ void WriteFlux(const DevicePlan &P, const double *a, double *flux, Team &R,
+                          bool lead = true)
 {
   for (int c = 0; c < P.C; ++c) {
     double s[1] = {0.0};
-    for (int t = threadIdx.x; t < P.T; t += kBlockThreads) {
+    for (int t = R.S.tri0 + threadIdx.x; t < R.S.tri1; t += kBlockThreads) {
       if (P.tri_circuit[t] != c)
         continue;
       const double *fw = P.tri_flux + 3 * static_cast<size_t>(t);
@@ -972,7 +1127,7 @@ __device__ void WriteFlux(const DevicePlan &P, const double *a, double *flux, Bl
               fw[2] * a[P.tri_nodes[3 * t + 2]];
     }
     R.Sum(s);
-    if (threadIdx.x == 0)
+    if (lead && threadIdx.x == 0)
       flux[c] = P.planar ? s[0] * P.depth : s[0];
   }
 }
@@ -996,10 +1151,10 @@ __device__ bool OptionsValid(const femm_newton_options_t &o)
 }
 
 // One linear solve of the current assembly into w.x (x0 = w.x when warm).
-template <bool kMixed>
+template <bool kMixed, class Team>
 __device__ LinearInfo LinearSolve(const DevicePlan &P, const ItemPointers &w,
                                   const femm_solve_config_t &cfg, double tol, int maxit,
-                                  femm_item_result_t &res, BlockReducer &R)
+                                  femm_item_result_t &res, Team &R)
 {
   LinearInfo li;
   if constexpr (kMixed) {
@@ -1013,7 +1168,7 @@ __device__ LinearInfo LinearSolve(const DevicePlan &P, const ItemPointers &w,
     li = Pcg64(P, w.vals, w.inv, w.rhs, w.x, w.r, w.p, w.ap, tol, maxit, cfg.pcg_warm_start != 0,
                R);
   }
-  __syncthreads();
+  R.Sync();
   res.linear_iterations += li.iterations;
   res.last_linear_tolerance = tol;
   res.linear_residual = li.residual;
@@ -1042,12 +1197,12 @@ __device__ LinearInfo LinearSolve(const DevicePlan &P, const ItemPointers &w,
 // start (nonlinear_solver.cu:565-576), independent of any warm start.  The two policies
 // share one body so that a kernel running both (the learned mode) holds one copy of the
 // assembly and the linear solve.
-template <bool kMixed>
+template <bool kMixed, class Team>
 __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers &w,
                                         const double *cur, double *a, bool inputs_ok,
                                         const femm_newton_options_t &o,
                                         const femm_solve_config_t &cfg, const double *warm,
-                                        bool safeguarded, BlockReducer &R)
+                                        bool safeguarded, Team &R)
 {
   femm_item_result_t res = FreshResult();
   inputs_ok = inputs_ok && OptionsValid(o);
@@ -1058,14 +1213,14 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
   // Plain start: prescribed values on every fixed node (signed roots expanded), zero
   // elsewhere; the residual scale is taken here, then the warm start replaces the free
   // nodes.
-  for (int v = threadIdx.x; v < P.N; v += kBlockThreads)
+  for (int v = R.S.node0 + threadIdx.x; v < R.S.node1; v += kBlockThreads)
     a[v] = P.node_to_free[v] < 0 ? P.boundary[v] : 0.0;
-  __syncthreads();
+  R.Sync();
   int status = inputs_ok ? FEMM_NONLINEAR_SOLVE_NOT_CONVERGED : FEMM_INVALID_ARGUMENT;
   double scale = 1.0, norm = INFINITY;
   if (inputs_ok && (safeguarded || cfg.verify_equation_residual || gate_eq)) {
     if (AssembleItem(P, w, a, cur, true, R)) {
-      __syncthreads();
+      R.Sync();
       norm = EquationNorm(P, w, a, R, w.x);
       scale = fmax(norm, DBL_MIN);
     } else if (safeguarded) {
@@ -1073,13 +1228,13 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
     }
   }
   if (has_warm) {
-    for (int v = threadIdx.x; v < P.N; v += kBlockThreads)
+    for (int v = R.S.node0 + threadIdx.x; v < R.S.node1; v += kBlockThreads)
       if (P.node_to_free[v] >= 0)
         a[v] = warm[v];
-    __syncthreads();
+    R.Sync();
     if (safeguarded && status == FEMM_NONLINEAR_SOLVE_NOT_CONVERGED) {
       if (AssembleItem(P, w, a, cur, true, R)) {
-        __syncthreads();
+        R.Sync();
         norm = EquationNorm(P, w, a, R, w.x);
       } else {
         status = FEMM_ASSEMBLY_FAILED;
@@ -1106,7 +1261,7 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
         status = FEMM_ASSEMBLY_FAILED;
         break;
       }
-      __syncthreads();
+      R.Sync();
       if (pending) {
         // The Newton assembly at the returned state also yields its equation
         // residual (nonlinear_solver.cu:549-558); accept only if it passes too.
@@ -1123,9 +1278,9 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
     }
     const double tol = safeguarded ? o.linear_relative_tolerance : LinearTolerance(o, adaptive, prev);
     if (cfg.pcg_warm_start) {
-      for (int i = threadIdx.x; i < P.n; i += kBlockThreads)
+      for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
         w.x[i] = a[P.free_nodes[i]];
-      __syncthreads();
+      R.Sync();
     }
     const LinearInfo li = LinearSolve<kMixed>(P, w, cfg, tol, o.max_linear_iterations, res, R);
     if (li.status != FEMM_OK) {
@@ -1134,7 +1289,7 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
     }
     // Undamped step ||A_cand - A|| / ||A_cand|| over all nodes (nonlinear_solver.cu:326-335).
     double s[2] = {0.0, 0.0};
-    for (int v = threadIdx.x; v < P.N; v += kBlockThreads) {
+    for (int v = R.S.node0 + threadIdx.x; v < R.S.node1; v += kBlockThreads) {
       const int f = P.node_to_free[v];
       const double cand = f >= 0 ? static_cast<double>(P.node_sign[v]) * w.x[f] : P.boundary[v];
       const double d = cand - a[v];
@@ -1157,24 +1312,24 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
       bool accepted = false;
       double alpha = 1.0;
       for (int bt = 0; bt < 20; ++bt, alpha *= 0.5) {
-        for (int v = threadIdx.x; v < P.N; v += kBlockThreads) {
+        for (int v = R.S.node0 + threadIdx.x; v < R.S.node1; v += kBlockThreads) {
           const int f = P.node_to_free[v];
           const double cand =
               f >= 0 ? static_cast<double>(P.node_sign[v]) * w.x[f] : P.boundary[v];
           w.trial[v] = a[v] + alpha * (cand - a[v]);
         }
-        __syncthreads();
+        R.Sync();
         if (!AssembleItem(P, w, w.trial, cur, true, R)) {
           status = FEMM_ASSEMBLY_FAILED;
           break;
         }
-        __syncthreads();
+        R.Sync();
         const double tn = EquationNorm(P, w, w.trial, R, w.p);
         if (isfinite(tn) &&
             (tn <= (1.0 - 1e-4 * alpha) * norm || tn / scale <= o.relative_tolerance)) {
-          for (int v = threadIdx.x; v < P.N; v += kBlockThreads)
+          for (int v = R.S.node0 + threadIdx.x; v < R.S.node1; v += kBlockThreads)
             a[v] = w.trial[v];
-          __syncthreads();
+          R.Sync();
           norm = tn;
           accepted = true;
           break;
@@ -1191,12 +1346,12 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
       else
         omega += 0.1 * (1.0 - omega);
     }
-    for (int v = threadIdx.x; v < P.N; v += kBlockThreads) {
+    for (int v = R.S.node0 + threadIdx.x; v < R.S.node1; v += kBlockThreads) {
       const int f = P.node_to_free[v];
       const double cand = f >= 0 ? static_cast<double>(P.node_sign[v]) * w.x[f] : P.boundary[v];
       a[v] += omega * (cand - a[v]);
     }
-    __syncthreads();
+    R.Sync();
     eq_current = false;
     // NewtonStepConverged (nonlinear_model.cpp:186-193).
     if (it > 0 && step < 100.0 * o.relative_tolerance && tol <= o.linear_relative_tolerance) {
@@ -1214,7 +1369,7 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
   } else if (inputs_ok && cfg.verify_equation_residual && !eq_current &&
              (status == FEMM_OK || status == FEMM_NONLINEAR_SOLVE_NOT_CONVERGED)) {
     if (AssembleItem(P, w, a, cur, true, R)) {
-      __syncthreads();
+      R.Sync();
       res.equation_residual = EquationNorm(P, w, a, R, w.x) / scale;
     } else {
       res.equation_residual = INFINITY;
@@ -1231,26 +1386,19 @@ enum SolveMode : int32_t {
                          // plain start in the same workgroup if that attempt is not accepted
 };
 
-// One workgroup per batch item runs the whole nonlinear solve of `kMode`.  With prior
-// results, an item the earlier launch kept (KeepPrior) is left untouched and the others
-// add their counters to the earlier ones.
-template <bool kMixed, int kMode>
-__global__ __launch_bounds__(kBlockThreads) void SolveKernel(
-    DevicePlan base, DeviceWork W, const double *currents, NewtonArgs args, const double *warm,
-    double *A, double *flux_out, femm_item_result_t *results, const femm_item_result_t *prior,
-    DeviceVariants VS, const int32_t *variant_of_item)
+// The nonlinear solve of `kMode` for one item by `R`'s team, results of workgroup 0
+// written out.  With prior results, an item the earlier launch kept (KeepPrior) is left
+// untouched and the others add their counters to the earlier ones.
+template <bool kMixed, int kMode, class Team>
+__device__ void SolveItem(const DevicePlan &P, const DeviceWork &W, int item, bool lead,
+                          bool inputs_ok, const double *currents, const NewtonArgs &args,
+                          const double *warm, double *A, double *flux_out,
+                          femm_item_result_t *results, const femm_item_result_t *prior,
+                          Team &R)
 {
-  __shared__ double lds[2 * kWaves * 4];
-  BlockReducer R{lds, 0};
-  const int item = blockIdx.x;
   femm_item_result_t earlier{};
-  if (prior != nullptr) {
+  if (prior != nullptr)
     earlier = prior[item];
-    if (KeepPrior(earlier, args.prior_mode))
-      return;
-  }
-  DevicePlan P = base;
-  bool inputs_ok = ApplyVariant(P, VS, variant_of_item, item);
   const ItemPointers w = ItemWork(P, W, item, kMixed);
   double *a = A + static_cast<size_t>(item) * P.N;
   const double *cur = currents + static_cast<size_t>(item) * P.C;
@@ -1276,14 +1424,63 @@ __global__ __launch_bounds__(kBlockThreads) void SolveKernel(
       break;
     spent = res;
   }
-  WriteFlux(P, a, flux_out + static_cast<size_t>(item) * P.C, R);
+  WriteFlux(P, a, flux_out + static_cast<size_t>(item) * P.C, R, lead);
   if (prior != nullptr) {
     res.newton_iterations += earlier.newton_iterations;
     res.linear_iterations += earlier.linear_iterations;
     res.refinements += earlier.refinements;
   }
-  if (threadIdx.x == 0)
+  if (lead && threadIdx.x == 0)
     results[item] = res;
+}
+
+// One workgroup per batch item.
+template <bool kMixed, int kMode>
+__global__ __launch_bounds__(kBlockThreads) void SolveKernel(
+    DevicePlan base, DeviceWork W, const double *currents, NewtonArgs args, const double *warm,
+    double *A, double *flux_out, femm_item_result_t *results, const femm_item_result_t *prior,
+    DeviceVariants VS, const int32_t *variant_of_item)
+{
+  __shared__ double lds[2 * kWaves * 4];
+  const int item = blockIdx.x;
+  if (prior != nullptr && KeepPrior(prior[item], args.prior_mode))
+    return;
+  DevicePlan P = base;
+  const bool inputs_ok = ApplyVariant(P, VS, variant_of_item, item);
+  BlockTeam R{BlockReducer{lds, 0}, WholeItem(P)};
+  SolveItem<kMixed, kMode>(P, W, item, true, inputs_ok, currents, args, warm, A, flux_out, results,
+                           prior, R);
+}
+
+// `partitions` workgroups per batch item (grid = items x partitions, cooperative launch).
+// An aborted team (a barrier wait that ran out) reports INTERNAL_ERROR for its item.
+template <bool kMixed, int kMode>
+__global__ __launch_bounds__(kBlockThreads) void TeamSolveKernel(
+    DevicePlan base, DeviceWork W, const double *currents, NewtonArgs args, const double *warm,
+    double *A, double *flux_out, femm_item_result_t *results, const femm_item_result_t *prior,
+    DeviceVariants VS, const int32_t *variant_of_item, int partitions)
+{
+  __shared__ double lds[2 * kWaves * 4];
+  const int item = UniformInt(static_cast<int>(blockIdx.x) / partitions);
+  const int g = UniformInt(static_cast<int>(blockIdx.x) % partitions);
+  if (prior != nullptr && KeepPrior(prior[item], args.prior_mode))
+    return;
+  DevicePlan P = base;
+  const bool inputs_ok = ApplyVariant(P, VS, variant_of_item, item);
+  GridTeam R{BlockReducer{lds, 0},
+             Partition(P, partitions, g),
+             W.team_partials + static_cast<size_t>(item) * 2 * kMaxPartitions * 4,
+             W.team_barrier + 2 * static_cast<size_t>(item),
+             W.team_abort + item,
+             partitions,
+             g,
+             0};
+  SolveItem<kMixed, kMode>(P, W, item, g == 0, inputs_ok, currents, args, warm, A, flux_out,
+                           results, prior, R);
+  R.Sync();
+  if (g == 0 && threadIdx.x == 0 &&
+      __hip_atomic_load(R.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+    results[item].status = FEMM_INTERNAL_ERROR;
 }
 
 __global__ __launch_bounds__(kBlockThreads) void AssembleKernel(DevicePlan P, DeviceWork W,
@@ -1292,7 +1489,7 @@ __global__ __launch_bounds__(kBlockThreads) void AssembleKernel(DevicePlan P, De
                                                                 int newton, int32_t *valid)
 {
   __shared__ double lds[2 * kWaves * 4];
-  BlockReducer R{lds, 0};
+  BlockTeam R{BlockReducer{lds, 0}, WholeItem(P)};
   const int item = blockIdx.x;
   const ItemPointers w = ItemWork(P, W, item, false);
   const bool ok = AssembleItem(P, w, states + static_cast<size_t>(item) * P.N,
@@ -1305,14 +1502,14 @@ __global__ __launch_bounds__(kBlockThreads) void PcgKernel(DevicePlan P, DeviceW
                                                            int maxit, double *info)
 {
   __shared__ double lds[2 * kWaves * 4];
-  BlockReducer R{lds, 0};
+  BlockTeam R{BlockReducer{lds, 0}, WholeItem(P)};
   const int item = blockIdx.x;
   const ItemPointers w = ItemWork(P, W, item, false);
-  for (int row = threadIdx.x; row < P.n; row += kBlockThreads) {
+  for (int row = R.S.row0 + threadIdx.x; row < R.S.row1; row += kBlockThreads) {
     const double d = w.vals[P.diag_sell[row]];
     w.inv[row] = (isfinite(d) && d > 0.0) ? 1.0 / d : 0.0;
   }
-  __syncthreads();
+  R.Sync();
   const LinearInfo li = Pcg64(P, w.vals, w.inv, w.rhs, w.x, w.r, w.p, w.ap, tol, maxit, false, R);
   if (threadIdx.x == 0) {
     info[3 * item] = li.status;
@@ -1346,7 +1543,7 @@ __global__ __launch_bounds__(kBlockThreads) void FluxKernel(DevicePlan P, const 
                                                             double *flux)
 {
   __shared__ double lds[2 * kWaves * 4];
-  BlockReducer R{lds, 0};
+  BlockTeam R{BlockReducer{lds, 0}, WholeItem(P)};
   const int item = blockIdx.x;
   WriteFlux(P, states + static_cast<size_t>(item) * P.N, flux + static_cast<size_t>(item) * P.C, R);
 }
@@ -1596,6 +1793,7 @@ int UploadPlan(const HostPlan &h, DeviceBuffers *dev)
     P.sell_cols = up(h.sell_cols);
     P.slot_to_sell = up(h.slot_to_sell);
     P.diag_sell = up(h.diag_sell);
+    P.row_offsets = up(h.row_offsets);
     if (pass == 0) {
       void *mem = nullptr;
       if (hipMalloc(&mem, offset) != hipSuccess)
@@ -1762,6 +1960,8 @@ int EnsureWork(const HostPlan &h, int32_t items, bool mixed, DeviceBuffers *dev)
                  6 * Bytes<double>(ci * h.n) + Bytes<double>(ci * h.N);
   if (want_mixed)
     bytes += Bytes<float>(ci * h.sell_size) + 5 * Bytes<float>(ci * h.n);
+  bytes += Bytes<double>(ci * 2 * kMaxPartitions * 4) + Bytes<uint32_t>(ci * 2) +
+           Bytes<int32_t>(ci);
   void *base = nullptr;
   if (hipMalloc(&base, bytes) != hipSuccess)
     return FEMM_GPU_ALLOCATION_FAILED;
@@ -1791,6 +1991,9 @@ int EnsureWork(const HostPlan &h, int32_t items, bool mixed, DeviceBuffers *dev)
     w.d32 = Carve<float>(cur, ci * h.n);
     w.inv32 = Carve<float>(cur, ci * h.n);
   }
+  w.team_partials = Carve<double>(cur, ci * 2 * kMaxPartitions * 4);
+  w.team_barrier = Carve<uint32_t>(cur, ci * 2);
+  w.team_abort = Carve<int32_t>(cur, ci);
   dev->work_base = base;
   dev->work_bytes = bytes;
   return hipDeviceSynchronize() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
@@ -1813,14 +2016,116 @@ void ReleaseDevice(DeviceBuffers *dev)
 namespace
 {
 
+// Largest grid a cooperative launch of `kernel` may use on the current device.
+int CoResidentBlocks(const void *kernel)
+{
+  int device = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&device) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlockThreads, 0) !=
+          hipSuccess)
+    return 0;
+  return cus * per_cu;
+}
+
+}  // namespace
+
+int AutoPartitions(const DeviceBuffers &dev, int32_t items)
+{
+  for (const auto &t : dev.tuned)
+    if (t.first == items)
+      return t.second;
+  int device = 0, cus = 0;
+  if (items <= 0 || hipGetDevice(&device) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+    return 1;
+  // Untuned rule: one workgroup per item once the batch fills half the CUs; otherwise
+  // split the CUs over the items with about 32 slices (2,048 rows) per workgroup, where
+  // team barriers and per-CU bandwidth balanced on the 58k-node motor.
+  if (items * 2 >= cus)
+    return 1;
+  const int by_cus = cus / items;
+  const int by_rows = std::max(1, dev.plan.nslices / 32);
+  return std::max(1, std::min({by_cus, by_rows, kMaxPartitions}));
+}
+
+namespace
+{
+
+template <bool kMixed, int kMode>
+int LaunchTeams(const DeviceBuffers &dev, int32_t items, int32_t partitions,
+                const double *currents, const NewtonArgs &args, const double *warm, double *a,
+                double *flux, femm_item_result_t *results, hipStream_t stream,
+                const LaunchExtras &x)
+{
+  const void *kernel = reinterpret_cast<const void *>(&TeamSolveKernel<kMixed, kMode>);
+  const long grid = static_cast<long>(items) * partitions;
+  if (partitions > kMaxPartitions || grid > CoResidentBlocks(kernel))
+    return FEMM_INVALID_ARGUMENT;
+  // Fresh barrier words and abort flags for the launched items.
+  if (hipMemsetAsync(dev.work.team_barrier, 0, sizeof(uint32_t) * 2 * items, stream) !=
+          hipSuccess ||
+      hipMemsetAsync(dev.work.team_abort, 0, sizeof(int32_t) * items, stream) != hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  DevicePlan plan = dev.plan;
+  DeviceWork work = dev.work;
+  NewtonArgs launch_args = args;
+  DeviceVariants variants = dev.variants;
+  const femm_item_result_t *prior = x.prior;
+  const int32_t *variant_of_item = x.variant_of_item;
+  int parts = partitions;
+  void *params[] = {&plan, &work, &currents, &launch_args, &warm, &a, &flux, &results,
+                    &prior, &variants, &variant_of_item, &parts};
+  return hipLaunchCooperativeKernel(kernel, dim3(static_cast<unsigned>(grid)), dim3(kBlockThreads),
+                                    params, 0, stream) == hipSuccess
+             ? FEMM_OK
+             : FEMM_INTERNAL_ERROR;
+}
+
 template <int kMode>
 int LaunchSolve(const DeviceBuffers &dev, int32_t items, const double *currents,
-                const NewtonArgs &args, const double *warm, double *a, double *flux,
+                const NewtonArgs &args_in, const double *warm, double *a, double *flux,
                 femm_item_result_t *results, hipStream_t stream, const LaunchExtras &x)
 {
   if (items <= 0 || items > dev.work.capacity)
     return FEMM_INVALID_ARGUMENT;
-  if (args.config.precision == FEMM_PRECISION_MIXED)
+  NewtonArgs args = args_in;
+  if (args.config.partitions == 0)
+    args.config.partitions = AutoPartitions(dev, items);
+  const int32_t partitions = args.config.partitions;
+  const bool mixed = args.config.precision == FEMM_PRECISION_MIXED;
+  if (partitions > 1) {
+    if constexpr (kMode == kModeLearned) {
+      // Partitioned: the warm attempt and the plain-start fallback as two launches.
+      NewtonArgs warm_args = args;
+      warm_args.options = args.warm_options;
+      int st = mixed ? LaunchTeams<true, kModeSafeguarded>(dev, items, partitions, currents,
+                                                         warm_args, warm, a, flux, results,
+                                                         stream, x)
+                     : LaunchTeams<false, kModeSafeguarded>(dev, items, partitions, currents,
+                                                          warm_args, warm, a, flux, results,
+                                                          stream, x);
+      if (st != FEMM_OK)
+        return st;
+      LaunchExtras redo = x;
+      redo.prior = results;
+      NewtonArgs plain_args = args;
+      plain_args.prior_mode = kPriorRedoFailed;
+      return mixed ? LaunchTeams<true, kModeNewton>(dev, items, partitions, currents, plain_args,
+                                                   nullptr, a, flux, results, stream, redo)
+                   : LaunchTeams<false, kModeNewton>(dev, items, partitions, currents,
+                                                    plain_args, nullptr, a, flux, results, stream,
+                                                    redo);
+    } else {
+      return mixed ? LaunchTeams<true, kMode>(dev, items, partitions, currents, args, warm, a,
+                                             flux, results, stream, x)
+                   : LaunchTeams<false, kMode>(dev, items, partitions, currents, args, warm, a,
+                                              flux, results, stream, x);
+    }
+  }
+  if (partitions < 0)
+    return FEMM_INVALID_ARGUMENT;
+  if (mixed)
     hipLaunchKernelGGL((SolveKernel<true, kMode>), dim3(items), dim3(kBlockThreads), 0, stream,
                        dev.plan, dev.work, currents, args, warm, a, flux, results, x.prior,
                        dev.variants, x.variant_of_item);

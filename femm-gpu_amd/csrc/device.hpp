@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <utility>
 #include <vector>
 
 #include "femm_gpu_amd.h"
@@ -31,6 +32,7 @@ struct DevicePlan {
   const int32_t *slot_off, *contrib, *row_off, *rcontrib;
   const double *pm_rhs, *circuit_rhs;
   const int32_t *slice_ptr, *slice_width, *sell_cols, *slot_to_sell, *diag_sell;
+  const int32_t *row_offsets;   // CSR [n + 1]: slot range of a row partition
 };
 
 // Per-item device workspaces, item-major.  Strides in elements.
@@ -42,7 +44,14 @@ struct DeviceWork {
   double *rhs, *inv, *x, *r, *p, *ap;   // [items][n]
   double *trial;     // [items][N] Armijo trial state (safeguarded Newton)
   float *r32, *p32, *ap32, *d32, *inv32; // [items][n]   (mixed precision)
+  // Partitioned solves (several workgroups per item): per-item reduction partials
+  // [items][2][kMaxPartitions][4], barrier words [items][2] and abort flags [items].
+  double *team_partials;
+  uint32_t *team_barrier;
+  int32_t *team_abort;
 };
+
+constexpr int kMaxPartitions = 256;   // workgroups per item in a partitioned solve
 
 // Per-item material variants on the plan's mesh: every array is [count][stride] with the
 // plan's sizes (T triangles, `mats` materials, `knots` B-H knots, n free rows, C circuits).
@@ -65,6 +74,7 @@ struct DeviceBuffers {
   DeviceWork work{};
   void *work_base = nullptr;
   size_t work_bytes = 0;
+  std::vector<std::pair<int32_t, int32_t>> tuned;   // (items, partitions) from the autotuner
 };
 
 int UploadPlan(const HostPlan &host, DeviceBuffers *dev);
@@ -83,6 +93,9 @@ enum PriorMode : int32_t {
   kPriorRedoFailed = 1,    // items whose prior status is not FEMM_OK
   kPriorRedoUnconverged = 2  // items whose prior status is NONLINEAR_SOLVE_NOT_CONVERGED
 };
+
+// Workgroups per item chosen for `items` items when the config asks for it (partitions = 0).
+int AutoPartitions(const DeviceBuffers &dev, int32_t items);
 
 // Optional per-launch inputs: earlier results (see PriorMode) and the material variant
 // of each item (null = the plan's own materials).

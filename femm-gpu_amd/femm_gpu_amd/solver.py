@@ -31,7 +31,8 @@ MIXED = 1
 class SolveConfig(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("pcg_warm_start", ctypes.c_int32),
                 ("verify_equation_residual", ctypes.c_int32), ("max_refinements", ctypes.c_int32),
-                ("inner_relative_tolerance", ctypes.c_double)]
+                ("inner_relative_tolerance", ctypes.c_double), ("partitions", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class ItemResult(ctypes.Structure):
@@ -58,7 +59,7 @@ RESULT_DTYPE = np.dtype([("status", np.int32), ("newton_iterations", np.int32),
 def reference_config(**kw) -> SolveConfig:
     """The reference algorithm: FP64 Jacobi PCG from x0 = 0 in every Newton correction."""
     values = dict(precision=FP64, pcg_warm_start=0, verify_equation_residual=1,
-                  max_refinements=64, inner_relative_tolerance=1e-4)
+                  max_refinements=64, inner_relative_tolerance=0.1, partitions=1)
     values.update(kw)
     return SolveConfig(**values)
 
@@ -67,7 +68,7 @@ def mixed_config(**kw) -> SolveConfig:
     """FP32 Jacobi PCG with FP64 reliable updates (true residual replaced whenever the FP32
     residual has dropped 10x), each Newton correction starting from the current state."""
     values = dict(precision=MIXED, pcg_warm_start=1, verify_equation_residual=1,
-                  max_refinements=64, inner_relative_tolerance=0.1)
+                  max_refinements=64, inner_relative_tolerance=0.1, partitions=1)
     values.update(kw)
     return SolveConfig(**values)
 
@@ -203,6 +204,15 @@ class Plan:
             ctypes.byref(config), None if warm_ptr is None else ctypes.c_void_p(warm_ptr),
             ctypes.c_void_p(a_ptr), ctypes.c_void_p(flux_ptr), ctypes.c_void_p(results_ptr),
             ctypes.c_void_p(stream_ptr)), "femm_newton_batch_device")
+
+    def autotune(self, items, config: SolveConfig = None):
+        """Time partition counts for batches of `items` items (femm_plan_autotune); later
+        solves with config.partitions = 0 use the fastest.  Returns it."""
+        config = config or reference_config()
+        out = ctypes.c_int32(0)
+        _check(self._lib.femm_plan_autotune(self._handle, int(items), ctypes.byref(config),
+                                            ctypes.byref(out)), "femm_plan_autotune")
+        return int(out.value)
 
     def set_variants(self, models):
         """Per-item material variants on this plan's mesh (femm_plan_set_variants)."""

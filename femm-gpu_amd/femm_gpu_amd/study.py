@@ -31,13 +31,14 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=4096)
     ap.add_argument("--test", type=int, default=1024)
-    ap.add_argument("--chunk", type=int, default=256)
+    ap.add_argument("--chunk", type=int, default=256, help="held-out items per launch")
+    ap.add_argument("--train-chunk", type=int, default=512, help="training items per launch")
     ap.add_argument("--angular-cells", type=int, default=480)
     ap.add_argument("--train-seed", type=int, default=5001)
     ap.add_argument("--test-seed", type=int, default=9001)
     ap.add_argument("--modes", type=int, default=64)
     ap.add_argument("--steps", type=int, default=6000)
-    ap.add_argument("--warm-budget", type=int, default=12)
+    ap.add_argument("--warm-budget", type=int, default=16)
     ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)
 
@@ -59,15 +60,15 @@ def main(argv=None):
     _log(f"# study mesh N={N} T={plan.T} n={plan.n} nnz={plan.nnz}; train {args.train}, "
          f"test {args.test}, chunk {args.chunk}")
 
-    def run_chunks(x, solve, label):
+    def run_chunks(x, solve, label, chunk):
         """solve(index_dev, cur_dev, count, a_dev, flux_dev, res_dev) per chunk of variants."""
         total = len(x)
         a_all = torch.zeros((total, N), dtype=torch.float64, device="cuda")
         flux_all = np.zeros((total, C))
         res_all = np.zeros(total, RESULT_DTYPE)
         gpu_ms = 0.0
-        for lo in range(0, total, args.chunk):
-            hi = min(total, lo + args.chunk)
+        for lo in range(0, total, chunk):
+            hi = min(total, lo + chunk)
             t0 = time.perf_counter()
             plan.set_variants([meshes.study_model(fam, s) for s in x[lo:hi]])
             setup = time.perf_counter() - t0
@@ -96,7 +97,7 @@ def main(argv=None):
     a_train, _, res_train, train_ms = run_chunks(
         x_train, lambda idx, cur, B, a, flux, res, lo: plan.solve_batch_variants_device(
             idx.data_ptr(), cur.data_ptr(), B, a.data_ptr(), flux.data_ptr(), res.data_ptr(),
-            stream.cuda_stream, opts, mixed_config()), "train")
+            stream.cuda_stream, opts, mixed_config()), "train", args.train_chunk)
     ok = _passing(res_train)
     feats = torch.from_numpy(np.ascontiguousarray(x_train[ok])).to("cuda")
     states = a_train[torch.from_numpy(np.flatnonzero(ok)).to("cuda")].contiguous()
@@ -114,13 +115,13 @@ def main(argv=None):
     a_plain, flux_plain, res_plain, plain_ms = run_chunks(
         x_test, lambda idx, cur, B, a, flux, res, lo: plan.solve_batch_variants_device(
             idx.data_ptr(), cur.data_ptr(), B, a.data_ptr(), flux.data_ptr(), res.data_ptr(),
-            stream.cuda_stream, opts, reference_config()), "plain")
+            stream.cuda_stream, opts, reference_config()), "plain", args.chunk)
     feats_test = torch.from_numpy(np.ascontiguousarray(x_test)).to("cuda")
     a_learn, flux_learn, res_learn, learn_ms = run_chunks(
         x_test, lambda idx, cur, B, a, flux, res, lo: ws.solve_batch_device(
             B, feats_test[lo:lo + B].data_ptr(), cur.data_ptr(), a.data_ptr(), flux.data_ptr(),
             res.data_ptr(), stream.cuda_stream, opts, mixed_config(), args.warm_budget,
-            variant_ptr=idx.data_ptr()), "learned")
+            variant_ptr=idx.data_ptr()), "learned", args.chunk)
     da = ((a_learn - a_plain).norm(dim=1) / a_plain.norm(dim=1).clamp_min(1e-300)).cpu().numpy()
     dflux = np.abs(flux_learn - flux_plain).max(1) / np.maximum(np.abs(flux_plain).max(1), 1e-300)
 

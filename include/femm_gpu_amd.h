@@ -35,6 +35,11 @@ typedef struct femm_solve_config {
   int32_t max_refinements;     /* cap on FP64 residual replacements per solve (mixed mode) */
   double inner_relative_tolerance; /* mixed mode: replace the residual once the FP32 one has
                                       dropped by this factor since the last replacement */
+  int32_t partitions;          /* workgroups per item: 1 = one workgroup owns an item;
+                                  > 1 = the item's rows, nodes and triangles are split over
+                                  that many workgroups (small batches, large meshes);
+                                  0 = chosen by the library (femm_plan_autotune) */
+  int32_t reserved;
 } femm_solve_config_t;
 
 /* Per-item outcome (reference SolveInfo plus the Newton counters of
@@ -99,6 +104,14 @@ int femm_newton_batch_device(femm_plan *plan, int32_t items, const double *curre
                              const femm_newton_options_t *options,
                              const femm_solve_config_t *config, const double *warm_starts,
                              double *a, double *flux, femm_item_result_t *results, void *stream);
+
+/* Autotuner: times a short probe solve (two Newton iterations of 200 PCG iterations each)
+ * of `items` items for partition counts 1, 2, 4, ... that fit the device, and records the
+ * fastest in the plan; later solves of `items` items with config->partitions = 0 use it.
+ * The reference picks its cooperative shard count by a fixed rule
+ * (PcgCooperativeShardCount, solvers/options.cpp:43-53). */
+int femm_plan_autotune(femm_plan *plan, int32_t items, const femm_solve_config_t *config,
+                       int32_t *partitions);
 
 /* Per-item material variants on the plan's mesh.  Each variant is a full model with the
  * plan's nodes, triangles, Dirichlet nodes and values, node constraints, problem type,

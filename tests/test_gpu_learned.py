@@ -253,3 +253,28 @@ def test_variants_match_per_item_oracle_models():
     p = plan.solve_batch(x[:1, :3], opts, reference_config())
     assert np.array_equal(v["a"], p["a"])
     plan.close()
+
+
+def test_learned_partitioned_matches_one_workgroup_path():
+    """The learned solve with 8 workgroups per item (two launches: warm attempt, then the
+    plain-start fallback) passes the same gates and agrees with the one-workgroup path at
+    the gates' resolution."""
+    say("case learned partitioned: motor_small 2 items x 8 partitions")
+    model, _ = meshes.motor_sector(angular_cells=160)
+    plan = Plan(model)
+    train_cur, train = _small_motor_data(plan, 64, seed=31)
+    ws = WarmStart(plan, inputs=3, modes=16, hidden=64, seed=4)
+    ws.fit(train_cur, train["a"], train_settings(steps=800, batch=32))
+    cur = meshes.motor_currents(2, seed=41)
+    one = ws.solve_batch(cur, cur, accepted_options(), mixed_config(), warm_iterations=16)
+    part = ws.solve_batch(cur, cur, accepted_options(), mixed_config(partitions=8),
+                          warm_iterations=16)
+    for b in range(2):
+        r = part["results"][b]
+        say(f"  item {b}: newton {r['newton_iterations']} (one-workgroup "
+            f"{one['results'][b]['newton_iterations']}) dA {_rel_l2(part['a'][b], one['a'][b]):.2e}")
+        assert r["status"] == 0
+        assert r["relative_step"] < 1e-6 and r["equation_residual"] <= 1e-8
+        assert _rel_l2(part["a"][b], one["a"][b]) <= 5e-8
+    ws.close()
+    plan.close()

@@ -271,3 +271,51 @@ def test_mixed_precision_form_matches_oracle():
     again = plan.solve_batch(currents, opts, mixed_config())
     assert np.array_equal(again["a"], out["a"])
     plan.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_partitioned_solve_matches_oracle(precision):
+    """Several workgroups per item (rows, nodes and triangles split, team reductions in
+    workgroup order): same acceptance as the one-workgroup path, A within 1e-9 relative L2
+    of the FP64 oracle for the reference form (5e-8 for the mixed form), repeatable bitwise
+    for a fixed partition count; the partition count does not change the iteration counts of
+    the reference form."""
+    say(f"case partitioned solve ({precision}): pm_coil x 3 partitions, motor_small 2 x 16")
+    cfg = reference_config if precision == "fp64" else mixed_config
+    bar = 1e-9 if precision == "fp64" else 5e-8
+    model, a_exp, current, _ = ff.nonlinear_pm_coil()
+    plan = Plan(model)
+    o = plain_options(relative_tolerance=1e-8, max_newton_iterations=128)
+    out = plan.solve_batch(np.array([[current]]), o, cfg(partitions=3), fields=True)
+    ref = ol.newton_solve(model, [current], o)
+    r = out["results"][0]
+    say(f"  pm_coil: status {r['status']} newton {r['newton_iterations']} (oracle {ref['newton']}) "
+        f"dA {_rel_l2(out['a'][0], ref['a']):.2e}")
+    assert r["status"] == 0
+    assert _rel_l2(out["a"][0], ref["a"]) <= bar
+    if precision == "fp64":
+        assert np.abs(out["a"][0] - a_exp).max() <= 1e-9
+    plan.close()
+    model, _ = meshes.motor_sector(angular_cells=160)
+    currents = meshes.motor_currents(2, seed=5)
+    plan = Plan(model)
+    opts = accepted_options()
+    out = plan.solve_batch(currents, opts, cfg(partitions=16))
+    a_ref, flux_ref, res_ref = ol.newton_batch(model, currents, opts, threads=16)
+    for b in range(2):
+        r = out["results"][b]
+        say(f"  motor item {b}: status {r['status']} newton {r['newton_iterations']} "
+            f"(oracle {res_ref[b]['newton']}) pcg {r['linear_iterations']} "
+            f"(oracle {res_ref[b]['linear']}) dA {_rel_l2(out['a'][b], a_ref[b]):.2e}")
+        assert r["status"] == 0
+        assert r["relative_step"] < 1e-6 and r["equation_residual"] <= 1e-8
+        assert _rel_l2(out["a"][b], a_ref[b]) <= bar
+        assert np.abs(out["flux"][b] - flux_ref[b]).max() <= 1e-7 * np.abs(flux_ref[b]).max()
+        if precision == "fp64":
+            assert r["newton_iterations"] == res_ref[b]["newton"]
+    again = plan.solve_batch(currents, opts, cfg(partitions=16))
+    assert np.array_equal(again["a"], out["a"])
+    auto = plan.solve_batch(currents[:1], opts, cfg(partitions=0))
+    assert auto["results"][0]["status"] == 0
+    assert _rel_l2(auto["a"][0], a_ref[0]) <= bar
+    plan.close()
