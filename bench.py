@@ -284,7 +284,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic("learned_mixed", workload),
-                     "kernel": "SafeguardedKernel<true> + NewtonKernel<true> (one learned step)",
+                     "kernel": "SolveKernel<true, 2> (learned start + mixed form, one launch)",
                      "pcg_iteration_bytes_per_item": chosen["pcg_iter_bytes"]},
         "cpu_baseline": cpu,
     }

@@ -73,7 +73,7 @@ femm_solve_config_t DefaultConfig()
   c.max_refinements = 64;
   c.inner_relative_tolerance = 0.1;
   c.partitions = 1;
-  c.reserved = 0;
+  c.block_size = 1;
   return c;
 }
 
@@ -94,6 +94,73 @@ int EnsurePlanDevice(femm_plan *plan)
   return st;
 }
 
+// Probe solve of the autotuners: two Newton iterations of 200 PCG iterations each (a
+// tolerance no solve reaches) of `items` items at unit circuit currents with `partitions`
+// workgroups per item; *ms is the second of two timed launches.
+int ProbeSolve(femm_plan *plan, int32_t items, const femm_solve_config_t &config,
+               int32_t partitions, float *ms)
+{
+  femm_solve_config_t c = config;
+  if (!femm::ConfigValid(c))
+    return FEMM_INVALID_ARGUMENT;
+  int st = EnsurePlanDevice(plan);
+  if (st == FEMM_OK)
+    st = femm::EnsureWork(plan->host, items, c.precision == FEMM_PRECISION_MIXED, &plan->dev,
+                          c.block_size > 1);
+  if (st != FEMM_OK)
+    return st;
+  const femm::HostPlan &h = plan->host;
+  femm_newton_options_t o = DefaultOptions();
+  o.relative_tolerance = 1e-300;
+  o.linear_relative_tolerance = 1e-300;
+  o.max_newton_iterations = 2;
+  o.max_linear_iterations = 200;
+  o.adaptive_linear_tolerance = 0;
+  c.verify_equation_residual = 0;
+  c.partitions = partitions;
+  const size_t B = static_cast<size_t>(items);
+  const size_t sa = Round(sizeof(double) * B * h.N), sc = Round(sizeof(double) * B * std::max(h.C, 1));
+  const size_t sr = Round(sizeof(femm_item_result_t) * B);
+  if ((st = EnsureIo(plan, sa + 2 * sc + sr)) != FEMM_OK)
+    return st;
+  char *io = static_cast<char *>(plan->io);
+  double *d_a = reinterpret_cast<double *>(io);
+  double *d_cur = reinterpret_cast<double *>(io + sa);
+  double *d_flux = reinterpret_cast<double *>(io + sa + sc);
+  auto *d_res = reinterpret_cast<femm_item_result_t *>(io + sa + 2 * sc);
+  std::vector<double> ones(B * std::max(h.C, 1), 1.0);
+  if (hipMemcpy(d_cur, ones.data(), sizeof(double) * B * h.C, hipMemcpyHostToDevice) != hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return FEMM_INTERNAL_ERROR;
+  }
+  for (int rep = 0; rep < 2 && st == FEMM_OK; ++rep) {
+    (void)hipEventRecord(e0, nullptr);
+    st = femm::LaunchNewton(plan->dev, items, d_cur, o, c, nullptr, d_a, d_flux, d_res, nullptr);
+    (void)hipEventRecord(e1, nullptr);
+    if (st == FEMM_OK && (hipEventSynchronize(e1) != hipSuccess ||
+                          hipEventElapsedTime(ms, e0, e1) != hipSuccess))
+      st = FEMM_INTERNAL_ERROR;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return st;
+}
+
+bool ConfigValid(const femm_solve_config_t &c)
+{
+  return (c.precision == FEMM_PRECISION_FP64 || c.precision == FEMM_PRECISION_MIXED) &&
+         (c.block_size == 0 || c.block_size == 1 || c.block_size == 2 || c.block_size == 4 ||
+          c.block_size == 8);
+}
+
+femm_newton_options_t DefaultNewtonOptions() { return DefaultOptions(); }
+femm_solve_config_t DefaultSolveConfig() { return DefaultConfig(); }
+bool DevicePresent() { return GpuPresent(); }
 }  // namespace femm
 
 using femm::EnsurePlanDevice;
@@ -113,11 +180,12 @@ int SolveDevice(femm_plan *plan, int32_t items, const int32_t *variant_of_item,
     return FEMM_INVALID_ARGUMENT;
   const femm_newton_options_t o = options ? *options : DefaultOptions();
   const femm_solve_config_t c = config ? *config : DefaultConfig();
-  if (c.precision != FEMM_PRECISION_FP64 && c.precision != FEMM_PRECISION_MIXED)
+  if (!femm::ConfigValid(c))
     return FEMM_INVALID_ARGUMENT;
   int st = EnsurePlanDevice(plan);
   if (st == FEMM_OK)
-    st = femm::EnsureWork(plan->host, items, c.precision == FEMM_PRECISION_MIXED, &plan->dev);
+    st = femm::EnsureWork(plan->host, items, c.precision == FEMM_PRECISION_MIXED, &plan->dev,
+                          c.block_size > 1);
   if (st != FEMM_OK)
     return st;
   const hipStream_t s = static_cast<hipStream_t>(stream);
@@ -297,12 +365,21 @@ int femm_assemble_batch(femm_plan *plan, int32_t items, const double *states,
 int femm_pcg_batch(femm_plan *plan, int32_t items, const double *values, const double *rhs,
                    double relative_tolerance, int32_t max_iterations, double *x, double *info)
 {
-  if (plan == nullptr || items <= 0 || values == nullptr || rhs == nullptr)
+  return femm_pcg_batch_block(plan, items, values, rhs, relative_tolerance, max_iterations, 1, x,
+                              info);
+}
+
+int femm_pcg_batch_block(femm_plan *plan, int32_t items, const double *values, const double *rhs,
+                         double relative_tolerance, int32_t max_iterations, int32_t block_size,
+                         double *x, double *info)
+{
+  if (plan == nullptr || items <= 0 || values == nullptr || rhs == nullptr ||
+      (block_size != 1 && block_size != 2 && block_size != 4 && block_size != 8))
     return FEMM_INVALID_ARGUMENT;
   const femm::HostPlan &h = plan->host;
   int st = EnsurePlanDevice(plan);
   if (st == FEMM_OK)
-    st = femm::EnsureWork(h, items, false, &plan->dev);
+    st = femm::EnsureWork(h, items, false, &plan->dev, block_size > 1);
   if (st != FEMM_OK)
     return st;
   std::vector<double> sell(static_cast<size_t>(items) * h.sell_size, 0.0);
@@ -320,7 +397,7 @@ int femm_pcg_batch(femm_plan *plan, int32_t items, const double *values, const d
                 hipMemcpyHostToDevice) != hipSuccess)
     return FEMM_INTERNAL_ERROR;
   if ((st = femm::LaunchPcg(plan->dev, items, relative_tolerance, max_iterations, d_info,
-                            nullptr)) != FEMM_OK)
+                            nullptr, block_size)) != FEMM_OK)
     return st;
   if (hipDeviceSynchronize() != hipSuccess ||
       (x && hipMemcpy(x, plan->dev.work.x, sizeof(double) * items * static_cast<size_t>(h.n),
@@ -358,72 +435,26 @@ int femm_plan_autotune(femm_plan *plan, int32_t items, const femm_solve_config_t
 {
   if (plan == nullptr || items <= 0 || partitions == nullptr)
     return FEMM_INVALID_ARGUMENT;
-  femm_solve_config_t c = config ? *config : DefaultConfig();
-  if (c.precision != FEMM_PRECISION_FP64 && c.precision != FEMM_PRECISION_MIXED)
-    return FEMM_INVALID_ARGUMENT;
-  int st = EnsurePlanDevice(plan);
-  if (st == FEMM_OK)
-    st = femm::EnsureWork(plan->host, items, c.precision == FEMM_PRECISION_MIXED, &plan->dev);
-  if (st != FEMM_OK)
-    return st;
-  const femm::HostPlan &h = plan->host;
+  const femm_solve_config_t c = config ? *config : DefaultConfig();
   int device = 0, cus = 0;
-  if (hipGetDevice(&device) != hipSuccess ||
+  if (!GpuPresent() || hipGetDevice(&device) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
     return FEMM_GPU_UNAVAILABLE;
-  // Probe: two Newton iterations of 200 PCG iterations each (a tolerance no solve reaches)
-  // at unit circuit currents, timed per candidate partition count.
-  femm_newton_options_t o = DefaultOptions();
-  o.relative_tolerance = 1e-300;
-  o.linear_relative_tolerance = 1e-300;
-  o.max_newton_iterations = 2;
-  o.max_linear_iterations = 200;
-  o.adaptive_linear_tolerance = 0;
-  c.verify_equation_residual = 0;
-  const size_t B = static_cast<size_t>(items);
-  const size_t sa = Round(sizeof(double) * B * h.N), sc = Round(sizeof(double) * B * std::max(h.C, 1));
-  const size_t sr = Round(sizeof(femm_item_result_t) * B);
-  if ((st = EnsureIo(plan, sa + 2 * sc + sr)) != FEMM_OK)
-    return st;
-  char *io = static_cast<char *>(plan->io);
-  double *d_a = reinterpret_cast<double *>(io);
-  double *d_cur = reinterpret_cast<double *>(io + sa);
-  double *d_flux = reinterpret_cast<double *>(io + sa + sc);
-  auto *d_res = reinterpret_cast<femm_item_result_t *>(io + sa + 2 * sc);
-  std::vector<double> ones(B * std::max(h.C, 1), 1.0);
-  if (hipMemcpy(d_cur, ones.data(), sizeof(double) * B * h.C, hipMemcpyHostToDevice) != hipSuccess)
-    return FEMM_INTERNAL_ERROR;
-  hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
-    return FEMM_INTERNAL_ERROR;
-  int best = 1;
+  int best = 1, st = FEMM_OK;
   float best_ms = -1.0f;
   for (int p = 1; p <= femm::kMaxPartitions; p *= 2) {
-    if (B * p > static_cast<size_t>(cus) || (p > 1 && p > h.nslices))
+    if (static_cast<long>(items) * p > cus || (p > 1 && p > plan->host.nslices))
       break;
-    if (p == 1 && B * 4 < static_cast<size_t>(cus))
+    if (p == 1 && items * 4 < cus)
       continue;   // a lone workgroup per item cannot fill the chip
-    c.partitions = p;
     float ms = 0.0f;
-    for (int rep = 0; rep < 2 && st == FEMM_OK; ++rep) {
-      (void)hipEventRecord(e0, nullptr);
-      st = femm::LaunchNewton(plan->dev, items, d_cur, o, c, nullptr, d_a, d_flux, d_res, nullptr);
-      (void)hipEventRecord(e1, nullptr);
-      if (st == FEMM_OK && (hipEventSynchronize(e1) != hipSuccess ||
-                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-        st = FEMM_INTERNAL_ERROR;
-    }
-    if (st != FEMM_OK)
-      break;
+    if ((st = femm::ProbeSolve(plan, items, c, p, &ms)) != FEMM_OK)
+      return st;
     if (best_ms < 0.0f || ms < best_ms) {
       best_ms = ms;
       best = p;
     }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (st != FEMM_OK)
-    return st;
   auto &tuned = plan->dev.tuned;
   tuned.erase(std::remove_if(tuned.begin(), tuned.end(),
                              [items](const std::pair<int32_t, int32_t> &t) { return t.first == items; }),

@@ -44,6 +44,8 @@ struct DeviceWork {
   double *rhs, *inv, *x, *r, *p, *ap;   // [items][n]
   double *trial;     // [items][N] Armijo trial state (safeguarded Newton)
   float *r32, *p32, *ap32, *d32, *inv32; // [items][n]   (mixed precision)
+  double *bjac;      // [items][kMaxBlock][n] block-Jacobi planes (block_size > 1 only)
+  float *bjac32;     // [items][kMaxBlock][n] (mixed precision with block-Jacobi)
   // Partitioned solves (several workgroups per item): per-item reduction partials
   // [items][2][kMaxPartitions][4], barrier words [items][2] and abort flags [items].
   double *team_partials;
@@ -52,6 +54,7 @@ struct DeviceWork {
 };
 
 constexpr int kMaxPartitions = 256;   // workgroups per item in a partitioned solve
+constexpr int kMaxBlock = 8;          // largest block-Jacobi block (rows)
 
 // Per-item material variants on the plan's mesh: every array is [count][stride] with the
 // plan's sizes (T triangles, `mats` materials, `knots` B-H knots, n free rows, C circuits).
@@ -82,8 +85,18 @@ int UploadPlan(const HostPlan &host, DeviceBuffers *dev);
 // fills variant `index` from a material part built with BuildHostPlan(..., &topology).
 int AllocateVariants(const HostPlan &topology, int32_t count, DeviceBuffers *dev);
 int CopyVariant(const HostPlan &topology, int32_t index, const HostPlan &part, DeviceBuffers *dev);
-int EnsureWork(const HostPlan &host, int32_t items, bool mixed, DeviceBuffers *dev);
+// Workspace for `items` items; `mixed` adds the FP32 arrays, `blocks` the block-Jacobi planes.
+int EnsureWork(const HostPlan &host, int32_t items, bool mixed, DeviceBuffers *dev,
+               bool blocks = false);
 void ReleaseDevice(DeviceBuffers *dev);
+
+// Options of one solve launch (kernel argument).
+struct NewtonArgs {
+  femm_newton_options_t options;
+  femm_solve_config_t config;
+  int32_t prior_mode;
+  femm_newton_options_t warm_options;   // learned mode: the warm attempt's options
+};
 
 // Which items a launch with prior results solves again (`prior` may alias `results`;
 // the others keep the earlier outputs untouched, and re-solved items add their
@@ -93,6 +106,26 @@ enum PriorMode : int32_t {
   kPriorRedoFailed = 1,    // items whose prior status is not FEMM_OK
   kPriorRedoUnconverged = 2  // items whose prior status is NONLINEAR_SOLVE_NOT_CONVERGED
 };
+
+// One plan's batch inside a group launch (meshes of uneven size in one launch).
+struct GroupEntry {
+  DevicePlan plan;
+  DeviceWork work;
+  const double *currents;
+  double *a, *flux;
+  femm_item_result_t *results;
+  int32_t items, partitions, first_block, reserved;
+};
+
+// The entries (also copied to `entries_device`, same order) in one cooperative launch of the
+// reference driver: entry e's items use blocks [first_block, first_block + items x partitions).
+// recovery: the bounded safeguarded re-solve of the items an earlier group launch left
+// unconverged (results read and rewritten in place).
+int LaunchGroup(const GroupEntry *entries_device, const std::vector<GroupEntry> &entries,
+                const femm_newton_options_t &options, const femm_solve_config_t &config,
+                bool recovery, hipStream_t stream);
+// Co-resident workgroup capacity of a group launch on the current device.
+int GroupCapacity(bool mixed);
 
 // Workgroups per item chosen for `items` items when the config asks for it (partitions = 0).
 int AutoPartitions(const DeviceBuffers &dev, int32_t items);
@@ -104,6 +137,18 @@ struct LaunchExtras {
   int32_t prior_mode = kPriorNone;
   const int32_t *variant_of_item = nullptr;
 };
+
+// Block-Jacobi solve forms (device_block.hip): `form` carries the precision bit (1) and the
+// block bit (2); `mode` is the solve mode (0 reference driver, 1 safeguarded, 2 learned).
+int LaunchBlockSolve(int form, int mode, const DeviceBuffers &dev, int32_t items,
+                     int32_t partitions, const double *currents, const NewtonArgs &args,
+                     const double *warm, double *a, double *flux, femm_item_result_t *results,
+                     hipStream_t stream, const LaunchExtras &x);
+// The group kernel of a block-Jacobi form (reference driver or safeguarded recovery).
+const void *BlockGroupKernel(int form, bool recovery);
+// PCG with block-Jacobi on work.vals / work.rhs (x0 = 0); info[items][3].
+int LaunchBlockPcg(const DeviceBuffers &dev, int32_t items, double tol, int32_t maxit,
+                   int32_t block_size, double *info, hipStream_t stream);
 
 // Batched Newton solve with the reference driver's policy (nonlinear_solver.cu:183-530)
 // on device pointers, enqueued on `stream`; `warm` null = the plain start.
@@ -130,7 +175,7 @@ int LaunchAssemble(const DeviceBuffers &dev, int32_t items, const double *states
                    const double *currents, int32_t newton, int32_t *valid, hipStream_t stream);
 // PCG only on work.vals / work.rhs (x0 = 0); info[items][3].
 int LaunchPcg(const DeviceBuffers &dev, int32_t items, double tol, int32_t maxit, double *info,
-              hipStream_t stream);
+              hipStream_t stream, int32_t block_size = 1);
 // Element B and circuit flux for states[items][N].
 int LaunchFinalize(const DeviceBuffers &dev, int32_t items, const double *states, double *bx,
                    double *by, double *flux, hipStream_t stream);

@@ -36,4 +36,14 @@ namespace femm
 {
 // Uploads the immutable plan on first use (femm_plan_create works without a GPU).
 int EnsurePlanDevice(femm_plan *plan);
+// Probe solve of the autotuners: two Newton iterations of 200 PCG iterations each (a
+// tolerance no solve reaches) of `items` items at unit circuit currents with `partitions`
+// workgroups per item; *ms is the second of two timed launches.
+int ProbeSolve(femm_plan *plan, int32_t items, const femm_solve_config_t &config,
+               int32_t partitions, float *ms);
+// Precision and preconditioner block size of a solve config are supported values.
+bool ConfigValid(const femm_solve_config_t &c);
+femm_newton_options_t DefaultNewtonOptions();
+femm_solve_config_t DefaultSolveConfig();
+bool DevicePresent();
 }  // namespace femm

@@ -674,12 +674,12 @@ int femm_newton_batch_learned_device(femm_plan *plan, femm_warm_start *ws, int32
     return FEMM_INVALID_ARGUMENT;
   if (!ws->fitted)
     return FEMM_INVALID_ARGUMENT;
-  if (config->precision != FEMM_PRECISION_FP64 && config->precision != FEMM_PRECISION_MIXED)
+  if (!femm::ConfigValid(*config))
     return FEMM_INVALID_ARGUMENT;
   int st = femm::EnsurePlanDevice(plan);
   if (st == FEMM_OK)
     st = femm::EnsureWork(plan->host, items, config->precision == FEMM_PRECISION_MIXED,
-                          &plan->dev);
+                          &plan->dev, config->block_size > 1);
   if (st != FEMM_OK)
     return st;
   if (variant_of_item != nullptr && plan->dev.variants.count == 0)

@@ -1,0 +1,1987 @@
+// Device code of the batched Newton solve shared by the solve translation units:
+// reductions, B-H evaluation, assembly, SELL SpMV, block-Jacobi, FP64 and mixed PCG,
+// the Newton drivers and the solve kernels, plus the launch templates.  Each solve form
+// (kFormMixed | kFormBlock) is instantiated in one translation unit only
+// (device.hip: point Jacobi; device_block.hip: block-Jacobi), so the compiler's
+// inlining and register allocation of one form do not depend on the others.
+//
+// Work decomposition: one 512-thread workgroup owns one batch item for the
+// whole Newton solve (or `partitions` workgroups own it together, with team
+// reductions in workgroup order).  Every reduction is a fixed-shape wave64 shuffle
+// tree followed by a fixed-order sum of the 8 wave partials, so results are
+// bitwise reproducible from run to run and independent of batch composition.
+//
+// Memory layout (HBM, item-major): per item the nodal state A[N], the element
+// Newton blocks elem[T][12], the operator values in SELL-64 order, and the PCG
+// vectors x, r, p, Ap, D^-1 [n].  The SELL column indices, contributor lists and
+// element geometry are shared by all items and stay resident in L2 / MALL.
+
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+#include "post.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <type_traits>
+#include <vector>
+
+namespace femm
+{
+namespace
+{
+
+constexpr int kWaves = kBlockThreads / 64;
+
+#define HIP_TRY(expr)                                                                              \
+  do {                                                                                             \
+    if ((expr) != hipSuccess)                                                                      \
+      return FEMM_INTERNAL_ERROR;                                                                  \
+  } while (0)
+
+// ----------------------------------------------------------------------------- reductions
+
+// The same value in every lane, made wave-uniform (held in scalar registers).
+__device__ inline double Uniform(double x)
+{
+  const uint64_t bits = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(bits));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(bits >> 32));
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+struct BlockReducer {
+  double *lds;   // [2][kWaves][4]
+  int buf;
+
+  template <int K>
+  __device__ void Sum(double (&v)[K])
+  {
+    static_assert(K <= 4, "at most four simultaneous sums");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double x = v[k];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1)
+        x += __shfl_down(x, off, 64);
+      v[k] = x;
+    }
+    double *b = lds + buf * (kWaves * 4);
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        b[wave * 4 + k] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double s = b[k];
+#pragma unroll
+      for (int w = 1; w < kWaves; ++w)
+        s += b[w * 4 + k];
+      v[k] = Uniform(s);
+    }
+    buf ^= 1;
+  }
+
+  __device__ double Max(double v)
+  {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+      v = fmax(v, __shfl_down(v, off, 64));
+    double *b = lds + buf * (kWaves * 4);
+    if (lane == 0)
+      b[wave * 4] = v;
+    __syncthreads();
+    double s = b[0];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w)
+      s = fmax(s, b[w * 4]);
+    buf ^= 1;
+    return Uniform(s);
+  }
+
+  __device__ int AnyTrue(bool local)
+  {
+    double v = local ? 1.0 : 0.0;
+    return Max(v) > 0.0;
+  }
+};
+
+// Index ranges one workgroup covers for one item: free rows (slice-aligned), SELL
+// slices, CSR slots of those rows, nodes and triangles.
+struct Span {
+  int row0, row1, slice0, slice1, slot0, slot1, node0, node1, tri0, tri1;
+};
+
+// The whole item (one workgroup per item).
+__device__ inline Span WholeItem(const DevicePlan &P)
+{
+  return Span{0, P.n, 0, P.nslices, 0, P.nnz, 0, P.N, 0, P.T};
+}
+
+// The workgroups that solve one item together, as seen by one of them: its ranges,
+// team-wide reductions (identical in every lane and every workgroup) and a team-wide
+// barrier.  A single workgroup owns the whole item here.
+struct BlockTeam {
+  BlockReducer R;
+  Span S;
+  template <int K>
+  __device__ void Sum(double (&v)[K])
+  {
+    R.Sum(v);
+  }
+  __device__ double Max(double v) { return R.Max(v); }
+  __device__ int AnyTrue(bool local) { return R.AnyTrue(local); }
+  __device__ void Sync() { __syncthreads(); }
+};
+
+
+// Several workgroups solving one item together (a partitioned solve).  Workgroup g of G
+// owns a slice-aligned share of the free rows (with their CSR slots) and equal shares of
+// the nodes and triangles.  A reduction turns each workgroup's block sum into a partial in
+// global memory; after a team barrier every workgroup adds the G partials in workgroup
+// order, so the result is identical in every workgroup and repeatable for a fixed G.  The
+// barrier is sense-reversing on two words per item (arrivals, generation) with agent-scope
+// release / acquire, so each workgroup's global writes are visible to the others after
+// it.  All workgroups of a launch are co-resident (cooperative launch); a wait that still
+// exceeds kMaxBarrierPolls (seconds) marks the team aborted, and every later barrier of
+// the team returns at once, so no workgroup can wait forever.
+constexpr long kMaxBarrierPolls = 1L << 27;
+
+// Thread 0's part of the team barrier; one out-of-line copy keeps the spin loop from
+// being expanded at every barrier of the solver.
+__device__ __noinline__ void TeamArrive(uint32_t *bar, int32_t *abort, int G)
+{
+  __threadfence();   // the workgroup's writes (ordered before by the block barrier)
+  if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+    return;
+  const uint32_t gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t arrived =
+      __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  if (arrived == static_cast<uint32_t>(G - 1)) {
+    __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    // Relaxed polls (no cache invalidation per poll); the fence below acquires once.
+    for (long polls = 0;
+         __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen; ++polls) {
+      if (polls > kMaxBarrierPolls ||
+          __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  __threadfence();   // acquire: the other workgroups' writes, for the loads after the barrier
+}
+
+struct GridTeam {
+  BlockReducer R;
+  Span S;
+  double *partials;   // [2][G][4]
+  uint32_t *bar;      // [2]: arrivals, generation
+  int32_t *abort;
+  int G, g, buf;
+
+  __device__ void Sync()
+  {
+    __syncthreads();
+    if (threadIdx.x == 0)
+      TeamArrive(bar, abort, G);
+    __syncthreads();
+  }
+
+  template <int K>
+  __device__ void Sum(double (&v)[K])
+  {
+    R.Sum(v);
+    double *mine = partials + (static_cast<size_t>(buf) * G + g) * 4;
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        mine[k] = v[k];
+    }
+    Sync();
+    const double *all = partials + static_cast<size_t>(buf) * G * 4;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double s = 0.0;
+      for (int h = 0; h < G; ++h)
+        s += all[h * 4 + k];
+      v[k] = Uniform(s);
+    }
+    buf ^= 1;
+  }
+
+  __device__ double Max(double v)
+  {
+    v = R.Max(v);
+    double *mine = partials + (static_cast<size_t>(buf) * G + g) * 4;
+    if (threadIdx.x == 0)
+      mine[0] = v;
+    Sync();
+    const double *all = partials + static_cast<size_t>(buf) * G * 4;
+    double m = all[0];
+    for (int h = 1; h < G; ++h)
+      m = fmax(m, all[h * 4]);
+    buf ^= 1;
+    return Uniform(m);
+  }
+
+  __device__ int AnyTrue(bool local) { return Max(local ? 1.0 : 0.0) > 0.0; }
+};
+
+__device__ inline int UniformInt(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Workgroup g's share of an item split over G workgroups (wave-uniform values).
+__device__ inline Span Partition(const DevicePlan &P, int G, int g)
+{
+  Span s;
+  s.slice0 = static_cast<int>(static_cast<long>(P.nslices) * g / G);
+  s.slice1 = static_cast<int>(static_cast<long>(P.nslices) * (g + 1) / G);
+  s.row0 = min(s.slice0 * kSliceRows, P.n);
+  s.row1 = min(s.slice1 * kSliceRows, P.n);
+  s.slot0 = P.row_offsets[s.row0];
+  s.slot1 = P.row_offsets[s.row1];
+  s.node0 = static_cast<int>(static_cast<long>(P.N) * g / G);
+  s.node1 = static_cast<int>(static_cast<long>(P.N) * (g + 1) / G);
+  s.tri0 = static_cast<int>(static_cast<long>(P.T) * g / G);
+  s.tri1 = static_cast<int>(static_cast<long>(P.T) * (g + 1) / G);
+  return Span{UniformInt(s.row0),  UniformInt(s.row1),  UniformInt(s.slice0), UniformInt(s.slice1),
+              UniformInt(s.slot0), UniformInt(s.slot1), UniformInt(s.node0),  UniformInt(s.node1),
+              UniformInt(s.tri0),  UniformInt(s.tri1)};
+}
+
+// ----------------------------------------------------------------------------- B-H curve
+
+// Hermite B-H evaluation of the reference (physics/nonlinear_model.cpp:115-148,
+// physics/nonlinear_assembly.cu:12-54): linear scan for the segment, linear
+// extrapolation beyond the last knot, |B| = 0 gives nu = slope[0], nu' = 0.
+__device__ bool EvalBh(const DevicePlan &P, int off, int cnt, double field, double *nu,
+                       double *dnu)
+{
+  const double b = fabs(field);
+  if (b == 0.0) {
+    *nu = P.curve_s[off];
+    *dnu = 0.0;
+    return isfinite(*nu);
+  }
+  const double *cb = P.curve_b + off, *ch = P.curve_h + off, *cs = P.curve_s + off;
+  const int last = cnt - 1;
+  double h = 0.0, dh = 0.0;
+  if (b > cb[last]) {
+    h = ch[last] + cs[last] * (b - cb[last]);
+    dh = cs[last];
+  } else {
+    bool found = false;
+    for (int i = 0; i < last; ++i) {
+      if (b >= cb[i] && b <= cb[i + 1]) {
+        const double L = cb[i + 1] - cb[i];
+        const double z = (b - cb[i]) / L, z2 = z * z;
+        h = (1.0 - 3.0 * z2 + 2.0 * z2 * z) * ch[i] + z * (1.0 - 2.0 * z + z2) * L * cs[i] +
+            z2 * (3.0 - 2.0 * z) * ch[i + 1] + z2 * (z - 1.0) * L * cs[i + 1];
+        dh = 6.0 * z * (z - 1.0) * ch[i] / L + (1.0 - 4.0 * z + 3.0 * z2) * cs[i] +
+             6.0 * z * (1.0 - z) * ch[i + 1] / L + z * (3.0 * z - 2.0) * cs[i + 1];
+        found = true;
+        break;
+      }
+    }
+    if (!found)
+      return false;
+  }
+  *nu = h / b;
+  *dnu = 0.5 * (dh / (b * b) - h / (b * b * b));
+  return isfinite(*nu) && isfinite(*dnu);
+}
+
+// ----------------------------------------------------------------------------- assembly
+
+// Element pass: B at every quadrature point from the nodal state, nu and
+// dnu/dB^2, then the 3x3 Newton block K + C and the per-node rhs terms
+// C a_old - (K + C) A_boundary (physics/nonlinear_assembly.cu:56-98 and the
+// term-by-term formulas of :100-186).  Returns true if every element is valid.
+__device__ bool ElementPass(const DevicePlan &P, const Span &S, const double *a, double *elem,
+                            bool newton)
+{
+  bool ok = true;
+  for (int t = S.tri0 + threadIdx.x; t < S.tri1; t += kBlockThreads) {
+    const int n0 = P.tri_nodes[3 * t], n1 = P.tri_nodes[3 * t + 1], n2 = P.tri_nodes[3 * t + 2];
+    const double av[3] = {a[n0], a[n1], a[n2]};
+    double bv[3] = {0.0, 0.0, 0.0};
+    if (P.any_boundary_value) {
+      bv[0] = P.boundary[n0];
+      bv[1] = P.boundary[n1];
+      bv[2] = P.boundary[n2];
+    }
+    const int mat = P.tri_mat[t];
+    const int coff = P.mat_curve_off[mat], ccnt = P.mat_curve_cnt[mat];
+    const double nu0 = P.mat_nu0[mat], alpha = P.mat_alpha[mat];
+    double K[9], e[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      K[k] = 0.0;
+    for (int q = 0; q < P.Q; ++q) {
+      const size_t g = (static_cast<size_t>(t) * P.Q + q) * 3;
+      const double w = P.tri_w[static_cast<size_t>(t) * P.Q + q];
+      const double gx[3] = {P.tri_gx[g], P.tri_gx[g + 1], P.tri_gx[g + 2]};
+      const double gy[3] = {P.tri_gy[g], P.tri_gy[g + 1], P.tri_gy[g + 2]};
+      const double bx = av[0] * gx[0] + av[1] * gx[1] + av[2] * gx[2];
+      const double by = av[0] * gy[0] + av[1] * gy[1] + av[2] * gy[2];
+      const double mag = hypot(bx, by);
+      double nu = 0.0, dnu = 0.0;
+      bool valid = isfinite(mag);
+      if (valid && ccnt > 0) {
+        if (!newton) {
+          nu = P.mat_cold[mat];
+          dnu = 0.0;
+        } else {
+          valid = EvalBh(P, coff, ccnt, mag, &nu, &dnu);
+        }
+      } else if (valid) {
+        nu = nu0 * (1.0 + alpha * mag * mag);
+        dnu = newton ? nu0 * alpha : 0.0;
+      }
+      if (!valid || !(nu > 0.0) || !isfinite(dnu))
+        ok = false;
+      double bdot[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        bdot[i] = bx * gx[i] + by * gy[i];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const double k = w * nu * (gx[i] * gx[j] + gy[i] * gy[j]);
+          const double c = newton ? 2.0 * w * dnu * bdot[i] * bdot[j] : 0.0;
+          K[3 * i + j] += k + c;
+          if (newton)
+            e[i] += c * av[j];
+          if (bv[j] != 0.0)
+            e[i] -= (k + c) * bv[j];
+        }
+      }
+    }
+    double *out = elem + static_cast<size_t>(t) * kElemStride;
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      out[k] = K[k];
+    out[9] = e[0];
+    out[10] = e[1];
+    out[11] = e[2];
+  }
+  return ok;
+}
+
+// Slot-owned ordered gather into SELL positions (physics/nonlinear_assembly.cu:100-140):
+// contributor order is the host element/i/j emission order; one lane owns a slot.
+template <typename Real32>
+__device__ void GatherValues(const DevicePlan &P, const Span &S, const double *elem, double *vals,
+                             Real32 *vals32)
+{
+  for (int slot = S.slot0 + threadIdx.x; slot < S.slot1; slot += kBlockThreads) {
+    double sum = 0.0;
+    const int end = P.slot_off[slot + 1];
+    for (int k = P.slot_off[slot]; k < end; ++k) {
+      const int c = P.contrib[k];
+      const double v = elem[static_cast<size_t>(c >> 5) * kElemStride + ((c >> 3) & 3) * 3 +
+                            ((c >> 1) & 3)];
+      sum += (c & 1) ? -v : v;
+    }
+    const int pos = P.slot_to_sell[slot];
+    vals[pos] = sum;
+    if (vals32)
+      vals32[pos] = static_cast<float>(sum);
+  }
+}
+
+// Row-owned rhs (physics/nonlinear_assembly.cu:142-186): PM + circuit sources, then the
+// ordered element terms.  Also the Jacobi inverse (:188-206).
+__device__ bool GatherRhs(const DevicePlan &P, const Span &S, const double *elem,
+                          const double *vals, const double *cur, double *rhs, double *inv,
+                          float *inv32)
+{
+  bool ok = true;
+  for (int row = S.row0 + threadIdx.x; row < S.row1; row += kBlockThreads) {
+    double s = P.pm_rhs[row];
+    for (int c = 0; c < P.C; ++c)
+      s += cur[c] * P.circuit_rhs[static_cast<size_t>(c) * P.n + row];
+    const int end = P.row_off[row + 1];
+    for (int k = P.row_off[row]; k < end; ++k) {
+      const int c = P.rcontrib[k];
+      const double v = elem[static_cast<size_t>(c >> 3) * kElemStride + 9 + ((c >> 1) & 3)];
+      s += (c & 1) ? -v : v;
+    }
+    rhs[row] = s;
+    const double d = vals[P.diag_sell[row]];
+    const bool good = isfinite(d) && d > 0.0;
+    inv[row] = good ? 1.0 / d : 0.0;
+    if (inv32)
+      inv32[row] = good ? static_cast<float>(1.0 / d) : 0.0f;
+    ok = ok && good;
+  }
+  return ok;
+}
+
+// ----------------------------------------------------------------------------- SpMV
+
+// y = A v over SELL-64 (lane = row of the slice); returns the local sum of
+// w[row] * y[row] when w is given.  A workgroup owns one item, so the memory
+// parallelism of one CU must come from within each wave.  Slices of width
+// <= kUnrolledWidth are software-pipelined: the gathers of slice s are issued
+// first, then the column and value loads of the wave's next slice, so waiting for
+// the gathers (vector loads complete in order) leaves the next slice's loads in
+// flight; every slice costs one memory round trip instead of two.  The sum order
+// over a row's entries is the CSR order either way.
+constexpr int kUnrolledWidth = 10;
+
+template <typename T>
+struct SliceLoads {
+  int c[kUnrolledWidth];
+  T a[kUnrolledWidth];
+  int width;
+};
+
+// Branch-free loads of one slice: entries past the slice width repeat the last column
+// (a valid address in the same cache line) with a zero value, so the whole slice is one
+// straight-line run of loads and later adds only +-0 terms.
+template <typename T>
+__device__ inline void LoadSlice(const DevicePlan &P, const T *__restrict__ vals, int s, int lane,
+                                 SliceLoads<T> &L)
+{
+  const int base = P.slice_ptr[s] + lane;
+  L.width = P.slice_width[s];
+  const int last = L.width - 1;   // every slice holds a diagonal, so width >= 1
+#pragma unroll
+  for (int k = 0; k < kUnrolledWidth; ++k) {
+    const int pos = base + max(min(k, last), 0) * kSliceRows;
+    L.c[k] = P.sell_cols[pos];
+    const T value = vals[pos];
+    L.a[k] = k <= last ? value : T(0);
+  }
+}
+
+// kPipeline = false: the plain two-round-trip loop, for rarely used products (FP64
+// residuals inside the mixed form, equation residuals) where registers matter more.
+template <typename T, typename V, bool kPipeline = true>
+__device__ double Spmv(const DevicePlan &P, const Span &S, const T *__restrict__ vals,
+                       const V *__restrict__ v, V *__restrict__ y, const V *__restrict__ w)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double local = 0.0;
+  if constexpr (!kPipeline) {
+    for (int s = S.slice0 + wave; s < S.slice1; s += kWaves) {
+      const int base = P.slice_ptr[s] + lane, width = P.slice_width[s];
+      T sum = 0;
+      if (width <= kUnrolledWidth) {
+        int c[kUnrolledWidth];
+        T a[kUnrolledWidth];
+#pragma unroll
+        for (int k = 0; k < kUnrolledWidth; ++k)
+          if (k < width) {
+            c[k] = P.sell_cols[base + k * kSliceRows];
+            a[k] = vals[base + k * kSliceRows];
+          }
+        V g[kUnrolledWidth];
+#pragma unroll
+        for (int k = 0; k < kUnrolledWidth; ++k)
+          if (k < width)
+            g[k] = v[c[k]];
+#pragma unroll
+        for (int k = 0; k < kUnrolledWidth; ++k)
+          if (k < width)
+            sum += a[k] * static_cast<T>(g[k]);
+      } else {
+        for (int k = 0; k < width; ++k) {
+          const int pos = base + k * kSliceRows;
+          sum += vals[pos] * static_cast<T>(v[P.sell_cols[pos]]);
+        }
+      }
+      const int row = s * kSliceRows + lane;
+      if (row < P.n) {
+        y[row] = static_cast<V>(sum);
+        if (w)
+          local += static_cast<double>(w[row]) * static_cast<double>(sum);
+      }
+    }
+    return local;
+  }
+  SliceLoads<T> cur;
+  cur.width = 0;
+  if (S.slice0 + wave < S.slice1)
+    LoadSlice(P, vals, S.slice0 + wave, lane, cur);
+  for (int s = S.slice0 + wave; s < S.slice1; s += kWaves) {
+    T sum = 0;
+    const int next = s + kWaves;
+    if (cur.width <= kUnrolledWidth) {
+      V g[kUnrolledWidth];
+#pragma unroll
+      for (int k = 0; k < kUnrolledWidth; ++k)
+        g[k] = v[cur.c[k]];
+      SliceLoads<T> nxt;
+      // The last slice prefetches itself again (in bounds, never used).
+      LoadSlice(P, vals, next < S.slice1 ? next : s, lane, nxt);
+#pragma unroll
+      for (int k = 0; k < kUnrolledWidth; ++k)
+        sum += cur.a[k] * static_cast<T>(g[k]);
+      cur = nxt;
+    } else {
+      const int base = P.slice_ptr[s] + lane;
+      for (int k = 0; k < cur.width; ++k) {
+        const int pos = base + k * kSliceRows;
+        sum += vals[pos] * static_cast<T>(v[P.sell_cols[pos]]);
+      }
+      cur.width = 0;
+      if (next < S.slice1)
+        LoadSlice(P, vals, next, lane, cur);
+    }
+    const int row = s * kSliceRows + lane;
+    if (row < P.n) {
+      y[row] = static_cast<V>(sum);
+      if (w)
+        local += static_cast<double>(w[row]) * static_cast<double>(sum);
+    }
+  }
+  return local;
+}
+
+// Vector passes keep kVecUnroll independent rows in flight per thread (all
+// loads of a step are issued before any arithmetic).
+constexpr int kVecUnroll = 4;
+
+// ----------------------------------------------------------------------------- block-Jacobi
+
+// Block-Jacobi preconditioner (PcgPreconditioner::kBlockJacobi, solvers/options.h:8-10):
+// blocks of bs consecutive free rows (bs = 2, 4 or 8 divides the 64 rows of a wave, so a
+// block never leaves a wave or a partition's slice range).  planes[j][i] multiplies the
+// residual of row j of i's block: z_i = sum_j planes[j][i] r_(first + j), in j order.
+template <typename T>
+struct BlockPrec {
+  const T *planes;   // [kMaxBlock][n] of this item, or nullptr
+  int bs;            // 1: point Jacobi by the inverse diagonal
+  int n;
+};
+
+// All LDS accesses of this wave issued so far are complete before any later one
+// (lanes of one wave exchanging values through LDS).
+__device__ inline void WaveLdsSync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// z_i for row i; every lane of the wave calls it with its row (valid = row inside the
+// span; invalid lanes pass r = 0).  Rows are lane-contiguous: row i sits in lane i % 64.
+template <typename T>
+__device__ inline T BlockApply(const BlockPrec<T> &B, int i, bool valid, T ri)
+{
+  const int lane = threadIdx.x & 63;
+  const int lead = lane & ~(B.bs - 1);
+  T pj[kMaxBlock];
+#pragma unroll
+  for (int j = 0; j < kMaxBlock; ++j)
+    pj[j] = (j < B.bs && valid) ? B.planes[static_cast<size_t>(j) * B.n + i] : T(0);
+  T z = T(0);
+#pragma unroll
+  for (int j = 0; j < kMaxBlock; ++j)
+    if (j < B.bs) {
+      const T rj = __shfl(ri, lead + j, 64);
+      z += pj[j] * rj;
+    }
+  return z;
+}
+
+// BuildBlockInverse (solvers/block_jacobi_preconditioner.cu:73-168) for the rows of span S:
+// each block's principal submatrix, scaled by sqrt(diag) on both sides, must have finite
+// reciprocal diagonals >= DBL_MIN and be symmetric to 1e-12 relative to max(1, |a|, |b|);
+// the block's first lane Cholesky-factors it in LDS, then lane k solves column k of the
+// inverse by forward / back substitution and unscales it.  An unusable block becomes the
+// point Jacobi rows (the reference's NaN-marked fallback, pcg_device.cuh:56-72).  Lane k
+// writes plane k of its block's rows (column k of the inverse).
+__device__ __noinline__ void BuildBlockJacobi(const DevicePlan &P, const Span &S, const double *vals,
+                                 const double *inv, int bs, double *planes, float *planes32)
+{
+  __shared__ double blk[kWaves * (64 * kMaxBlock + 128)];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double *F_all = blk + wave * (64 * kMaxBlock + 128);
+  double *sc = F_all + 64 * kMaxBlock;         // sqrt(diag) by lane
+  double *flag = sc + 64;                      // per block: 1 = unusable
+  const int bl = lane / bs, k = lane - bl * bs, lead = bl * bs;
+  double *F = F_all + bl * bs * bs;
+  const size_t n = P.n;
+  for (int base = S.row0 + wave * 64; base < S.row1; base += kBlockThreads) {
+    const int i = base + lane, first = base + lead;
+    const bool valid = i < S.row1;
+    const int width = max(0, min(bs, S.row1 - first));
+    const double d = valid ? vals[P.diag_sell[i]] : 1.0;
+    sc[lane] = valid ? sqrt(d) : 1.0;
+    for (int c = 0; c < bs; ++c)
+      F[k * bs + c] = 0.0;
+    WaveLdsSync();
+    bool bad_row = false;
+    if (valid) {
+      const double reciprocal = 1.0 / d;
+      bad_row = !isfinite(reciprocal) || reciprocal < DBL_MIN;
+      const int slice = i / kSliceRows;
+      const int pos0 = P.slice_ptr[slice] + lane, entries = P.slice_width[slice];
+      for (int e = 0; e < entries; ++e) {
+        const int pos = pos0 + e * kSliceRows;
+        const int c = P.sell_cols[pos] - first;
+        if (c >= 0 && c < width)
+          F[k * bs + c] += vals[pos] / sc[lane] / sc[lead + c];
+      }
+    }
+    const uint64_t bad_rows = __ballot(bad_row);
+    WaveLdsSync();
+    if (k == 0 && width > 0) {
+      bool bad = ((bad_rows >> lead) & ((1ull << bs) - 1ull)) != 0;
+      for (int r = 0; r < width; ++r)
+        for (int c = 0; c <= r; ++c) {
+          const double u = F[r * bs + c], v = F[c * bs + r];
+          if (!isfinite(u) || !isfinite(v) || fabs(u - v) > 1e-12 * fmax(1.0, fmax(fabs(u), fabs(v))))
+            bad = true;
+        }
+      for (int c = 0; c < width && !bad; ++c) {
+        double pivot = F[c * bs + c];
+        for (int q = 0; q < c; ++q)
+          pivot -= F[c * bs + q] * F[c * bs + q];
+        if (!(pivot > 0.0) || !isfinite(pivot)) {
+          bad = true;
+          break;
+        }
+        F[c * bs + c] = sqrt(pivot);
+        for (int r = c + 1; r < width; ++r) {
+          double v = F[r * bs + c];
+          for (int q = 0; q < c; ++q)
+            v -= F[r * bs + q] * F[c * bs + q];
+          F[r * bs + c] = v / F[c * bs + c];
+        }
+      }
+      flag[bl] = bad ? 1.0 : 0.0;
+    }
+    WaveLdsSync();
+    bool bad = width > 0 && flag[bl] != 0.0;
+    double x[kMaxBlock];
+    bool broken = false;
+    if (!bad && k < width) {
+#pragma unroll
+      for (int r = 0; r < kMaxBlock; ++r)
+        if (r < width) {
+          double v = r == k ? 1.0 : 0.0;
+#pragma unroll
+          for (int q = 0; q < r; ++q)
+            v -= F[r * bs + q] * x[q];
+          x[r] = v / F[r * bs + r];
+        }
+#pragma unroll
+      for (int r = kMaxBlock - 1; r >= 0; --r)
+        if (r < width) {
+          double v = x[r];
+#pragma unroll
+          for (int q = r + 1; q < kMaxBlock; ++q)
+            if (q < width)
+              v -= F[q * bs + r] * x[q];
+          x[r] = v / F[r * bs + r];
+        }
+#pragma unroll
+      for (int r = 0; r < kMaxBlock; ++r)
+        if (r < width) {
+          x[r] = x[r] / sc[lead + r] / sc[lane];
+          broken = broken || !isfinite(x[r]);
+        }
+    }
+    const uint64_t broken_lanes = __ballot(broken);
+    bad = bad || ((broken_lanes >> lead) & ((1ull << bs) - 1ull)) != 0;
+    // Lane k writes plane k at its block's rows; rows past the span get nothing.
+#pragma unroll
+    for (int r = 0; r < kMaxBlock; ++r)
+      if (r < width) {
+        double v;
+        if (bad)
+          v = r == k ? inv[first + r] : 0.0;
+        else
+          v = k < width ? x[r] : 0.0;
+        planes[static_cast<size_t>(k) * n + first + r] = v;
+        if (planes32)
+          planes32[static_cast<size_t>(k) * n + first + r] = static_cast<float>(v);
+      }
+    WaveLdsSync();   // the block storage is rewritten by the next chunk
+  }
+}
+
+// ----------------------------------------------------------------------------- FP64 PCG
+
+struct LinearInfo {
+  int status;
+  int iterations;
+  double residual;
+};
+
+// Jacobi PCG of the reference (solvers/pcg_kernels.cu:10-344) for one item:
+// rhs scaled by max|b|, fused update of x, r and z = D^-1 r, acceptance only on
+// the recomputed true residual with restart otherwise.  With warm != 0 the
+// iteration starts from the x already in `x` (unscaled) instead of zero; the
+// acceptance test ||b - A x|| <= tol ||b|| is unchanged.
+template <bool kPipeline = true, bool kBlock = false, class Team = BlockTeam>
+__device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const double *inv,
+                            const BlockPrec<double> &B, const double *rhs, double *x, double *r,
+                            double *p, double *ap, double tol, int maxit, bool warm, Team &R)
+{
+  constexpr bool block = kBlock;
+  const int lane = threadIdx.x & 63;
+  const int wave_row0 = R.S.row0 + (threadIdx.x & ~63);   // block mode: wave-uniform row loops
+  LinearInfo info{FEMM_INTERNAL_ERROR, 0, INFINITY};
+  const int n = P.n;
+  if (maxit < 0 || !(tol > 0.0) || !(tol < 1.0) || !isfinite(tol)) {
+    info.status = FEMM_INVALID_ARGUMENT;
+    return info;
+  }
+  // First-invalid-row precedence of the reference's validation scan.
+  double bad = 0.0, bmax = 0.0;
+  for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
+    const double d = vals[P.diag_sell[i]];
+    if (!isfinite(rhs[i]) || !isfinite(d) || !(d > 0.0))
+      bad = fmax(bad, static_cast<double>(n - i));
+    bmax = fmax(bmax, fabs(rhs[i]));
+  }
+  const double first = R.Max(bad);
+  if (first != 0.0) {
+    const int row = n - static_cast<int>(first);
+    const double d = vals[P.diag_sell[row]];
+    info.status = (!isfinite(rhs[row]) || !isfinite(d)) ? FEMM_NUMERICAL_NONFINITE
+                                                        : FEMM_LINEAR_SOLVE_BREAKDOWN;
+    return info;
+  }
+  const double scale = R.Max(bmax);
+  if (scale == 0.0) {
+    for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
+      x[i] = 0.0;
+    R.Sync();
+    info.status = FEMM_OK;
+    info.residual = 0.0;
+    return info;
+  }
+  if (warm) {
+    for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
+      x[i] = x[i] / scale;
+    R.Sync();
+    Spmv<double, double, false>(P, R.S, vals, x, ap, static_cast<const double *>(nullptr));
+    R.Sync();
+  }
+  double s0[3] = {0.0, 0.0, 0.0};
+  if constexpr (block) {
+    for (int base = wave_row0; base < R.S.row1; base += kBlockThreads) {
+      const int i = base + lane;
+      const bool valid = i < R.S.row1;
+      double b = 0.0, ri = 0.0;
+      if (valid) {
+        b = rhs[i] / scale;
+        ri = warm ? b - ap[i] : b;
+        if (!warm)
+          x[i] = 0.0;
+        r[i] = ri;
+      }
+      const double zi = BlockApply(B, i, valid, ri);
+      if (valid) {
+        p[i] = zi;
+        s0[0] += ri * ri;
+        s0[1] += ri * zi;
+        s0[2] += b * b;
+      }
+    }
+  } else {
+    for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
+      const double b = rhs[i] / scale;
+      const double ri = warm ? b - ap[i] : b;
+      if (!warm)
+        x[i] = 0.0;
+      const double zi = ri * inv[i];
+      r[i] = ri;
+      p[i] = zi;
+      s0[0] += ri * ri;
+      s0[1] += ri * zi;
+      s0[2] += b * b;
+    }
+  }
+  R.Sum(s0);
+  if (!isfinite(s0[0]) || !isfinite(s0[1])) {
+    info.status = FEMM_NUMERICAL_NONFINITE;
+    return info;
+  }
+  info.residual = sqrt(s0[0]);
+  double rho = s0[1];
+  const double threshold = tol * sqrt(s0[2]);
+  if (warm && info.residual <= threshold) {
+    // The start already satisfies the acceptance test.
+    for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
+      x[i] *= scale;
+    R.Sync();
+    info.status = FEMM_OK;
+    info.residual *= scale;
+    return info;
+  }
+  for (int it = 0; it < maxit; ++it) {
+    double pap[1] = {Spmv<double, double, kPipeline>(P, R.S, vals, p, ap, p)};
+    R.Sum(pap);
+    if (!isfinite(pap[0]) || !isfinite(rho)) {
+      info.status = FEMM_NUMERICAL_NONFINITE;
+      return info;
+    }
+    if (!(pap[0] > 0.0) || !(rho > 0.0)) {
+      info.status = FEMM_LINEAR_SOLVE_BREAKDOWN;
+      return info;
+    }
+    const double alpha = rho / pap[0];
+    double s1[2] = {0.0, 0.0};
+    if constexpr (block) {
+      // x, r updated; z = M^-1 r written over Ap (dead until the next product).
+      for (int base = wave_row0; base < R.S.row1; base += kBlockThreads) {
+        const int i = base + lane;
+        const bool valid = i < R.S.row1;
+        double ri = 0.0;
+        if (valid) {
+          x[i] = x[i] + alpha * p[i];
+          ri = r[i] - alpha * ap[i];
+          r[i] = ri;
+          s1[0] += ri * ri;
+        }
+        const double zi = BlockApply(B, i, valid, ri);
+        if (valid) {
+          ap[i] = zi;
+          s1[1] += ri * zi;
+        }
+      }
+    } else
+    for (int i0 = R.S.row0 + threadIdx.x; i0 < R.S.row1; i0 += kBlockThreads * kVecUnroll) {
+      double xv[kVecUnroll], pv[kVecUnroll], rv[kVecUnroll], av[kVecUnroll], iv[kVecUnroll];
+#pragma unroll
+      for (int u = 0; u < kVecUnroll; ++u) {
+        const int i = i0 + u * kBlockThreads;
+        if (i < R.S.row1) {
+          xv[u] = x[i];
+          pv[u] = p[i];
+          rv[u] = r[i];
+          av[u] = ap[i];
+          iv[u] = inv[i];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kVecUnroll; ++u) {
+        const int i = i0 + u * kBlockThreads;
+        if (i < R.S.row1) {
+          x[i] = xv[u] + alpha * pv[u];
+          const double ri = rv[u] - alpha * av[u];
+          r[i] = ri;
+          s1[0] += ri * ri;
+          s1[1] += ri * (ri * iv[u]);
+        }
+      }
+    }
+    R.Sum(s1);
+    info.iterations = it + 1;
+    if (!isfinite(s1[0])) {
+      info.status = FEMM_NUMERICAL_NONFINITE;
+      return info;
+    }
+    info.residual = sqrt(s1[0]);
+    if (info.residual <= threshold) {
+      // True residual on the recomputed b/scale - A x (pcg_kernels.cu:192-270).
+      Spmv<double, double, false>(P, R.S, vals, x, ap, static_cast<const double *>(nullptr));
+      R.Sync();
+      double tr[1] = {0.0};
+      for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
+        const double ri = rhs[i] / scale - ap[i];
+        r[i] = ri;
+        tr[0] += ri * ri;
+      }
+      R.Sum(tr);
+      if (!isfinite(tr[0])) {
+        info.status = FEMM_NUMERICAL_NONFINITE;
+        return info;
+      }
+      info.residual = sqrt(tr[0]);
+      if (info.residual <= threshold) {
+        bool finite = true;
+        for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
+          x[i] *= scale;
+          finite = finite && isfinite(x[i]);
+        }
+        const bool nonfinite = R.AnyTrue(!finite);
+        info.residual *= scale;
+        info.status = (nonfinite || !isfinite(info.residual)) ? FEMM_NUMERICAL_NONFINITE : FEMM_OK;
+        return info;
+      }
+      double rs[1] = {0.0};
+      if constexpr (block) {
+        for (int base = wave_row0; base < R.S.row1; base += kBlockThreads) {
+          const int i = base + lane;
+          const bool valid = i < R.S.row1;
+          const double ri = valid ? r[i] : 0.0;
+          const double zi = BlockApply(B, i, valid, ri);
+          if (valid) {
+            p[i] = zi;
+            rs[0] += ri * zi;
+          }
+        }
+      } else {
+        for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
+          const double zi = r[i] * inv[i];
+          p[i] = zi;
+          rs[0] += r[i] * zi;
+        }
+      }
+      R.Sum(rs);
+      if (!isfinite(rs[0]) || !(rs[0] > 0.0)) {
+        info.status = FEMM_LINEAR_SOLVE_BREAKDOWN;
+        return info;
+      }
+      rho = rs[0];
+      continue;
+    }
+    if (!isfinite(s1[1]) || !(s1[1] > 0.0)) {
+      info.status = FEMM_LINEAR_SOLVE_BREAKDOWN;
+      return info;
+    }
+    const double beta = s1[1] / rho;
+    if constexpr (block) {
+      for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
+        p[i] = ap[i] + beta * p[i];
+    } else
+    for (int i0 = R.S.row0 + threadIdx.x; i0 < R.S.row1; i0 += kBlockThreads * kVecUnroll) {
+      double rv[kVecUnroll], iv[kVecUnroll], pv[kVecUnroll];
+#pragma unroll
+      for (int u = 0; u < kVecUnroll; ++u) {
+        const int i = i0 + u * kBlockThreads;
+        if (i < R.S.row1) {
+          rv[u] = r[i];
+          iv[u] = inv[i];
+          pv[u] = p[i];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kVecUnroll; ++u) {
+        const int i = i0 + u * kBlockThreads;
+        if (i < R.S.row1)
+          p[i] = rv[u] * iv[u] + beta * pv[u];
+      }
+    }
+    rho = s1[1];
+    R.Sync();
+  }
+  // Iteration cap: report the true residual (pcg_kernels.cu:302-343).
+  Spmv<double, double, false>(P, R.S, vals, x, ap, static_cast<const double *>(nullptr));
+  R.Sync();
+  double fr[1] = {0.0};
+  for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
+    const double ri = rhs[i] / scale - ap[i];
+    fr[0] += ri * ri;
+  }
+  R.Sum(fr);
+  for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
+    x[i] *= scale;
+  R.Sync();
+  info.residual = sqrt(fr[0]) * scale;
+  info.status = isfinite(info.residual) ? FEMM_LINEAR_SOLVE_NOT_CONVERGED : FEMM_NUMERICAL_NONFINITE;
+  return info;
+}
+
+// ----------------------------------------------------------------------------- mixed precision
+
+struct RefinedInfo {
+  LinearInfo linear;
+  int refinements;   // FP64 true-residual evaluations
+};
+
+// r = b - A64 x in FP64; returns ||r||_2 and max|r| through the reducer.
+template <class Team>
+__device__ void TrueResidual(const DevicePlan &P, const double *vals, const double *rhs,
+                             const double *x, double *r, double *ap, double *norm, double *amax,
+                             Team &R)
+{
+  Spmv<double, double, false>(P, R.S, vals, x, ap, static_cast<const double *>(nullptr));
+  R.Sync();
+  double rr[1] = {0.0};
+  double rmax = 0.0;
+  for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
+    const double ri = rhs[i] - ap[i];
+    r[i] = ri;
+    rr[0] += ri * ri;
+    rmax = fmax(rmax, fabs(ri));
+  }
+  R.Sum(rr);
+  *amax = R.Max(rmax);
+  *norm = sqrt(rr[0]);
+}
+
+// Mixed-precision Jacobi PCG with reliable updates (Sleijpen and van der Vorst,
+// "Reliable updated residuals in hybrid Bi-CG methods", Computing 56, 1996): the
+// iteration runs in FP32 (values, SpMV, preconditioner, vectors; dot products
+// accumulated in FP64) on a correction d relative to the FP64 iterate x.  Whenever
+// the FP32 recurrence residual has dropped by `delta` since the last FP64 residual,
+// or claims convergence, x += sigma d and the FP64 true residual b - A64 x replaces
+// the recurrence residual (FP32 vectors are kept in units of sigma = max|r| at the last
+// replacement, so they never underflow); the search direction is kept, so the Krylov
+// space is not restarted.  Acceptance is the reference's test on the FP64 true residual,
+// ||b - A x|| <= tol ||b||.  A breakdown in FP32, a replacement that fails to halve the
+// true residual, or running out of replacements finishes the solve in FP64 PCG
+// from the current x.
+template <bool kBlock, class Team>
+__device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, const double *inv,
+                                     const float *vals32, const float *inv32,
+                                     const BlockPrec<double> &B64, const BlockPrec<float> &B32,
+                                     const double *rhs, double *x, double *r, double *p,
+                                     double *ap, float *d32, float *r32, float *p32, float *ap32,
+                                     double tol, int maxit, bool warm, int max_updates,
+                                     double delta, Team &R)
+{
+  RefinedInfo out{{FEMM_INTERNAL_ERROR, 0, INFINITY}, 0};
+  constexpr bool block = kBlock;
+  const int lane = threadIdx.x & 63;
+  const int wave_row0 = R.S.row0 + (threadIdx.x & ~63);   // block mode: wave-uniform row loops
+  if (maxit < 0 || !(tol > 0.0) || !(tol < 1.0) || !isfinite(tol)) {
+    out.linear.status = FEMM_INVALID_ARGUMENT;
+    return out;
+  }
+  double b2[1] = {0.0};
+  bool finite = true;
+  for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
+    b2[0] += rhs[i] * rhs[i];
+    finite = finite && isfinite(rhs[i]);
+    if (!warm)
+      x[i] = 0.0;
+  }
+  R.Sum(b2);
+  if (R.AnyTrue(!finite)) {
+    out.linear.status = FEMM_NUMERICAL_NONFINITE;
+    return out;
+  }
+  const double bnorm = sqrt(b2[0]);
+  if (bnorm == 0.0) {
+    for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
+      x[i] = 0.0;
+    R.Sync();
+    out.linear = {FEMM_OK, 0, 0.0};
+    return out;
+  }
+  const double threshold = tol * bnorm;
+  if (!(delta > 0.0) || !(delta < 1.0))
+    delta = 0.1;
+  double rnorm, rmax;
+  TrueResidual(P, vals, rhs, x, r, ap, &rnorm, &rmax, R);
+  out.refinements = 1;
+  out.linear.residual = rnorm;
+  if (!isfinite(rnorm)) {
+    out.linear.status = FEMM_NUMERICAL_NONFINITE;
+    return out;
+  }
+  if (rnorm <= threshold) {
+    out.linear.status = FEMM_OK;
+    return out;
+  }
+  double sigma = rmax;
+  double rho, rr;
+  {
+    double s0[2] = {0.0, 0.0};
+    if constexpr (block) {
+      for (int base = wave_row0; base < R.S.row1; base += kBlockThreads) {
+        const int i = base + lane;
+        const bool valid = i < R.S.row1;
+        const float ri = valid ? static_cast<float>(r[i] / sigma) : 0.0f;
+        const float zi = BlockApply(B32, i, valid, ri);
+        if (valid) {
+          r32[i] = ri;
+          p32[i] = zi;
+          d32[i] = 0.0f;
+          s0[0] += static_cast<double>(ri) * ri;
+          s0[1] += static_cast<double>(ri) * zi;
+        }
+      }
+    } else {
+      for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
+        const float ri = static_cast<float>(r[i] / sigma);
+        const float zi = ri * inv32[i];
+        r32[i] = ri;
+        p32[i] = zi;
+        d32[i] = 0.0f;
+        s0[0] += static_cast<double>(ri) * ri;
+        s0[1] += static_cast<double>(ri) * zi;
+      }
+    }
+    R.Sum(s0);
+    rr = s0[0];
+    rho = s0[1];
+  }
+  double rr_ref = rr;           // ||r32||^2 at the last replacement
+  double last_true = rnorm;     // FP64 true residual at the last replacement
+  bool fallback = false;
+  int it = 0;
+  while (it < maxit) {
+    double pap[1] = {Spmv(P, R.S, vals32, p32, ap32, p32)};
+    R.Sum(pap);
+    if (!(pap[0] > 0.0) || !(rho > 0.0) || !isfinite(pap[0]) || !isfinite(rho)) {
+      fallback = true;
+      break;
+    }
+    const float alpha = static_cast<float>(rho / pap[0]);
+    double s1[2] = {0.0, 0.0};
+    if constexpr (block) {
+      // d, r updated; z = M^-1 r written over Ap (dead until the next product).
+      for (int base = wave_row0; base < R.S.row1; base += kBlockThreads) {
+        const int i = base + lane;
+        const bool valid = i < R.S.row1;
+        float ri = 0.0f;
+        if (valid) {
+          d32[i] = d32[i] + alpha * p32[i];
+          ri = r32[i] - alpha * ap32[i];
+          r32[i] = ri;
+          s1[0] += static_cast<double>(ri) * ri;
+        }
+        const float zi = BlockApply(B32, i, valid, ri);
+        if (valid) {
+          ap32[i] = zi;
+          s1[1] += static_cast<double>(ri) * zi;
+        }
+      }
+    } else
+    for (int i0 = R.S.row0 + threadIdx.x; i0 < R.S.row1; i0 += kBlockThreads * kVecUnroll) {
+      float dv[kVecUnroll], pv[kVecUnroll], rv[kVecUnroll], av[kVecUnroll], iv[kVecUnroll];
+#pragma unroll
+      for (int u = 0; u < kVecUnroll; ++u) {
+        const int i = i0 + u * kBlockThreads;
+        if (i < R.S.row1) {
+          dv[u] = d32[i];
+          pv[u] = p32[i];
+          rv[u] = r32[i];
+          av[u] = ap32[i];
+          iv[u] = inv32[i];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kVecUnroll; ++u) {
+        const int i = i0 + u * kBlockThreads;
+        if (i < R.S.row1) {
+          d32[i] = dv[u] + alpha * pv[u];
+          const float ri = rv[u] - alpha * av[u];
+          r32[i] = ri;
+          s1[0] += static_cast<double>(ri) * ri;
+          s1[1] += static_cast<double>(ri) * (ri * iv[u]);
+        }
+      }
+    }
+    R.Sum(s1);
+    ++it;
+    if (!isfinite(s1[0]) || !isfinite(s1[1])) {
+      fallback = true;
+      break;
+    }
+    double rho_next = s1[1];
+    const bool claims = s1[0] * sigma * sigma <= threshold * threshold;
+    if (claims || s1[0] <= delta * delta * rr_ref) {
+      // Reliable update: fold the correction into x and replace the residual.
+      for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
+        x[i] += sigma * static_cast<double>(d32[i]);
+        d32[i] = 0.0f;
+      }
+      R.Sync();
+      TrueResidual(P, vals, rhs, x, r, ap, &rnorm, &rmax, R);
+      ++out.refinements;
+      out.linear.residual = rnorm;
+      if (!isfinite(rnorm)) {
+        out.linear.iterations = it;
+        out.linear.status = FEMM_NUMERICAL_NONFINITE;
+        return out;
+      }
+      if (rnorm <= threshold) {
+        out.linear.iterations = it;
+        out.linear.status = FEMM_OK;
+        return out;
+      }
+      if (rnorm > 0.5 * last_true || out.refinements > max_updates) {
+        fallback = true;
+        break;
+      }
+      last_true = rnorm;
+      // Rescale to the new residual unit; the direction carries over.
+      const double q = sigma / rmax;
+      sigma = rmax;
+      double s2[2] = {0.0, 0.0};
+      if constexpr (block) {
+        for (int base = wave_row0; base < R.S.row1; base += kBlockThreads) {
+          const int i = base + lane;
+          const bool valid = i < R.S.row1;
+          float ri = 0.0f;
+          if (valid) {
+            ri = static_cast<float>(r[i] / sigma);
+            r32[i] = ri;
+            p32[i] = static_cast<float>(q * static_cast<double>(p32[i]));
+            s2[0] += static_cast<double>(ri) * ri;
+          }
+          const float zi = BlockApply(B32, i, valid, ri);
+          if (valid) {
+            ap32[i] = zi;
+            s2[1] += static_cast<double>(ri) * zi;
+          }
+        }
+      } else {
+        for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
+          const float ri = static_cast<float>(r[i] / sigma);
+          r32[i] = ri;
+          p32[i] = static_cast<float>(q * static_cast<double>(p32[i]));
+          s2[0] += static_cast<double>(ri) * ri;
+          s2[1] += static_cast<double>(ri) * (ri * inv32[i]);
+        }
+      }
+      R.Sum(s2);
+      rho *= q * q;
+      rr_ref = s2[0];
+      rho_next = s2[1];
+    }
+    if (!(rho_next > 0.0)) {
+      fallback = true;
+      break;
+    }
+    const float beta = static_cast<float>(rho_next / rho);
+    if constexpr (block) {
+      for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
+        p32[i] = ap32[i] + beta * p32[i];
+    } else
+    for (int i0 = R.S.row0 + threadIdx.x; i0 < R.S.row1; i0 += kBlockThreads * kVecUnroll) {
+      float rv[kVecUnroll], iv[kVecUnroll], pv[kVecUnroll];
+#pragma unroll
+      for (int u = 0; u < kVecUnroll; ++u) {
+        const int i = i0 + u * kBlockThreads;
+        if (i < R.S.row1) {
+          rv[u] = r32[i];
+          iv[u] = inv32[i];
+          pv[u] = p32[i];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kVecUnroll; ++u) {
+        const int i = i0 + u * kBlockThreads;
+        if (i < R.S.row1)
+          p32[i] = rv[u] * iv[u] + beta * pv[u];
+      }
+    }
+    rho = rho_next;
+    R.Sync();
+  }
+  // Fold any pending correction, then finish in FP64 from x (or report the cap).
+  for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
+    x[i] += sigma * static_cast<double>(d32[i]);
+  R.Sync();
+  out.linear.iterations = it;
+  const LinearInfo li = Pcg64<false, kBlock>(P, vals, inv, B64, rhs, x, r, p, ap, tol,
+                                     fallback ? max(0, maxit - it) : 0, true, R);
+  out.linear.iterations += li.iterations;
+  out.linear.residual = li.residual;
+  out.linear.status = li.status;
+  if (li.status == FEMM_OK || fallback)
+    return out;
+  out.linear.status = FEMM_LINEAR_SOLVE_NOT_CONVERGED;
+  return out;
+}
+
+// ----------------------------------------------------------------------------- Newton
+
+struct ItemPointers {
+  double *elem, *vals, *rhs, *inv, *x, *r, *p, *ap, *trial;
+  float *vals32, *r32, *p32, *ap32, *d32, *inv32;
+  double *bjac;      // [kMaxBlock][n] block-Jacobi planes (nullptr: not allocated)
+  float *bjac32;
+};
+
+__device__ ItemPointers ItemWork(const DevicePlan &P, const DeviceWork &W, int item, bool mixed)
+{
+  ItemPointers ip;
+  const size_t n = P.n, s = P.sell_size;
+  ip.elem = W.elem + static_cast<size_t>(item) * P.T * kElemStride;
+  ip.vals = W.vals + item * s;
+  ip.rhs = W.rhs + item * n;
+  ip.inv = W.inv + item * n;
+  ip.x = W.x + item * n;
+  ip.r = W.r + item * n;
+  ip.p = W.p + item * n;
+  ip.ap = W.ap + item * n;
+  ip.trial = W.trial + static_cast<size_t>(item) * P.N;
+  ip.vals32 = mixed ? W.vals32 + item * s : nullptr;
+  ip.r32 = mixed ? W.r32 + item * n : nullptr;
+  ip.p32 = mixed ? W.p32 + item * n : nullptr;
+  ip.ap32 = mixed ? W.ap32 + item * n : nullptr;
+  ip.d32 = mixed ? W.d32 + item * n : nullptr;
+  ip.inv32 = mixed ? W.inv32 + item * n : nullptr;
+  ip.bjac = W.bjac ? W.bjac + item * kMaxBlock * n : nullptr;
+  ip.bjac32 = (mixed && W.bjac32) ? W.bjac32 + item * kMaxBlock * n : nullptr;
+  return ip;
+}
+
+// Full device assembly for one item; returns false if any element or diagonal is invalid.
+template <class Team>
+__device__ bool AssembleItem(const DevicePlan &P, const ItemPointers &w, const double *a,
+                             const double *cur, bool newton, Team &R)
+{
+  const bool e_ok = ElementPass(P, R.S, a, w.elem, newton);
+  R.Sync();
+  GatherValues(P, R.S, w.elem, w.vals, w.vals32);
+  __syncthreads();   // GatherRhs reads the diagonal of this workgroup's own rows
+  const bool r_ok = GatherRhs(P, R.S, w.elem, w.vals, cur, w.rhs, w.inv, w.inv32);
+  return !R.AnyTrue(!(e_ok && r_ok));
+}
+
+// ||K(A) A - b(A)|| over the free rows, with A taken at each free root
+// (solvers/nonlinear_solver.cu:549-558).  Requires a fresh assembly at `a`.
+// `xf` receives A at the free roots (n values).
+template <class Team>
+__device__ double EquationNorm(const DevicePlan &P, const ItemPointers &w, const double *a,
+                               Team &R, double *xf)
+{
+  for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
+    xf[i] = a[P.free_nodes[i]];
+  R.Sync();
+  Spmv<double, double, false>(P, R.S, w.vals, xf, w.ap, static_cast<const double *>(nullptr));
+  R.Sync();
+  double s[1] = {0.0};
+  for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
+    const double v = w.ap[i] - w.rhs[i];
+    s[0] += v * v;
+  }
+  R.Sum(s);
+  return sqrt(s[0]);
+}
+
+// Points the item's plan at its material variant; false for an index outside the set.
+__device__ bool ApplyVariant(DevicePlan &P, const DeviceVariants &V, const int32_t *variant_of_item,
+                             int item)
+{
+  if (variant_of_item == nullptr)
+    return true;
+  const int v = variant_of_item[item];
+  if (v < 0 || v >= V.count)
+    return false;
+  const size_t T = P.T, M = V.mats, K = V.knots, n = P.n;
+  P.tri_mat = V.tri_mat + v * T;
+  P.tri_circuit = V.tri_circuit + v * T;
+  P.tri_flux = V.tri_flux + v * 3 * T;
+  P.mat_nu0 = V.mat_nu0 + v * M;
+  P.mat_alpha = V.mat_alpha + v * M;
+  P.mat_cold = V.mat_cold + v * M;
+  P.mat_curve_off = V.mat_curve_off + v * M;
+  P.mat_curve_cnt = V.mat_curve_cnt + v * M;
+  P.curve_b = V.curve_b + v * K;
+  P.curve_h = V.curve_h + v * K;
+  P.curve_s = V.curve_s + v * K;
+  P.pm_rhs = V.pm_rhs + v * n;
+  P.circuit_rhs = V.circuit_rhs + v * static_cast<size_t>(max(P.C, 1)) * n;
+  P.any_nonlinear = V.any_nonlinear[v];
+  return true;
+}
+
+// True when a launch with prior results leaves this item as the earlier attempt left it.
+__device__ bool KeepPrior(const femm_item_result_t &earlier, int32_t mode)
+{
+  if (mode == kPriorRedoFailed)
+    return earlier.status == FEMM_OK;
+  if (mode == kPriorRedoUnconverged)
+    return earlier.status != FEMM_NONLINEAR_SOLVE_NOT_CONVERGED;
+  return false;
+}
+
+// Inexact-Newton inner tolerance (physics/nonlinear_model.cpp:166-174).
+__device__ double LinearTolerance(const femm_newton_options_t &o, bool adaptive, double prev)
+{
+  const double exact = o.linear_relative_tolerance;
+  if (!adaptive || !isfinite(prev) || prev <= 100.0 * o.relative_tolerance || !(exact > 0.0) ||
+      !isfinite(exact))
+    return exact;
+  return fmax(exact, 1e-8 * fmin(1.0, prev));
+}
+
+template <class Team>
+__device__ void WriteFlux(const DevicePlan &P, const double *a, double *flux, Team &R,
+                          bool lead = true)
+{
+  for (int c = 0; c < P.C; ++c) {
+    double s[1] = {0.0};
+    for (int t = R.S.tri0 + threadIdx.x; t < R.S.tri1; t += kBlockThreads) {
+      if (P.tri_circuit[t] != c)
+        continue;
+      const double *fw = P.tri_flux + 3 * static_cast<size_t>(t);
+      s[0] += fw[0] * a[P.tri_nodes[3 * t]] + fw[1] * a[P.tri_nodes[3 * t + 1]] +
+              fw[2] * a[P.tri_nodes[3 * t + 2]];
+    }
+    R.Sum(s);
+    if (lead && threadIdx.x == 0)
+      flux[c] = P.planar ? s[0] * P.depth : s[0];
+  }
+}
+
+// Result of an item before its solve starts.
+__device__ femm_item_result_t FreshResult()
+{
+  femm_item_result_t res{};
+  res.status = FEMM_INVALID_ARGUMENT;
+  res.relative_step = INFINITY;
+  res.equation_residual = INFINITY;
+  res.last_linear_tolerance = NAN;
+  res.linear_residual = INFINITY;
+  return res;
+}
+
+__device__ bool OptionsValid(const femm_newton_options_t &o)
+{
+  return o.relative_tolerance > 0.0 && isfinite(o.relative_tolerance) &&
+         o.max_newton_iterations >= 0;
+}
+
+// One linear solve of the current assembly into w.x (x0 = w.x when warm).
+// Solve forms (template argument of the solve kernels): FP64 or mixed precision, point or
+// block-Jacobi preconditioner.
+constexpr int kFormMixed = 1, kFormBlock = 2;
+
+template <int kForm, class Team>
+__device__ LinearInfo LinearSolve(const DevicePlan &P, const ItemPointers &w,
+                                  const femm_solve_config_t &cfg, double tol, int maxit,
+                                  femm_item_result_t &res, Team &R)
+{
+  constexpr bool kMixed = (kForm & kFormMixed) != 0, kBlock = (kForm & kFormBlock) != 0;
+  LinearInfo li;
+  // Block-Jacobi of the current operator (rebuilt before every solve, as the reference's
+  // BlockJacobiPlan::Prepare, block_jacobi_preconditioner.cu:188-258).
+  const int bs = kBlock ? cfg.block_size : 1;
+  if constexpr (kBlock) {
+    BuildBlockJacobi(P, R.S, w.vals, w.inv, bs, w.bjac, kMixed ? w.bjac32 : nullptr);
+    R.Sync();
+  }
+  const BlockPrec<double> B64{w.bjac, bs, P.n};
+  if constexpr (kMixed) {
+    const BlockPrec<float> B32{w.bjac32, bs, P.n};
+    const RefinedInfo ri = ReliableSolve<kBlock>(P, w.vals, w.inv, w.vals32, w.inv32, B64, B32, w.rhs, w.x,
+                                         w.r, w.p, w.ap, w.d32, w.r32, w.p32, w.ap32, tol, maxit,
+                                         cfg.pcg_warm_start != 0, cfg.max_refinements,
+                                         cfg.inner_relative_tolerance, R);
+    li = ri.linear;
+    res.refinements += ri.refinements;
+  } else {
+    li = Pcg64<true, kBlock>(P, w.vals, w.inv, B64, w.rhs, w.x, w.r, w.p, w.ap, tol, maxit,
+                             cfg.pcg_warm_start != 0, R);
+  }
+  R.Sync();
+  res.linear_iterations += li.iterations;
+  res.last_linear_tolerance = tol;
+  res.linear_residual = li.residual;
+  return li;
+}
+
+// One item's nonlinear solve; leaves the state in `a`.
+//
+// safeguarded = false: the reference Newton driver (NonlinearP1Solver::Solve / SolveBatch,
+// nonlinear_solver.cu:183-530, 667-769): plain start (cold secant at iteration 0) or
+// `warm` (Newton terms from iteration 0), per-item inexact-Newton tolerance, fkn relaxation
+// after iteration 5, NewtonStepConverged, and with require_equation_residual the
+// equation-residual gate at the returned state.
+//
+// safeguarded = true: SolveSafeguarded (nonlinear_solver.cu:534-659).  Each iteration
+// solves for the Newton candidate at the current state with the exact linear tolerance;
+// the item is accepted when the candidate's undamped step is below 100x
+// relative_tolerance and the equation residual at the current state is within
+// relative_tolerance (the returned state is the current one).  Otherwise the first alpha
+// in 1, 1/2, ..., 2^-19 whose trial state either reduces the equation residual by the
+// Armijo factor (1 - 1e-4 alpha) or already passes the residual gate becomes the new
+// state; if none does, the item stops unconverged.  From `warm` (the reference's warm-start
+// hook, nonlinear_solver.cu:667-698) the iteration starts there.
+//
+// Both policies scale the equation residual by its norm at the Dirichlet-expanded plain
+// start (nonlinear_solver.cu:565-576), independent of any warm start.  The two policies
+// share one body so that a kernel running both (the learned mode) holds one copy of the
+// assembly and the linear solve.
+template <int kForm, class Team>
+__device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers &w,
+                                        const double *cur, double *a, bool inputs_ok,
+                                        const femm_newton_options_t &o,
+                                        const femm_solve_config_t &cfg, const double *warm,
+                                        bool safeguarded, Team &R)
+{
+  femm_item_result_t res = FreshResult();
+  inputs_ok = inputs_ok && OptionsValid(o);
+  const bool has_warm = warm != nullptr;
+  const bool adaptive = o.adaptive_linear_tolerance && P.any_nonlinear;
+  const bool gate_eq = o.require_equation_residual != 0;
+
+  // Plain start: prescribed values on every fixed node (signed roots expanded), zero
+  // elsewhere; the residual scale is taken here, then the warm start replaces the free
+  // nodes.
+  for (int v = R.S.node0 + threadIdx.x; v < R.S.node1; v += kBlockThreads)
+    a[v] = P.node_to_free[v] < 0 ? P.boundary[v] : 0.0;
+  R.Sync();
+  int status = inputs_ok ? FEMM_NONLINEAR_SOLVE_NOT_CONVERGED : FEMM_INVALID_ARGUMENT;
+  double scale = 1.0, norm = INFINITY;
+  if (inputs_ok && (safeguarded || cfg.verify_equation_residual || gate_eq)) {
+    if (AssembleItem(P, w, a, cur, true, R)) {
+      R.Sync();
+      norm = EquationNorm(P, w, a, R, w.x);
+      scale = fmax(norm, DBL_MIN);
+    } else if (safeguarded) {
+      status = FEMM_ASSEMBLY_FAILED;
+    }
+  }
+  if (has_warm) {
+    for (int v = R.S.node0 + threadIdx.x; v < R.S.node1; v += kBlockThreads)
+      if (P.node_to_free[v] >= 0)
+        a[v] = warm[v];
+    R.Sync();
+    if (safeguarded && status == FEMM_NONLINEAR_SOLVE_NOT_CONVERGED) {
+      if (AssembleItem(P, w, a, cur, true, R)) {
+        R.Sync();
+        norm = EquationNorm(P, w, a, R, w.x);
+      } else {
+        status = FEMM_ASSEMBLY_FAILED;
+      }
+    }
+  }
+
+  double omega = 1.0, prev = INFINITY;
+  bool pending = false;   // step gate passed; equation residual not yet checked
+  bool eq_current = false;
+  for (int it = 0; status == FEMM_NONLINEAR_SOLVE_NOT_CONVERGED; ++it) {
+    if (safeguarded) {
+      // The assembly at the current state is current (start or accepted trial).
+      if (it >= o.max_newton_iterations)
+        break;
+      if (!isfinite(norm)) {
+        status = FEMM_NUMERICAL_NONFINITE;
+        break;
+      }
+    } else {
+      if (!pending && it >= o.max_newton_iterations)
+        break;
+      if (!AssembleItem(P, w, a, cur, has_warm || it > 0, R)) {
+        status = FEMM_ASSEMBLY_FAILED;
+        break;
+      }
+      R.Sync();
+      if (pending) {
+        // The Newton assembly at the returned state also yields its equation
+        // residual (nonlinear_solver.cu:549-558); accept only if it passes too.
+        res.equation_residual = EquationNorm(P, w, a, R, w.x) / scale;
+        eq_current = true;
+        pending = false;
+        if (res.equation_residual <= o.relative_tolerance) {
+          status = FEMM_OK;
+          break;
+        }
+        if (it >= o.max_newton_iterations)
+          break;
+      }
+    }
+    const double tol = safeguarded ? o.linear_relative_tolerance : LinearTolerance(o, adaptive, prev);
+    if (cfg.pcg_warm_start) {
+      for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
+        w.x[i] = a[P.free_nodes[i]];
+      R.Sync();
+    }
+    const LinearInfo li = LinearSolve<kForm>(P, w, cfg, tol, o.max_linear_iterations, res, R);
+    if (li.status != FEMM_OK) {
+      status = li.status;
+      break;
+    }
+    // Undamped step ||A_cand - A|| / ||A_cand|| over all nodes (nonlinear_solver.cu:326-335).
+    double s[2] = {0.0, 0.0};
+    for (int v = R.S.node0 + threadIdx.x; v < R.S.node1; v += kBlockThreads) {
+      const int f = P.node_to_free[v];
+      const double cand = f >= 0 ? static_cast<double>(P.node_sign[v]) * w.x[f] : P.boundary[v];
+      const double d = cand - a[v];
+      s[0] += d * d;
+      s[1] += cand * cand;
+    }
+    R.Sum(s);
+    const double step = sqrt(s[0]) / fmax(sqrt(s[1]), DBL_MIN);
+    res.newton_iterations = it + 1;
+    res.relative_step = step;
+    if (!isfinite(step)) {
+      status = FEMM_NUMERICAL_NONFINITE;
+      break;
+    }
+    if (safeguarded) {
+      if (step < 100.0 * o.relative_tolerance && norm / scale <= o.relative_tolerance) {
+        status = FEMM_OK;
+        break;
+      }
+      bool accepted = false;
+      double alpha = 1.0;
+      for (int bt = 0; bt < 20; ++bt, alpha *= 0.5) {
+        for (int v = R.S.node0 + threadIdx.x; v < R.S.node1; v += kBlockThreads) {
+          const int f = P.node_to_free[v];
+          const double cand =
+              f >= 0 ? static_cast<double>(P.node_sign[v]) * w.x[f] : P.boundary[v];
+          w.trial[v] = a[v] + alpha * (cand - a[v]);
+        }
+        R.Sync();
+        if (!AssembleItem(P, w, w.trial, cur, true, R)) {
+          status = FEMM_ASSEMBLY_FAILED;
+          break;
+        }
+        R.Sync();
+        const double tn = EquationNorm(P, w, w.trial, R, w.p);
+        if (isfinite(tn) &&
+            (tn <= (1.0 - 1e-4 * alpha) * norm || tn / scale <= o.relative_tolerance)) {
+          for (int v = R.S.node0 + threadIdx.x; v < R.S.node1; v += kBlockThreads)
+            a[v] = w.trial[v];
+          R.Sync();
+          norm = tn;
+          accepted = true;
+          break;
+        }
+      }
+      if (!accepted)
+        break;
+      continue;
+    }
+    // fkn-style relaxation after iteration 5 (nonlinear_solver.cu:344-351).
+    if (it > 5) {
+      if (step > prev && omega > 0.125)
+        omega *= 0.5;
+      else
+        omega += 0.1 * (1.0 - omega);
+    }
+    for (int v = R.S.node0 + threadIdx.x; v < R.S.node1; v += kBlockThreads) {
+      const int f = P.node_to_free[v];
+      const double cand = f >= 0 ? static_cast<double>(P.node_sign[v]) * w.x[f] : P.boundary[v];
+      a[v] += omega * (cand - a[v]);
+    }
+    R.Sync();
+    eq_current = false;
+    // NewtonStepConverged (nonlinear_model.cpp:186-193).
+    if (it > 0 && step < 100.0 * o.relative_tolerance && tol <= o.linear_relative_tolerance) {
+      if (!gate_eq) {
+        status = FEMM_OK;
+        break;
+      }
+      pending = true;
+    }
+    prev = step;
+  }
+  if (safeguarded) {
+    if (inputs_ok)
+      res.equation_residual = norm / scale;
+  } else if (inputs_ok && cfg.verify_equation_residual && !eq_current &&
+             (status == FEMM_OK || status == FEMM_NONLINEAR_SOLVE_NOT_CONVERGED)) {
+    if (AssembleItem(P, w, a, cur, true, R)) {
+      R.Sync();
+      res.equation_residual = EquationNorm(P, w, a, R, w.x) / scale;
+    } else {
+      res.equation_residual = INFINITY;
+    }
+  }
+  res.status = status;
+  return res;
+}
+
+enum SolveMode : int32_t {
+  kModeNewton = 0,       // the reference driver
+  kModeSafeguarded = 1,  // SolveSafeguarded
+  kModeLearned = 2       // safeguarded from `warm` within a budget, then the driver from the
+                         // plain start in the same workgroup if that attempt is not accepted
+};
+
+// The nonlinear solve of `kMode` for one item by `R`'s team, results of workgroup 0
+// written out.  With prior results, an item the earlier launch kept (KeepPrior) is left
+// untouched and the others add their counters to the earlier ones.
+template <int kForm, int kMode, class Team>
+__device__ __attribute__((always_inline)) void SolveItem(const DevicePlan &P, const DeviceWork &W, int item, bool lead,
+                          bool inputs_ok, const double *currents, const NewtonArgs &args,
+                          const double *warm, double *A, double *flux_out,
+                          femm_item_result_t *results, const femm_item_result_t *prior,
+                          Team &R)
+{
+  femm_item_result_t earlier{};
+  if (prior != nullptr)
+    earlier = prior[item];
+  const ItemPointers w = ItemWork(P, W, item, (kForm & kFormMixed) != 0);
+  double *a = A + static_cast<size_t>(item) * P.N;
+  const double *cur = currents + static_cast<size_t>(item) * P.C;
+  for (int c = 0; c < P.C; ++c)
+    inputs_ok = inputs_ok && isfinite(cur[c]);
+  const double *warm_item = warm ? warm + static_cast<size_t>(item) * P.N : nullptr;
+  // One call site of ItemSolve (the learned mode loops over its two attempts) keeps a
+  // single inlined copy of the assembly and the linear solve.
+  constexpr int kAttempts = kMode == kModeLearned ? 2 : 1;
+  femm_item_result_t res{}, spent{};
+#pragma clang loop unroll(disable)
+  for (int attempt = 0; attempt < kAttempts; ++attempt) {
+    const bool first = attempt == 0;
+    const bool safeguarded = kMode == kModeSafeguarded || (kMode == kModeLearned && first);
+    const femm_newton_options_t &o =
+        (kMode == kModeLearned && first) ? args.warm_options : args.options;
+    res = ItemSolve<kForm>(P, w, cur, a, inputs_ok, o, args.config, first ? warm_item : nullptr,
+                            safeguarded, R);
+    res.newton_iterations += spent.newton_iterations;
+    res.linear_iterations += spent.linear_iterations;
+    res.refinements += spent.refinements;
+    if (res.status == FEMM_OK)
+      break;
+    spent = res;
+  }
+  WriteFlux(P, a, flux_out + static_cast<size_t>(item) * P.C, R, lead);
+  if (prior != nullptr) {
+    res.newton_iterations += earlier.newton_iterations;
+    res.linear_iterations += earlier.linear_iterations;
+    res.refinements += earlier.refinements;
+  }
+  if (lead && threadIdx.x == 0)
+    results[item] = res;
+}
+
+// One workgroup per batch item.
+template <int kForm, int kMode>
+__global__ __launch_bounds__(kBlockThreads) void SolveKernel(
+    DevicePlan base, DeviceWork W, const double *currents, NewtonArgs args, const double *warm,
+    double *A, double *flux_out, femm_item_result_t *results, const femm_item_result_t *prior,
+    DeviceVariants VS, const int32_t *variant_of_item)
+{
+  __shared__ double lds[2 * kWaves * 4];
+  const int item = blockIdx.x;
+  if (prior != nullptr && KeepPrior(prior[item], args.prior_mode))
+    return;
+  DevicePlan P = base;
+  const bool inputs_ok = ApplyVariant(P, VS, variant_of_item, item);
+  BlockTeam R{BlockReducer{lds, 0}, WholeItem(P)};
+  SolveItem<kForm, kMode>(P, W, item, true, inputs_ok, currents, args, warm, A, flux_out, results,
+                           prior, R);
+}
+
+// `partitions` workgroups per batch item (grid = items x partitions, cooperative launch).
+// An aborted team (a barrier wait that ran out) reports INTERNAL_ERROR for its item.
+template <int kForm, int kMode>
+__global__ __launch_bounds__(kBlockThreads) void TeamSolveKernel(
+    DevicePlan base, DeviceWork W, const double *currents, NewtonArgs args, const double *warm,
+    double *A, double *flux_out, femm_item_result_t *results, const femm_item_result_t *prior,
+    DeviceVariants VS, const int32_t *variant_of_item, int partitions)
+{
+  __shared__ double lds[2 * kWaves * 4];
+  const int item = UniformInt(static_cast<int>(blockIdx.x) / partitions);
+  const int g = UniformInt(static_cast<int>(blockIdx.x) % partitions);
+  if (prior != nullptr && KeepPrior(prior[item], args.prior_mode))
+    return;
+  DevicePlan P = base;
+  const bool inputs_ok = ApplyVariant(P, VS, variant_of_item, item);
+  GridTeam R{BlockReducer{lds, 0},
+             Partition(P, partitions, g),
+             W.team_partials + static_cast<size_t>(item) * 2 * kMaxPartitions * 4,
+             W.team_barrier + 2 * static_cast<size_t>(item),
+             W.team_abort + item,
+             partitions,
+             g,
+             0};
+  SolveItem<kForm, kMode>(P, W, item, g == 0, inputs_ok, currents, args, warm, A, flux_out,
+                           results, prior, R);
+  R.Sync();
+  if (g == 0 && threadIdx.x == 0 &&
+      __hip_atomic_load(R.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+    results[item].status = FEMM_INTERNAL_ERROR;
+}
+
+// Several plans (meshes of uneven size) in one cooperative launch: entry e covers
+// blocks [first_block, first_block + items * partitions) and solves its items with its
+// own plan and workspace, `partitions` workgroups per item (one workgroup: no team
+// barrier or partial exchange at all).  With `prior`, each entry's results are the
+// earlier launch's (read and rewritten in place, as KeepPrior decides).
+template <int kForm, int kMode>
+__global__ __launch_bounds__(kBlockThreads) void GroupSolveKernel(const GroupEntry *entries,
+                                                                  int count, NewtonArgs args,
+                                                                  int prior)
+{
+  __shared__ double lds[2 * kWaves * 4];
+  int e = 0;
+  while (e + 1 < count && static_cast<int>(blockIdx.x) >= entries[e + 1].first_block)
+    ++e;
+  e = UniformInt(e);
+  const GroupEntry &E = entries[e];
+  const int local = static_cast<int>(blockIdx.x) - E.first_block;
+  const int G = UniformInt(E.partitions);
+  const int item = UniformInt(local / G), g = UniformInt(local % G);
+  const femm_item_result_t *earlier = prior ? E.results : nullptr;
+  if (earlier != nullptr && KeepPrior(earlier[item], args.prior_mode))
+    return;
+  const DevicePlan P = E.plan;
+  const DeviceWork W = E.work;
+  if (G == 1) {
+    BlockTeam R{BlockReducer{lds, 0}, WholeItem(P)};
+    SolveItem<kForm, kMode>(P, W, item, true, true, E.currents, args, nullptr, E.a, E.flux,
+                             E.results, earlier, R);
+    return;
+  }
+  GridTeam R{BlockReducer{lds, 0},
+             Partition(P, G, g),
+             W.team_partials + static_cast<size_t>(item) * 2 * kMaxPartitions * 4,
+             W.team_barrier + 2 * static_cast<size_t>(item),
+             W.team_abort + item,
+             G,
+             g,
+             0};
+  SolveItem<kForm, kMode>(P, W, item, g == 0, true, E.currents, args, nullptr, E.a, E.flux,
+                           E.results, earlier, R);
+  R.Sync();
+  if (g == 0 && threadIdx.x == 0 &&
+      __hip_atomic_load(R.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+    E.results[item].status = FEMM_INTERNAL_ERROR;
+}
+
+// Fresh team barrier words (two per item) and abort flags before a partitioned launch; a
+// kernel rather than a memset so the pair can sit in a captured graph.
+__global__ void ClearTeamKernel(uint32_t *barrier, int32_t *abort, int items)
+{
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < items; i += gridDim.x * blockDim.x) {
+    barrier[2 * i] = 0u;
+    barrier[2 * i + 1] = 0u;
+    abort[i] = 0;
+  }
+}
+
+// The same for every partitioned entry of a group (block e clears entry e).
+__global__ void ClearGroupKernel(const GroupEntry *entries, int count)
+{
+  const GroupEntry &E = entries[blockIdx.x];
+  if (E.partitions > 1)
+    for (int i = threadIdx.x; i < E.items; i += blockDim.x) {
+      E.work.team_barrier[2 * i] = 0u;
+      E.work.team_barrier[2 * i + 1] = 0u;
+      E.work.team_abort[i] = 0;
+    }
+}
+
+// ----------------------------------------------------------------------------- launches
+
+// Largest grid a cooperative launch of `kernel` may use on the current device.
+int CoResidentBlocks(const void *kernel)
+{
+  int device = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&device) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlockThreads, 0) !=
+          hipSuccess)
+    return 0;
+  return cus * per_cu;
+}
+
+template <int kForm, int kMode>
+int LaunchTeams(const DeviceBuffers &dev, int32_t items, int32_t partitions,
+                const double *currents, const NewtonArgs &args, const double *warm, double *a,
+                double *flux, femm_item_result_t *results, hipStream_t stream,
+                const LaunchExtras &x)
+{
+  const void *kernel = reinterpret_cast<const void *>(&TeamSolveKernel<kForm, kMode>);
+  const long grid = static_cast<long>(items) * partitions;
+  if (partitions > kMaxPartitions || grid > CoResidentBlocks(kernel))
+    return FEMM_INVALID_ARGUMENT;
+  // Fresh barrier words and abort flags for the launched items.
+  ClearTeamKernel<<<(items + 255) / 256, 256, 0, stream>>>(dev.work.team_barrier,
+                                                           dev.work.team_abort, items);
+  if (hipGetLastError() != hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  DevicePlan plan = dev.plan;
+  DeviceWork work = dev.work;
+  NewtonArgs launch_args = args;
+  DeviceVariants variants = dev.variants;
+  const femm_item_result_t *prior = x.prior;
+  const int32_t *variant_of_item = x.variant_of_item;
+  int parts = partitions;
+  void *params[] = {&plan, &work, &currents, &launch_args, &warm, &a, &flux, &results,
+                    &prior, &variants, &variant_of_item, &parts};
+  return hipLaunchCooperativeKernel(kernel, dim3(static_cast<unsigned>(grid)), dim3(kBlockThreads),
+                                    params, 0, stream) == hipSuccess
+             ? FEMM_OK
+             : FEMM_INTERNAL_ERROR;
+}
+
+// Runs f(std::integral_constant<int, form>{}) for a runtime solve form (kFormMixed |
+// kFormBlock bits).
+template <class F>
+int WithForm(int form, F &&f)
+{
+  switch (form & 3) {
+  case 0:
+    return f(std::integral_constant<int, 0>{});
+  case 1:
+    return f(std::integral_constant<int, 1>{});
+  case 2:
+    return f(std::integral_constant<int, 2>{});
+  default:
+    return f(std::integral_constant<int, 3>{});
+  }
+}
+
+
+// One solve form's launch: `partitions` > 1 runs the team kernel cooperatively (the learned
+// mode as a warm attempt and a plain-start redo of the failed items), otherwise one
+// workgroup per item.
+template <int kForm, int kMode>
+int LaunchFormSolve(const DeviceBuffers &dev, int32_t items, int32_t partitions,
+                    const double *currents, const NewtonArgs &args, const double *warm, double *a,
+                    double *flux, femm_item_result_t *results, hipStream_t stream,
+                    const LaunchExtras &x)
+{
+  if (partitions > 1) {
+    if constexpr (kMode == kModeLearned) {
+      NewtonArgs warm_args = args;
+      warm_args.options = args.warm_options;
+      const int st = LaunchTeams<kForm, kModeSafeguarded>(dev, items, partitions, currents,
+                                                          warm_args, warm, a, flux, results,
+                                                          stream, x);
+      if (st != FEMM_OK)
+        return st;
+      LaunchExtras redo = x;
+      redo.prior = results;
+      NewtonArgs plain_args = args;
+      plain_args.prior_mode = kPriorRedoFailed;
+      return LaunchTeams<kForm, kModeNewton>(dev, items, partitions, currents, plain_args,
+                                             nullptr, a, flux, results, stream, redo);
+    } else {
+      return LaunchTeams<kForm, kMode>(dev, items, partitions, currents, args, warm, a, flux,
+                                       results, stream, x);
+    }
+  }
+  if (partitions < 0)
+    return FEMM_INVALID_ARGUMENT;
+  hipLaunchKernelGGL((SolveKernel<kForm, kMode>), dim3(items), dim3(kBlockThreads), 0, stream,
+                     dev.plan, dev.work, currents, args, warm, a, flux, results, x.prior,
+                     dev.variants, x.variant_of_item);
+  return hipGetLastError() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
+}
+
+// Runs one of the three solve modes of form kForm.
+template <int kForm>
+int LaunchFormSolveMode(int mode, const DeviceBuffers &dev, int32_t items, int32_t partitions,
+                        const double *currents, const NewtonArgs &args, const double *warm,
+                        double *a, double *flux, femm_item_result_t *results, hipStream_t stream,
+                        const LaunchExtras &x)
+{
+  switch (mode) {
+  case kModeNewton:
+    return LaunchFormSolve<kForm, kModeNewton>(dev, items, partitions, currents, args, warm, a,
+                                               flux, results, stream, x);
+  case kModeSafeguarded:
+    return LaunchFormSolve<kForm, kModeSafeguarded>(dev, items, partitions, currents, args, warm,
+                                                    a, flux, results, stream, x);
+  default:
+    return LaunchFormSolve<kForm, kModeLearned>(dev, items, partitions, currents, args, warm, a,
+                                                flux, results, stream, x);
+  }
+}
+
+// The group kernel of form kForm (reference driver or its safeguarded recovery).
+template <int kForm>
+const void *FormGroupKernel(bool recovery)
+{
+  return recovery ? reinterpret_cast<const void *>(&GroupSolveKernel<kForm, kModeSafeguarded>)
+                  : reinterpret_cast<const void *>(&GroupSolveKernel<kForm, kModeNewton>);
+}
+
+}  // namespace
+}  // namespace femm

@@ -32,7 +32,7 @@ class SolveConfig(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("pcg_warm_start", ctypes.c_int32),
                 ("verify_equation_residual", ctypes.c_int32), ("max_refinements", ctypes.c_int32),
                 ("inner_relative_tolerance", ctypes.c_double), ("partitions", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("block_size", ctypes.c_int32)]
 
 
 class ItemResult(ctypes.Structure):
@@ -59,7 +59,7 @@ RESULT_DTYPE = np.dtype([("status", np.int32), ("newton_iterations", np.int32),
 def reference_config(**kw) -> SolveConfig:
     """The reference algorithm: FP64 Jacobi PCG from x0 = 0 in every Newton correction."""
     values = dict(precision=FP64, pcg_warm_start=0, verify_equation_residual=1,
-                  max_refinements=64, inner_relative_tolerance=0.1, partitions=1)
+                  max_refinements=64, inner_relative_tolerance=0.1, partitions=1, block_size=1)
     values.update(kw)
     return SolveConfig(**values)
 
@@ -68,7 +68,7 @@ def mixed_config(**kw) -> SolveConfig:
     """FP32 Jacobi PCG with FP64 reliable updates (true residual replaced whenever the FP32
     residual has dropped 10x), each Newton correction starting from the current state."""
     values = dict(precision=MIXED, pcg_warm_start=1, verify_equation_residual=1,
-                  max_refinements=64, inner_relative_tolerance=0.1, partitions=1)
+                  max_refinements=64, inner_relative_tolerance=0.1, partitions=1, block_size=1)
     values.update(kw)
     return SolveConfig(**values)
 
@@ -162,13 +162,19 @@ class Plan:
                                              int(newton), _dp(values), _dp(rhs), _dp(diag))
         return code, values, rhs, diag
 
-    def pcg_batch(self, values, rhs, tol, maxit):
+    def pcg_batch(self, values, rhs, tol, maxit, block_size=1):
+        """Batched PCG (femm_pcg_batch; block_size > 1: femm_pcg_batch_block)."""
         values = np.ascontiguousarray(values, np.float64).reshape(-1, self.nnz)
         B = values.shape[0]
         rhs = np.ascontiguousarray(rhs, np.float64).reshape(B, self.n)
         x, info = np.zeros((B, self.n)), np.zeros((B, 3))
-        _check(self._lib.femm_pcg_batch(self._handle, B, _dp(values), _dp(rhs), c_d(tol),
-                                        int(maxit), _dp(x), _dp(info)), "femm_pcg_batch")
+        if block_size == 1:
+            _check(self._lib.femm_pcg_batch(self._handle, B, _dp(values), _dp(rhs), c_d(tol),
+                                            int(maxit), _dp(x), _dp(info)), "femm_pcg_batch")
+        else:
+            _check(self._lib.femm_pcg_batch_block(self._handle, B, _dp(values), _dp(rhs),
+                                                  c_d(tol), int(maxit), int(block_size), _dp(x),
+                                                  _dp(info)), "femm_pcg_batch_block")
         return x, info
 
     def solve_batch(self, currents, options: CNewtonOptions = None, config: SolveConfig = None,
@@ -295,3 +301,67 @@ class Plan:
         _check(self._lib.femm_finalize_batch(self._handle, B, _dp(states), _dp(bx), _dp(by),
                                              _dp(flux)), "femm_finalize_batch")
         return bx, by, flux
+
+
+class GroupEntry(ctypes.Structure):
+    _fields_ = [("plan", ctypes.c_void_p), ("items", ctypes.c_int32),
+                ("partitions", ctypes.c_int32), ("currents", ctypes.c_void_p),
+                ("a", ctypes.c_void_p), ("flux", ctypes.c_void_p), ("results", ctypes.c_void_p)]
+
+
+class Group:
+    """Plans of uneven size solved together in one cooperative launch (femm_group_*).
+
+    entries: sequence of dicts with plan, items, currents / a / flux / results (device
+    pointers of the sizes Plan.solve_batch_device takes) and optionally partitions (1).
+    The plans must stay open while the group is used."""
+
+    def __init__(self, entries):
+        self._lib = library()
+        self.plans = [e["plan"] for e in entries]
+        arr = (GroupEntry * len(entries))()
+        for i, e in enumerate(entries):
+            arr[i] = GroupEntry(e["plan"].handle, int(e["items"]), int(e.get("partitions", 1)),
+                                e["currents"], e["a"], e["flux"], e["results"])
+        self.count = len(entries)
+        self.partitions = [int(e.get("partitions", 1)) for e in entries]
+        self._handle = ctypes.c_void_p()
+        _check(self._lib.femm_group_create(self.count, arr, ctypes.byref(self._handle)),
+               "femm_group_create")
+
+    def close(self):
+        if self._handle:
+            self._lib.femm_group_destroy(self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_partitions(self, partitions):
+        p = np.ascontiguousarray(partitions, np.int32).reshape(self.count)
+        _check(self._lib.femm_group_set_partitions(self._handle, _ip(p)),
+               "femm_group_set_partitions")
+        self.partitions = [int(v) for v in p]
+
+    def autotune(self, config: SolveConfig = None, work=None):
+        """Per-entry partitions from the group autotuner; returns (partitions, blocks)."""
+        config = config or reference_config()
+        p = np.zeros(self.count, np.int32)
+        blocks = ctypes.c_int32(0)
+        w = None if work is None else np.ascontiguousarray(work, np.float64).reshape(self.count)
+        _check(self._lib.femm_group_autotune(self._handle, ctypes.byref(config),
+                                             None if w is None else _dp(w), _ip(p),
+                                             ctypes.byref(blocks)), "femm_group_autotune")
+        self.partitions = [int(v) for v in p]
+        return self.partitions, int(blocks.value)
+
+    def solve(self, stream_ptr, options: CNewtonOptions = None, config: SolveConfig = None):
+        """One launch over every entry (femm_newton_group); nothing is synchronised."""
+        options = options or production_options()
+        config = config or reference_config()
+        _check(self._lib.femm_newton_group(self._handle, ctypes.byref(options),
+                                           ctypes.byref(config), ctypes.c_void_p(stream_ptr)),
+               "femm_newton_group")

@@ -39,7 +39,11 @@ typedef struct femm_solve_config {
                                   > 1 = the item's rows, nodes and triangles are split over
                                   that many workgroups (small batches, large meshes);
                                   0 = chosen by the library (femm_plan_autotune) */
-  int32_t reserved;
+  int32_t block_size;          /* PCG preconditioner: 0 or 1 = point Jacobi (the reference
+                                  default); 2, 4 or 8 = block-Jacobi on blocks of that many
+                                  consecutive free rows (PcgPreconditioner::kBlockJacobi,
+                                  solvers/options.h:8-10, block_jacobi_preconditioner.cu),
+                                  FP32 planes in the mixed form */
 } femm_solve_config_t;
 
 /* Per-item outcome (reference SolveInfo plus the Newton counters of
@@ -86,6 +90,14 @@ int femm_assemble_batch(femm_plan *plan, int32_t items, const double *states,
 int femm_pcg_batch(femm_plan *plan, int32_t items, const double *values, const double *rhs,
                    double relative_tolerance, int32_t max_iterations, double *x, double *info);
 
+/* Same with the block-Jacobi preconditioner of block_size (2, 4 or 8) consecutive rows,
+ * built on the device from each item's values (DeterministicPcgKernel<..., kBlock = true>,
+ * solvers/pcg_kernels.cu:10-344, with BlockJacobiPlan::Prepare,
+ * block_jacobi_preconditioner.cu:188-258); block_size 1 = femm_pcg_batch. */
+int femm_pcg_batch_block(femm_plan *plan, int32_t items, const double *values, const double *rhs,
+                         double relative_tolerance, int32_t max_iterations, int32_t block_size,
+                         double *x, double *info);
+
 /* Batched nonlinear Newton solve with host arrays.  Replaces
  * NonlinearP1Solver::SolveBatch (solvers/nonlinear_solver.cu:183-530) and
  * FinalizeResult (:16-83).  currents[items][C]; warm_starts[items][N] or NULL;
@@ -112,6 +124,47 @@ int femm_newton_batch_device(femm_plan *plan, int32_t items, const double *curre
  * (PcgCooperativeShardCount, solvers/options.cpp:43-53). */
 int femm_plan_autotune(femm_plan *plan, int32_t items, const femm_solve_config_t *config,
                        int32_t *partitions);
+
+/* One plan's batch inside a group solve: meshes of uneven size solved in one launch.
+ * Pointers are device memory of the sizes femm_newton_batch_device takes for `plan`. */
+typedef struct femm_group_entry {
+  femm_plan *plan;
+  int32_t items;
+  int32_t partitions;            /* workgroups per item (>= 1) */
+  const double *currents;        /* [items][C] */
+  double *a;                     /* [items][N] */
+  double *flux;                  /* [items][C] */
+  femm_item_result_t *results;   /* [items] */
+} femm_group_entry_t;
+
+/* A fixed set of entries solved together (no counterpart in the reference, which batches
+ * items of one mesh only: solvers/nonlinear_solver.cu:662-742).  The entries' pointers are
+ * kept; their contents are read / written by each femm_newton_group call.  An entry's plan
+ * may appear once per group.  The plans must outlive the group. */
+typedef struct femm_group femm_group;
+
+int femm_group_create(int32_t count, const femm_group_entry_t *entries, femm_group **out);
+void femm_group_destroy(femm_group *group);
+
+/* Workgroups per item for every entry (partitions[count]); all of the group's workgroups
+ * must be co-resident on the device. */
+int femm_group_set_partitions(femm_group *group, const int32_t *partitions);
+
+/* Group autotuner.  Times the probe solve of femm_plan_autotune on every entry's plan at
+ * partition counts 1, 2, 4, ... and chooses each entry's partitions so that the slowest
+ * entry, priced as work[e] x its probe time (work = expected PCG iterations per item; NULL:
+ * sqrt(free DOFs), the Jacobi-PCG growth on a 2D mesh), finishes as early as possible with
+ * all workgroups co-resident.  Applies the choice and returns it in partitions[count]
+ * (may be NULL) and the launch's workgroup count in *blocks (may be NULL). */
+int femm_group_autotune(femm_group *group, const femm_solve_config_t *config,
+                        const double *work, int32_t *partitions, int32_t *blocks);
+
+/* The reference Newton driver (per item as femm_newton_batch) for every entry of the group
+ * in one cooperative launch on `stream`, not synchronised; with options->recover_nonconvergence
+ * a second group launch re-solves exhausted items as femm_newton_batch does.  All entries
+ * share options and config (config->partitions is ignored: the group's own apply). */
+int femm_newton_group(femm_group *group, const femm_newton_options_t *options,
+                      const femm_solve_config_t *config, void *stream);
 
 /* Per-item material variants on the plan's mesh.  Each variant is a full model with the
  * plan's nodes, triangles, Dirichlet nodes and values, node constraints, problem type,

@@ -752,12 +752,121 @@ int PrepareSymbolic(const Model &m, Symbolic *s)
   return FEMM_OK;
 }
 
-// solvers/pcg_kernels.cu:10-344, restated with sequential sums.  Jacobi
-// preconditioner by the inverse diagonal; x0 = 0; rhs scaled by max|b|;
+// Block-Jacobi preconditioner (PcgPreconditioner::kBlockJacobi, solvers/options.h:8-10):
+// BuildBlockInverse (solvers/block_jacobi_preconditioner.cu:73-168).  Block b holds the
+// rows [b bs, b bs + width), width = min(bs, n - b bs).  Its principal submatrix, scaled by
+// sqrt(diag) on both sides, must have finite reciprocal diagonals >= DBL_MIN and be
+// symmetric to 1e-12 (relative to max(1, |a|, |b|)); it is Cholesky-factored and each
+// column j of the inverse solved by forward / back substitution, then unscaled.
+// planes[j n + b bs + k] = entry k of column j; fallback[b] = 1 marks an unusable block,
+// whose rows use the point Jacobi inverse instead (pcg_device.cuh:56-72).
+void BuildBlockJacobi(int n, const int32_t *rp, const int32_t *ci, const double *val,
+                      const double *dg, int bs, std::vector<double> *planes,
+                      std::vector<char> *fallback)
+{
+  const int blocks = (n + bs - 1) / bs;
+  planes->assign(static_cast<size_t>(bs) * n, 0.0);
+  fallback->assign(blocks, 0);
+  std::vector<double> f(bs * bs), scale(bs), x(bs);
+  for (int b = 0; b < blocks; ++b) {
+    const int first = b * bs, width = std::min(bs, n - first);
+    std::fill(f.begin(), f.end(), 0.0);
+    for (int k = 0; k < bs; ++k)
+      scale[k] = k < width ? std::sqrt(dg[first + k]) : 1.0;
+    for (int k = 0; k < width; ++k)
+      for (int e = rp[first + k]; e < rp[first + k + 1]; ++e) {
+        const int c = ci[e] - first;
+        if (c >= 0 && c < width)
+          f[k * bs + c] += val[e] / scale[k] / scale[c];
+      }
+    bool bad = false;
+    for (int r = 0; r < width; ++r) {
+      const double reciprocal = 1.0 / dg[first + r];
+      if (!std::isfinite(reciprocal) || reciprocal < std::numeric_limits<double>::min())
+        bad = true;
+      for (int c = 0; c <= r; ++c) {
+        const double u = f[r * bs + c], v = f[c * bs + r];
+        if (!std::isfinite(u) || !std::isfinite(v) ||
+            std::fabs(u - v) > 1e-12 * std::fmax(1.0, std::fmax(std::fabs(u), std::fabs(v))))
+          bad = true;
+      }
+    }
+    for (int c = 0; c < width && !bad; ++c) {
+      double pivot = f[c * bs + c];
+      for (int k = 0; k < c; ++k)
+        pivot -= f[c * bs + k] * f[c * bs + k];
+      if (!(pivot > 0.0) || !std::isfinite(pivot)) {
+        bad = true;
+        break;
+      }
+      f[c * bs + c] = std::sqrt(pivot);
+      for (int r = c + 1; r < width; ++r) {
+        double v = f[r * bs + c];
+        for (int k = 0; k < c; ++k)
+          v -= f[r * bs + k] * f[c * bs + k];
+        f[r * bs + c] = v / f[c * bs + c];
+      }
+    }
+    // Column j of the inverse (all columns, then the block is kept or dropped whole).
+    std::vector<double> cols(static_cast<size_t>(bs) * bs, 0.0);
+    for (int j = 0; j < width && !bad; ++j) {
+      for (int r = 0; r < width; ++r) {
+        double v = r == j ? 1.0 : 0.0;
+        for (int k = 0; k < r; ++k)
+          v -= f[r * bs + k] * x[k];
+        x[r] = v / f[r * bs + r];
+      }
+      for (int r = width - 1; r >= 0; --r) {
+        double v = x[r];
+        for (int k = r + 1; k < width; ++k)
+          v -= f[k * bs + r] * x[k];
+        x[r] = v / f[r * bs + r];
+      }
+      for (int r = 0; r < width; ++r) {
+        x[r] = x[r] / scale[r] / scale[j];
+        if (!std::isfinite(x[r]))
+          bad = true;
+        cols[j * bs + r] = x[r];
+      }
+    }
+    (*fallback)[b] = bad ? 1 : 0;
+    if (!bad)
+      for (int j = 0; j < width; ++j)
+        for (int k = 0; k < width; ++k)
+          (*planes)[static_cast<size_t>(j) * n + first + k] = cols[j * bs + k];
+  }
+}
+
+// z = M^-1 r: point Jacobi (bs <= 1) or the block form of BuildBlockJacobi
+// (PcgPreconditionRow, solvers/pcg_device.cuh:56-72; sum over the block's columns in order).
+void Precondition(int n, int bs, const std::vector<double> &planes,
+                  const std::vector<char> &fallback, const std::vector<double> &inv,
+                  const std::vector<double> &r, std::vector<double> *z)
+{
+  for (int i = 0; i < n; ++i) {
+    if (bs <= 1 || fallback[i / bs]) {
+      (*z)[i] = r[i] * inv[i];
+      continue;
+    }
+    const int first = i / bs * bs, width = std::min(bs, n - first);
+    double v = 0.0;
+    for (int j = 0; j < width; ++j)
+      v += planes[static_cast<size_t>(j) * n + i] * r[first + j];
+    (*z)[i] = v;
+  }
+}
+
+// Block size of the PCG preconditioner for the Newton drivers below (the reference
+// selects it process-wide, solvers/options.h:9 and app/cli.cpp:38-47); 1 = point Jacobi.
+std::atomic<int> g_block_size{1};
+
+// solvers/pcg_kernels.cu:10-344, restated with sequential sums.  Jacobi (or, with
+// block_size > 1, block-Jacobi) preconditioner; x0 = 0; rhs scaled by max|b|;
 // acceptance only on the recomputed true residual; restart otherwise.
 void Pcg(int n, const int32_t *rp, const int32_t *ci, const double *val, const double *dg,
-         const double *b, double tol, int maxit, double *x, double info[3])
+         const double *b, double tol, int maxit, double *x, double info[3], int block_size = -1)
 {
+  const int bs = block_size < 0 ? g_block_size.load() : std::max(block_size, 1);
   info[0] = FEMM_INTERNAL_ERROR;
   info[1] = 0;
   info[2] = std::numeric_limits<double>::infinity();
@@ -776,6 +885,10 @@ void Pcg(int n, const int32_t *rp, const int32_t *ci, const double *val, const d
     x[i] = 0.0;
     inv[i] = 1.0 / dg[i];
   }
+  std::vector<double> planes;
+  std::vector<char> fallback;
+  if (bs > 1)
+    BuildBlockJacobi(n, rp, ci, val, dg, bs, &planes, &fallback);
   double scale = 0.0;
   for (int i = 0; i < n; ++i)
     scale = std::fmax(scale, std::fabs(b[i]));
@@ -791,11 +904,11 @@ void Pcg(int n, const int32_t *rp, const int32_t *ci, const double *val, const d
     return s;
   };
   double rr = 0.0, rho = 0.0;
-  for (int i = 0; i < n; ++i) {
+  for (int i = 0; i < n; ++i)
     r[i] = b[i] / scale;
-    z[i] = r[i] * inv[i];
+  Precondition(n, bs, planes, fallback, inv, r, &z);
+  for (int i = 0; i < n; ++i)
     p[i] = z[i];
-  }
   for (int i = 0; i < n; ++i) {
     rr += r[i] * r[i];
     rho += r[i] * z[i];
@@ -826,9 +939,10 @@ void Pcg(int n, const int32_t *rp, const int32_t *ci, const double *val, const d
       x[i] += alpha * p[i];
       r[i] -= alpha * ap[i];
       rsq += r[i] * r[i];
-      z[i] = r[i] * inv[i];
-      rz += r[i] * z[i];
     }
+    Precondition(n, bs, planes, fallback, inv, r, &z);
+    for (int i = 0; i < n; ++i)
+      rz += r[i] * z[i];
     info[1] = it + 1;
     if (!std::isfinite(rsq)) {
       info[0] = FEMM_NUMERICAL_NONFINITE;
@@ -859,8 +973,8 @@ void Pcg(int n, const int32_t *rp, const int32_t *ci, const double *val, const d
         return;
       }
       double restart = 0.0;
+      Precondition(n, bs, planes, fallback, inv, r, &z);
       for (int i = 0; i < n; ++i) {
-        z[i] = r[i] * inv[i];
         p[i] = z[i];
         restart += r[i] * z[i];
       }
@@ -1609,8 +1723,31 @@ void oracle_pcg(int n, const int32_t *row_offsets, const int32_t *column_indices
                 double relative_tolerance, int max_iterations, double *x, double info[3])
 {
   Pcg(n, row_offsets, column_indices, values, diagonal, rhs, relative_tolerance, max_iterations, x,
-      info);
+      info, 1);
 }
+
+void oracle_pcg_block(int n, const int32_t *row_offsets, const int32_t *column_indices,
+                      const double *values, const double *diagonal, const double *rhs,
+                      double relative_tolerance, int max_iterations, int block_size, double *x,
+                      double info[3])
+{
+  Pcg(n, row_offsets, column_indices, values, diagonal, rhs, relative_tolerance, max_iterations, x,
+      info, block_size);
+}
+
+void oracle_block_jacobi(int n, const int32_t *row_offsets, const int32_t *column_indices,
+                         const double *values, const double *diagonal, int block_size,
+                         double *planes, int8_t *fallback)
+{
+  std::vector<double> pl;
+  std::vector<char> fb;
+  BuildBlockJacobi(n, row_offsets, column_indices, values, diagonal, block_size, &pl, &fb);
+  std::copy(pl.begin(), pl.end(), planes);
+  for (size_t b = 0; b < fb.size(); ++b)
+    fallback[b] = static_cast<int8_t>(fb[b]);
+}
+
+void oracle_set_block_size(int block_size) { g_block_size = std::max(block_size, 1); }
 
 int oracle_newton_solve(const femm_model_t *model, const double *currents,
                         const femm_newton_options_t *options, const double *warm_start, double *a,

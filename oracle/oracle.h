@@ -64,6 +64,23 @@ void oracle_pcg(int n, const int32_t *row_offsets, const int32_t *column_indices
                 const double *values, const double *diagonal, const double *rhs,
                 double relative_tolerance, int max_iterations, double *x, double info[3]);
 
+/* Same PCG with the block-Jacobi preconditioner of block_size consecutive rows
+ * (1 = point Jacobi). */
+void oracle_pcg_block(int n, const int32_t *row_offsets, const int32_t *column_indices,
+                      const double *values, const double *diagonal, const double *rhs,
+                      double relative_tolerance, int max_iterations, int block_size, double *x,
+                      double info[3]);
+
+/* The block-Jacobi inverse: planes[block_size][n] (planes[j n + i] multiplies r of the j-th
+ * row of i's block) and fallback[ceil(n / block_size)] (1 = block replaced by point Jacobi). */
+void oracle_block_jacobi(int n, const int32_t *row_offsets, const int32_t *column_indices,
+                         const double *values, const double *diagonal, int block_size,
+                         double *planes, int8_t *fallback);
+
+/* Preconditioner block size of every later Newton / safeguarded / batch solve (1 = point
+ * Jacobi, the default). */
+void oracle_set_block_size(int block_size);
+
 /* Per-item result of a nonlinear solve. */
 typedef struct oracle_result {
   int32_t status;

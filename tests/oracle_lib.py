@@ -158,6 +158,42 @@ def pcg(rows, cols, values, diag, rhs, tol, maxit):
     return x, int(info[0]), int(info[1]), info[2]
 
 
+def pcg_block(rows, cols, values, diag, rhs, tol, maxit, block_size):
+    n = len(rhs)
+    x = np.zeros(n)
+    info = np.zeros(3)
+    rows, cols = (np.ascontiguousarray(a, np.int32) for a in (rows, cols))
+    values, diag, rhs = (np.ascontiguousarray(a, np.float64) for a in (values, diag, rhs))
+    oracle().oracle_pcg_block(n, _ip(rows), _ip(cols), _dp(values), _dp(diag), _dp(rhs),
+                              c_d(tol), int(maxit), int(block_size), _dp(x), _dp(info))
+    return x, int(info[0]), int(info[1]), info[2]
+
+
+def block_jacobi(rows, cols, values, diag, block_size):
+    """(planes[block_size][n], fallback[blocks]) of the oracle's block-Jacobi inverse."""
+    n = len(diag)
+    rows, cols = (np.ascontiguousarray(a, np.int32) for a in (rows, cols))
+    values, diag = (np.ascontiguousarray(a, np.float64) for a in (values, diag))
+    planes = np.zeros((block_size, n))
+    fallback = np.zeros((n + block_size - 1) // block_size, np.int8)
+    oracle().oracle_block_jacobi(n, _ip(rows), _ip(cols), _dp(values), _dp(diag), int(block_size),
+                                 _dp(planes), fallback.ctypes.data_as(ctypes.c_void_p))
+    return planes, fallback
+
+
+class block_size:
+    """Context: later oracle Newton solves use block-Jacobi of this block size."""
+
+    def __init__(self, size):
+        self.size = size
+
+    def __enter__(self):
+        oracle().oracle_set_block_size(int(self.size))
+
+    def __exit__(self, *exc):
+        oracle().oracle_set_block_size(1)
+
+
 def newton_solve(model, currents, options, warm_start=None):
     N, T, C = model.node_count, model.triangle_count, model.circuit_count
     a, bx, by, flux = np.zeros(N), np.zeros(T), np.zeros(T), np.zeros(max(1, C))

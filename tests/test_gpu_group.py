@@ -1,0 +1,139 @@
+"""Group solves: meshes of uneven size in one cooperative launch (femm_newton_group).
+
+Per item the group launch runs the same solve as Plan.solve_batch with the same number of
+workgroups per item, so its outputs must be bitwise identical to the per-plan solves
+(A, flux and every result field); against the FP64 oracle the usual bars hold (A within
+1e-9 relative L2, flux within 1e-7 relative, step < 1e-6 and equation residual <= 1e-8).
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from femm_gpu_amd import meshes
+from femm_gpu_amd.model import accepted_options
+from femm_gpu_amd.solver import RESULT_DTYPE, Group, Plan, mixed_config, reference_config
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def say(*args):
+    print(*args, flush=True)
+    sys.stdout.flush()
+
+
+def _rel_l2(x, y):
+    return float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
+
+
+def _ring(cells, seed, bh):
+    return meshes.square_coil_ring(cells=cells, seed=seed, ring_bh=bh)
+
+
+class _Entry:
+    """Device buffers of one group entry and the per-plan reference solve."""
+
+    def __init__(self, cells, items, partitions, seed, bh=True):
+        self.model, base = _ring(cells, seed, bh)
+        self.plan = Plan(self.model)
+        scale = np.linspace(1.0, 0.4, items)[:, None]
+        self.currents = np.ascontiguousarray(base[None, :] * scale)
+        self.items, self.partitions = items, partitions
+        self.cur = torch.from_numpy(self.currents).cuda()
+        self.a = torch.zeros((items, self.plan.N), dtype=torch.float64, device="cuda")
+        self.flux = torch.zeros((items, self.plan.C), dtype=torch.float64, device="cuda")
+        self.res = torch.zeros(items * RESULT_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+
+    def spec(self):
+        return dict(plan=self.plan, items=self.items, partitions=self.partitions,
+                    currents=self.cur.data_ptr(), a=self.a.data_ptr(), flux=self.flux.data_ptr(),
+                    results=self.res.data_ptr())
+
+    def outputs(self):
+        return (self.a.cpu().numpy(), self.flux.cpu().numpy(),
+                np.frombuffer(self.res.cpu().numpy().tobytes(), dtype=RESULT_DTYPE))
+
+
+def _group_vs_plans(entries, cfg, opts):
+    group = Group([e.spec() for e in entries])
+    stream = torch.cuda.current_stream()
+    group.solve(stream.cuda_stream, opts, cfg())
+    torch.cuda.synchronize()
+    for e in entries:
+        a, flux, res = e.outputs()
+        single = e.plan.solve_batch(e.currents, opts, cfg(partitions=e.partitions))
+        say(f"  {e.model.name}: items {e.items} x {e.partitions} wg, newton "
+            f"{res['newton_iterations'].tolist()} status {res['status'].tolist()}")
+        assert np.array_equal(a, single["a"]), e.model.name
+        assert np.array_equal(flux, single["flux"]), e.model.name
+        assert res.tobytes() == single["results"].tobytes(), e.model.name
+    return group
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_group_matches_per_plan_solves(precision):
+    say(f"case group vs per-plan solves ({precision}): rings 8/12/16/24 cells")
+    cfg = reference_config if precision == "fp64" else mixed_config
+    entries = [_Entry(8, 2, 1, 1), _Entry(12, 1, 2, 2), _Entry(16, 3, 1, 3), _Entry(24, 1, 4, 4)]
+    opts = accepted_options()
+    group = _group_vs_plans(entries, cfg, opts)
+    # A second launch reuses the uploaded table and gives the same bits.
+    before = [e.outputs() for e in entries]
+    group.solve(torch.cuda.current_stream().cuda_stream, opts, cfg())
+    torch.cuda.synchronize()
+    for e, (a, flux, res) in zip(entries, before):
+        a2, flux2, res2 = e.outputs()
+        assert np.array_equal(a, a2) and np.array_equal(flux, flux2)
+        assert res.tobytes() == res2.tobytes()
+    group.close()
+    for e in entries:
+        e.plan.close()
+
+
+def test_group_matches_oracle():
+    say("case group vs oracle: rings 10 (x2) and 20 (x1, 2 wg), FP64 reference form")
+    entries = [_Entry(10, 2, 1, 7), _Entry(20, 1, 2, 8)]
+    opts = accepted_options()
+    group = Group([e.spec() for e in entries])
+    group.solve(torch.cuda.current_stream().cuda_stream, opts, reference_config())
+    torch.cuda.synchronize()
+    for e in entries:
+        a, flux, res = e.outputs()
+        for b in range(e.items):
+            ref = ol.newton_solve(e.model, e.currents[b], opts)
+            say(f"  {e.model.name} item {b}: newton {res['newton_iterations'][b]} (oracle "
+                f"{ref['newton']}) dA {_rel_l2(a[b], ref['a']):.2e}")
+            assert res["status"][b] == 0 and ref["status"] == 0
+            assert res["relative_step"][b] < 1e-6 and res["equation_residual"][b] <= 1e-8
+            assert res["newton_iterations"][b] == ref["newton"]
+            assert _rel_l2(a[b], ref["a"]) <= 1e-9
+            assert np.abs(flux[b] - ref["flux"]).max() <= 1e-7 * np.abs(ref["flux"]).max()
+    group.close()
+    for e in entries:
+        e.plan.close()
+
+
+def test_group_autotune_fits_and_solves():
+    say("case group autotune: linear rings (mu_r 1000) of 8..128 cells, one item each")
+    cells = [8, 11, 16, 23, 32, 45, 64, 90, 128]
+    entries = [_Entry(c, 1, 1, 20 + i, bh=False) for i, c in enumerate(cells)]
+    group = Group([e.spec() for e in entries])
+    parts, blocks = group.autotune(reference_config())
+    say(f"  partitions {parts}, blocks {blocks}")
+    assert blocks == sum(p * e.items for p, e in zip(parts, entries))
+    assert all(p >= 1 and (p & (p - 1)) == 0 for p in parts)
+    assert parts[-1] >= parts[0]
+    for e, p in zip(entries, parts):
+        e.partitions = p
+    _group_vs_plans(entries, reference_config, accepted_options())
+    for e in entries:
+        _, _, res = e.outputs()
+        assert res["status"][0] == 0
+        assert res["relative_step"][0] < 1e-6 and res["equation_residual"][0] <= 1e-8
+    group.close()
+    for e in entries:
+        e.plan.close()

@@ -319,3 +319,26 @@ def test_partitioned_solve_matches_oracle(precision):
     assert auto["results"][0]["status"] == 0
     assert _rel_l2(auto["a"][0], a_ref[0]) <= bar
     plan.close()
+
+
+def test_c_core_single_rhs_partitioned_matches_oracle():
+    """the 62k-node C-core (2 mm gap, NdFeB block, two coils, B-H iron),
+    one right-hand side, solved by the autotuned partitioned FP64 path: A within 1e-9
+    relative L2 of the oracle, flux within 1e-7, same Newton count, both gates."""
+    model, currents = meshes.c_core()
+    say(f"case c_core single RHS: N={model.node_count}")
+    plan = Plan(model)
+    opts = accepted_options()
+    parts = plan.autotune(1, reference_config(partitions=0))
+    out = plan.solve_batch(currents[None, :], opts, reference_config(partitions=0))
+    ref = ol.newton_solve(model, currents, opts)
+    r = out["results"][0]
+    say(f"  {parts} workgroups: status {r['status']} newton {r['newton_iterations']} "
+        f"(oracle {ref['newton']}) pcg {r['linear_iterations']} (oracle {ref['linear']}) "
+        f"dA {_rel_l2(out['a'][0], ref['a']):.2e}")
+    assert r["status"] == ref["status"] == 0
+    assert r["relative_step"] < 1e-6 and r["equation_residual"] <= 1e-8
+    assert r["newton_iterations"] == ref["newton"]
+    assert _rel_l2(out["a"][0], ref["a"]) <= 1e-9
+    assert np.abs(out["flux"][0] - ref["flux"]).max() <= 1e-7 * np.abs(ref["flux"]).max()
+    plan.close()

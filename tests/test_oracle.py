@@ -270,6 +270,141 @@ def test_pcg_edge_cases():
     assert ol.pcg(rows, cols, vals, diag, np.ones(n), float("inf"), 10)[1] == 2
 
 
+# ------------------------------------------------- block-Jacobi preconditioner (a11 option)
+
+def _weighted_grid(m, seed):
+    """Symmetric, diagonally dominant 5-point matrix with random edge weights (CSR)."""
+    rng = np.random.default_rng(seed)
+    n = m * m
+    w_right = rng.uniform(0.5, 1.5, n)
+    w_up = rng.uniform(0.5, 1.5, n)
+    dense_rows = [dict() for _ in range(n)]
+    for i in range(m):
+        for j in range(m):
+            r = i * m + j
+            if j < m - 1:
+                dense_rows[r][r + 1] = dense_rows[r + 1][r] = -w_right[r]
+            if i < m - 1:
+                dense_rows[r][r + m] = dense_rows[r + m][r] = -w_up[r]
+    rows, cols, vals, diag = [0], [], [], np.zeros(n)
+    for r in range(n):
+        d = 0.1 + sum(-v for v in dense_rows[r].values())
+        dense_rows[r][r] = d
+        diag[r] = d
+        for c in sorted(dense_rows[r]):
+            cols.append(c)
+            vals.append(dense_rows[r][c])
+        rows.append(len(cols))
+    return np.array(rows, np.int32), np.array(cols, np.int32), np.array(vals), diag
+
+
+def _dense(rows, cols, vals, n):
+    a = np.zeros((n, n))
+    for r in range(n):
+        a[r, cols[rows[r]:rows[r + 1]]] = vals[rows[r]:rows[r + 1]]
+    return a
+
+
+@pytest.mark.parametrize("bs", [2, 4, 8])
+def test_block_jacobi_inverse_equals_numpy_block_inverse(bs):
+    """block_jacobi_preconditioner.cu:73-168: every principal block of bs consecutive rows
+    (the last one partial: n = 21^2 = 441 is not a multiple of bs) is inverted; numpy's
+    LU inverse of the same dense block is the independent check."""
+    m = 21
+    n = m * m
+    rows, cols, vals, diag = _weighted_grid(m, seed=bs)
+    planes, fallback = ol.block_jacobi(rows, cols, vals, diag, bs)
+    assert not fallback.any()
+    a = _dense(rows, cols, vals, n)
+    for first in range(0, n, bs):
+        w = min(bs, n - first)
+        inv = np.linalg.inv(a[first:first + w, first:first + w])
+        for j in range(w):
+            for k in range(w):
+                got = planes[j, first + k]
+                assert abs(got - inv[k, j]) <= 1e-12 * np.abs(inv).max(), (first, j, k)
+        if w < bs:
+            assert not planes[w:, first:first + w].any()
+
+
+def test_block_jacobi_falls_back_on_unusable_blocks():
+    """A block that is not symmetric to 1e-12, or not positive definite, is replaced by the
+    point Jacobi rows (pcg_device.cuh:56-72 with the NaN marker of :159-166)."""
+    m = 12
+    n = m * m
+    rows, cols, vals, diag = _weighted_grid(m, seed=3)
+    vals = vals.copy()
+
+    def slot(r, c):
+        k = np.flatnonzero(cols[rows[r]:rows[r + 1]] == c)
+        return rows[r] + int(k[0])
+
+    vals[slot(4, 5)] *= 1.0 + 1e-9                 # block 1 (rows 4..7): asymmetric
+    vals[slot(9, 8)] = vals[slot(8, 9)] = -5.0 * diag[8]   # block 2 (rows 8..11): indefinite
+    planes, fallback = ol.block_jacobi(rows, cols, vals, diag, 4)
+    assert fallback[1] == 1 and fallback[2] == 1
+    assert fallback.sum() == 2
+    assert not planes[:, 4:12].any()
+    # PCG still converges; the fallback rows precondition by 1/diag.
+    b = np.linspace(1.0, 2.0, n)
+    x, st, it, res = ol.pcg_block(rows, cols, vals, diag, b, 1e-10, 2000, 4)
+    assert st in (0, 10)   # the perturbed operator may break down, never a crash
+
+
+@pytest.mark.parametrize("bs", [4, 8])
+def test_block_pcg_is_exact_on_block_diagonal_operator(bs):
+    """With 4x4 SPD diagonal blocks aligned to the block rows, block-Jacobi (block size 4
+    or 8) is the exact inverse: PCG accepts after one iteration with x = A^-1 b."""
+    rng = np.random.default_rng(5)
+    n = 4 * 30
+    rows, cols, vals = [0], [], []
+    a = np.zeros((n, n))
+    for first in range(0, n, 4):
+        q = rng.normal(size=(4, 4))
+        blk = q @ q.T + 4.0 * np.eye(4)
+        a[first:first + 4, first:first + 4] = blk
+    for r in range(n):
+        nz = np.flatnonzero(a[r])
+        cols.extend(nz.tolist())
+        vals.extend(a[r, nz].tolist())
+        rows.append(len(cols))
+    rows, cols, vals = np.array(rows, np.int32), np.array(cols, np.int32), np.array(vals)
+    diag = np.diag(a).copy()
+    b = rng.normal(size=n)
+    x, st, it, res = ol.pcg_block(rows, cols, vals, diag, b, 1e-12, 100, bs)
+    assert st == 0 and it == 1
+    assert np.abs(x - np.linalg.solve(a, b)).max() <= 1e-12 * np.abs(x).max()
+    assert ol.pcg_block(rows, cols, vals, diag, b, 1e-12, 100, 1)[2] > 1
+
+
+def test_block_size_one_is_point_jacobi_and_blocks_cut_iterations():
+    rows, cols, vals, diag = _weighted_grid(40, seed=9)
+    n = 1600
+    b = np.cos(np.arange(n) * 0.11)
+    x1, st1, it1, r1 = ol.pcg(rows, cols, vals, diag, b, 1e-10, 4096)
+    xb, stb, itb, rb = ol.pcg_block(rows, cols, vals, diag, b, 1e-10, 4096, 1)
+    assert st1 == stb == 0 and it1 == itb and np.array_equal(x1, xb) and r1 == rb
+    for bs in (4, 8):
+        x, st, it, res = ol.pcg_block(rows, cols, vals, diag, b, 1e-10, 4096, bs)
+        assert st == 0 and it < it1
+        assert np.linalg.norm(b - _spmv(rows, cols, vals, x)) <= 1.01e-10 * np.linalg.norm(b)
+
+
+def test_block_jacobi_newton_reaches_the_frozen_femm_answer():
+    """The nonlinear pm-coil fixture with block-Jacobi PCG (block size 4 and the reference's
+    8) lands on the same frozen FEMM state (|dA| <= 1e-9) as with point Jacobi."""
+    from femm_gpu_amd.model import plain_options
+    model, a_exp, current, _ = ff.nonlinear_pm_coil()
+    o = plain_options(relative_tolerance=1e-8, max_newton_iterations=128)
+    point = ol.newton_solve(model, [current], o)
+    for bs in (4, 8):
+        with ol.block_size(bs):
+            out = ol.newton_solve(model, [current], o)
+        assert out["status"] == 0
+        assert np.abs(out["a"] - a_exp).max() <= 1e-9
+        assert out["linear"] < point["linear"]
+
+
 # ---------------------------------------------------------------- signed reduction (a5, a6)
 
 def _ap_model():

@@ -129,3 +129,26 @@ def test_compute_without_gpu_reports_gpu_unavailable():
         plan.solve_batch(np.array([[2.0]]))
     assert err.value.code == 6
     plan.close()
+
+
+def test_group_entry_points_validate_without_gpu():
+    """femm_group_create checks its entries on the host; solving needs the GPU."""
+    import torch
+    from femm_gpu_amd.solver import Group
+
+    model, _ = meshes.square_coil_ring(cells=4, ring_bh=True)
+    plan = Plan(model)
+    spec = dict(plan=plan, items=1, currents=8, a=8, flux=8, results=8)
+    with pytest.raises(FemmError) as err:
+        Group([spec, dict(spec)])            # the same plan twice
+    assert err.value.code == 2
+    with pytest.raises(FemmError) as err:
+        Group([dict(spec, partitions=0)])
+    assert err.value.code == 2
+    group = Group([spec])
+    if not torch.cuda.is_available():
+        with pytest.raises(FemmError) as err:
+            group.solve(0)
+        assert err.value.code == 6           # GPU_UNAVAILABLE
+    group.close()
+    plan.close()
