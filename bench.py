@@ -51,11 +51,14 @@ def measured_traffic(kernel, workload):
     return rec.get("hbm_bytes_per_launch")
 
 
-def algorithmic_bytes(plan, results, B, mixed):
+def algorithmic_bytes(plan, results, B, mixed, block_size=1):
     """Algorithmic HBM bytes of one batched solve from the per-item kernel counters.
 
     PCG iteration (per item): s_v nnz + (4 nnz + 4 (n + 1)) / S + 13 s_x n with S = B (the
     CSR pattern is shared by every item); s_v = s_x = 8 (FP64) or 4 (FP32 iterations).
+    With block-Jacobi of b rows the vector passes are 12 + b (the b planes replace D^-1
+    and z is stored), and each Newton iteration also rebuilds the planes (8 nnz + 8 n read,
+    (8 + s_x) b n written).
     FP64 residual replacement (mixed form): 8 nnz + (4 nnz + 4 (n + 1)) / S + 76 n.
     Newton iteration (per item): element pass (read A: 8 N, write 12 doubles per
     triangle), slot and row gathers (read 12 doubles per triangle, write 8 nnz + 16 n),
@@ -63,10 +66,13 @@ def algorithmic_bytes(plan, results, B, mixed):
     N, T, n, nnz = plan.N, plan.T, plan.n, plan.nnz
     s = 4 if mixed else 8
     shared = (4 * nnz + 4 * (n + 1)) / B
-    pcg_iter = s * nnz + shared + 13 * s * n
+    passes = 13 if block_size <= 1 else 12 + block_size
+    pcg_iter = s * nnz + shared + passes * s * n
     replacement = 8 * nnz + shared + 76 * n
     assembly = 8 * N + 96 * T + 96 * T + 8 * nnz + 16 * n
     newton = assembly + 24 * N + 8 * nnz + 32 * n
+    if block_size > 1:
+        newton += 8 * nnz + 8 * n + (8 + (4 if mixed else 0)) * block_size * n
     total = 0.0
     for r in results:
         total += (r["linear_iterations"] * pcg_iter + r["newton_iterations"] * newton
@@ -124,6 +130,9 @@ def main():
     ap.add_argument("--angular-cells", type=int, default=480)
     ap.add_argument("--forms", default="plain_fp64,learned_mixed",
                     help="comma-separated subset of plain_fp64,learned_mixed,mixed_plain_start")
+    ap.add_argument("--block-size", type=int, default=1,
+                    help="PCG preconditioner of the mixed forms: 1 = point Jacobi, 2/4/8 = "
+                         "block-Jacobi")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first, as the known "
                          "reference dispatch for FETCH_SIZE / WRITE_SIZE scaling")
@@ -167,11 +176,11 @@ def main():
     flux = torch.zeros((B, plan.C), dtype=torch.float64, device="cuda")
     res = torch.zeros(B * RESULT_BYTES, dtype=torch.uint8, device="cuda")
 
-    def collect(ms, wall, mixed):
+    def collect(ms, wall, mixed, block_size=1):
         r = np.frombuffer(res.cpu().numpy().tobytes(), dtype=RESULT_DTYPE).copy()
         passing = (r["status"] == 0) & (r["relative_step"] < 1e-6) & \
                   (r["equation_residual"] <= 1e-8)
-        bytes_total, pcg_iter_bytes = algorithmic_bytes(plan, r, B, mixed)
+        bytes_total, pcg_iter_bytes = algorithmic_bytes(plan, r, B, mixed, block_size)
         newton = r["newton_iterations"]
         hist = {int(q): int(v) for q, v in zip(*np.unique(newton, return_counts=True))}
         return dict(ms=ms, wall=wall, passing=int(passing.sum()), newton_hist=hist,
@@ -180,6 +189,22 @@ def main():
                     pcg_mean=float(r["linear_iterations"].mean()),
                     achieved_gbs=bytes_total / (ms * 1e-3) / 1e9, pcg_iter_bytes=pcg_iter_bytes,
                     a=a.clone(), flux=flux.clone())
+
+    def spmv_rate(fp32, reps=40):
+        """The PCG's SpMV alone on the operators the last solve left (reps products per item):
+        algorithmic bytes s_v nnz + (4 nnz + 4 n) / B + 2 s_x n per product."""
+        dt = torch.float32 if fp32 else torch.float64
+        x = torch.ones((B, plan.n), dtype=dt, device="cuda")
+        y = torch.empty_like(x)
+        prec = 1 if fp32 else 0
+        plan.spmv_batch_device(B, prec, 2, x.data_ptr(), y.data_ptr(), stream)
+        ms, _ = timer.run(lambda: plan.spmv_batch_device(B, prec, reps, x.data_ptr(),
+                                                         y.data_ptr(), stream), 1, 0)
+        s = 4 if fp32 else 8
+        per = s * plan.nnz + (4 * plan.nnz + 4 * plan.n) / B + 2 * s * plan.n
+        gbs = per * B * reps / (ms * 1e-3) / 1e9
+        return dict(gb_s=gbs, frac=gbs / HBM_PEAK_GBS, us_per_product=ms * 1e3 / reps,
+                    bytes_per_item_product=per, items=B, reps=reps)
 
     def report(name, f):
         print(f"# {name}: {f['ms']:.1f} ms/step, pass {f['passing']}/{B}, newton "
@@ -196,12 +221,14 @@ def main():
             options, cfg), args.steps, args.warmup)
         forms["plain_fp64"] = collect(ms, wall, False)
         report("plain_fp64", forms["plain_fp64"])
+        forms["plain_fp64"]["spmv"] = spmv_rate(False)
+        print(f"# spmv fp64: {forms['plain_fp64']['spmv']}", file=sys.stderr, flush=True)
     if "mixed_plain_start" in wanted:
-        cfg = mixed_config()
+        cfg = mixed_config(block_size=args.block_size)
         ms, wall = timer.run(lambda: plan.solve_batch_device(
             currents_dev.data_ptr(), B, a.data_ptr(), flux.data_ptr(), res.data_ptr(), stream,
             options, cfg), args.steps, args.warmup)
-        forms["mixed_plain_start"] = collect(ms, wall, True)
+        forms["mixed_plain_start"] = collect(ms, wall, True, args.block_size)
         report("mixed_plain_start", forms["mixed_plain_start"])
     if "learned_mixed" in wanted:
         # Untimed training: solver solutions of a disjoint seeded sample set.
@@ -225,12 +252,14 @@ def main():
               f"POD energy {fit['pod_energy_fraction']:.8f}, loss {fit['first_loss']:.3e}"
               f" -> {fit['final_loss']:.3e}", file=sys.stderr, flush=True)
         del ta, tf, tr, train_cur
-        cfg = mixed_config()
+        cfg = mixed_config(block_size=args.block_size)
         ms, wall = timer.run(lambda: ws.solve_batch_device(
             B, currents_dev.data_ptr(), currents_dev.data_ptr(), a.data_ptr(), flux.data_ptr(),
             res.data_ptr(), stream, options, cfg, WARM_BUDGET), args.steps, args.warmup)
-        forms["learned_mixed"] = collect(ms, wall, True)
+        forms["learned_mixed"] = collect(ms, wall, True, args.block_size)
         report("learned_mixed", forms["learned_mixed"])
+        forms["learned_mixed"]["spmv"] = spmv_rate(True)
+        print(f"# spmv fp32: {forms['learned_mixed']['spmv']}", file=sys.stderr, flush=True)
 
     if "plain_fp64" not in forms or "learned_mixed" not in forms:
         print(json.dumps({"profiling_run": True, "forms": {
@@ -284,8 +313,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic("learned_mixed", workload),
-                     "kernel": "SolveKernel<true, 2> (learned start + mixed form, one launch)",
-                     "pcg_iteration_bytes_per_item": chosen["pcg_iter_bytes"]},
+                     "kernel": "SolveKernel<1, 2> (learned start + mixed form, one launch)",
+                     "pcg_iteration_bytes_per_item": chosen["pcg_iter_bytes"],
+                     "spmv_alone_fp32": chosen["spmv"],
+                     "spmv_alone_fp64": plain["spmv"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

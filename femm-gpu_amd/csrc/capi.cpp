@@ -408,6 +408,18 @@ int femm_pcg_batch_block(femm_plan *plan, int32_t items, const double *values, c
   return FEMM_OK;
 }
 
+int femm_spmv_batch_device(femm_plan *plan, int32_t items, int32_t precision, int32_t reps,
+                           const void *x, void *y, void *stream)
+{
+  if (plan == nullptr || items <= 0 || reps < 0 || x == nullptr || y == nullptr ||
+      (precision != FEMM_PRECISION_FP64 && precision != FEMM_PRECISION_MIXED))
+    return FEMM_INVALID_ARGUMENT;
+  if (plan->dev.base == nullptr || plan->dev.work_base == nullptr)
+    return FEMM_INVALID_ARGUMENT;   // no operator assembled on the device yet
+  return femm::LaunchSpmv(plan->dev, items, precision == FEMM_PRECISION_MIXED, reps, x, y,
+                          static_cast<hipStream_t>(stream));
+}
+
 int femm_newton_batch_device(femm_plan *plan, int32_t items, const double *currents,
                              const femm_newton_options_t *options,
                              const femm_solve_config_t *config, const double *warm_starts,

@@ -32,6 +32,27 @@ __global__ __launch_bounds__(kBlockThreads) void AssembleKernel(DevicePlan P, De
     valid[item] = ok ? 1 : 0;
 }
 
+// `reps` products y = A x per item with the operator values in the workspace (the last
+// assembled Newton operator), the same pipelined SELL-64 SpMV as the PCG: the product on
+// its own, for measuring it against HBM bandwidth.
+template <typename T>
+__global__ __launch_bounds__(kBlockThreads) void SpmvKernel(DevicePlan P, DeviceWork W, int reps,
+                                                            const T *x, T *y)
+{
+  const int item = blockIdx.x;
+  const T *vals = nullptr;
+  if constexpr (sizeof(T) == 8)
+    vals = W.vals + static_cast<size_t>(item) * P.sell_size;
+  else
+    vals = W.vals32 + static_cast<size_t>(item) * P.sell_size;
+  const size_t n = P.n;
+  for (int rep = 0; rep < reps; ++rep) {
+    Spmv<T, T, true>(P, WholeItem(P), vals, x + item * n, y + item * n,
+                     static_cast<const T *>(nullptr));
+    __syncthreads();
+  }
+}
+
 __global__ __launch_bounds__(kBlockThreads) void PcgKernel(DevicePlan P, DeviceWork W, double tol,
                                                            int maxit, double *info)
 {
@@ -709,6 +730,20 @@ int LaunchAssemble(const DeviceBuffers &dev, int32_t items, const double *states
     return FEMM_INVALID_ARGUMENT;
   hipLaunchKernelGGL(AssembleKernel, dim3(items), dim3(kBlockThreads), 0, stream, dev.plan,
                      dev.work, states, currents, newton, valid);
+  return hipGetLastError() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
+}
+
+int LaunchSpmv(const DeviceBuffers &dev, int32_t items, bool fp32, int32_t reps, const void *x,
+               void *y, hipStream_t stream)
+{
+  if (items <= 0 || items > dev.work.capacity || reps < 0 || (fp32 && dev.work.vals32 == nullptr))
+    return FEMM_INVALID_ARGUMENT;
+  if (fp32)
+    hipLaunchKernelGGL(SpmvKernel<float>, dim3(items), dim3(kBlockThreads), 0, stream, dev.plan,
+                       dev.work, reps, static_cast<const float *>(x), static_cast<float *>(y));
+  else
+    hipLaunchKernelGGL(SpmvKernel<double>, dim3(items), dim3(kBlockThreads), 0, stream, dev.plan,
+                       dev.work, reps, static_cast<const double *>(x), static_cast<double *>(y));
   return hipGetLastError() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
 }
 

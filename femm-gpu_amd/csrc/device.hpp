@@ -173,6 +173,10 @@ int LaunchSafeguarded(const DeviceBuffers &dev, int32_t items, const double *cur
 // Assembly only (values in SELL layout into work.vals, rhs, inv) for `items`.
 int LaunchAssemble(const DeviceBuffers &dev, int32_t items, const double *states,
                    const double *currents, int32_t newton, int32_t *valid, hipStream_t stream);
+// reps products y = A x per item on the workspace operator values (FP64 or the FP32 copy);
+// x, y [items][n] of that type.
+int LaunchSpmv(const DeviceBuffers &dev, int32_t items, bool fp32, int32_t reps, const void *x,
+               void *y, hipStream_t stream);
 // PCG only on work.vals / work.rhs (x0 = 0); info[items][3].
 int LaunchPcg(const DeviceBuffers &dev, int32_t items, double tol, int32_t maxit, double *info,
               hipStream_t stream, int32_t block_size = 1);

@@ -211,6 +211,14 @@ class Plan:
             ctypes.c_void_p(a_ptr), ctypes.c_void_p(flux_ptr), ctypes.c_void_p(results_ptr),
             ctypes.c_void_p(stream_ptr)), "femm_newton_batch_device")
 
+    def spmv_batch_device(self, items, precision, reps, x_ptr, y_ptr, stream_ptr):
+        """reps products per item on the workspace operator (femm_spmv_batch_device)."""
+        _check(self._lib.femm_spmv_batch_device(self._handle, int(items), int(precision),
+                                                int(reps), ctypes.c_void_p(x_ptr),
+                                                ctypes.c_void_p(y_ptr),
+                                                ctypes.c_void_p(stream_ptr)),
+               "femm_spmv_batch_device")
+
     def autotune(self, items, config: SolveConfig = None):
         """Time partition counts for batches of `items` items (femm_plan_autotune); later
         solves with config.partitions = 0 use the fastest.  Returns it."""

@@ -117,6 +117,14 @@ int femm_newton_batch_device(femm_plan *plan, int32_t items, const double *curre
                              const femm_solve_config_t *config, const double *warm_starts,
                              double *a, double *flux, femm_item_result_t *results, void *stream);
 
+/* Measurement entry (no counterpart in the reference): `reps` products y = A x per item with
+ * the operator values the last solve or assembly on this plan left in its workspace, by the
+ * PCG's own SELL-64 SpMV, enqueued on `stream`.  precision FEMM_PRECISION_FP64: x, y are
+ * double [items][free]; FEMM_PRECISION_MIXED: float [items][free] on the FP32 value copy
+ * (present after a mixed-precision solve). */
+int femm_spmv_batch_device(femm_plan *plan, int32_t items, int32_t precision, int32_t reps,
+                           const void *x, void *y, void *stream);
+
 /* Autotuner: times a short probe solve (two Newton iterations of 200 PCG iterations each)
  * of `items` items for partition counts 1, 2, 4, ... that fit the device, and records the
  * fastest in the plan; later solves of `items` items with config->partitions = 0 use it.
