@@ -333,7 +333,8 @@ int femm_assemble_batch(femm_plan *plan, int32_t items, const double *states,
                                  nullptr)) != FEMM_OK)
     return st;
   std::vector<double> sell(static_cast<size_t>(items) * h.sell_size);
-  std::vector<double> r(static_cast<size_t>(items) * h.n);
+  const size_t nvec = static_cast<size_t>(femm::VectorStride(h.n));   // workspace vector stride
+  std::vector<double> r(static_cast<size_t>(items) * nvec);
   std::vector<int32_t> valid(items);
   if (hipDeviceSynchronize() != hipSuccess ||
       hipMemcpy(sell.data(), plan->dev.work.vals, sizeof(double) * sell.size(),
@@ -352,8 +353,8 @@ int femm_assemble_batch(femm_plan *plan, int32_t items, const double *states,
       for (int32_t row = 0; row < h.n; ++row)
         diagonal[static_cast<size_t>(item) * h.n + row] = src[h.diag_sell[row]];
     if (rhs)
-      std::copy(r.begin() + static_cast<size_t>(item) * h.n,
-                r.begin() + static_cast<size_t>(item + 1) * h.n,
+      std::copy(r.begin() + static_cast<size_t>(item) * nvec,
+                r.begin() + static_cast<size_t>(item) * nvec + h.n,
                 rhs + static_cast<size_t>(item) * h.n);
   }
   for (int32_t v : valid)
@@ -391,17 +392,18 @@ int femm_pcg_batch_block(femm_plan *plan, int32_t items, const double *values, c
   if ((st = EnsureIo(plan, ni)) != FEMM_OK)
     return st;
   double *d_info = static_cast<double *>(plan->io);
+  const size_t pitch = sizeof(double) * femm::VectorStride(h.n);   // workspace vector stride
   if (hipMemcpy(plan->dev.work.vals, sell.data(), sizeof(double) * sell.size(),
                 hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(plan->dev.work.rhs, rhs, sizeof(double) * items * static_cast<size_t>(h.n),
-                hipMemcpyHostToDevice) != hipSuccess)
+      hipMemcpy2D(plan->dev.work.rhs, pitch, rhs, sizeof(double) * h.n, sizeof(double) * h.n,
+                  items, hipMemcpyHostToDevice) != hipSuccess)
     return FEMM_INTERNAL_ERROR;
   if ((st = femm::LaunchPcg(plan->dev, items, relative_tolerance, max_iterations, d_info,
                             nullptr, block_size)) != FEMM_OK)
     return st;
   if (hipDeviceSynchronize() != hipSuccess ||
-      (x && hipMemcpy(x, plan->dev.work.x, sizeof(double) * items * static_cast<size_t>(h.n),
-                      hipMemcpyDeviceToHost) != hipSuccess) ||
+      (x && hipMemcpy2D(x, sizeof(double) * h.n, plan->dev.work.x, pitch, sizeof(double) * h.n,
+                        items, hipMemcpyDeviceToHost) != hipSuccess) ||
       (info && hipMemcpy(info, d_info, sizeof(double) * 3 * items, hipMemcpyDeviceToHost) !=
                    hipSuccess))
     return FEMM_INTERNAL_ERROR;

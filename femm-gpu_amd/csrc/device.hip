@@ -516,13 +516,14 @@ int EnsureWork(const HostPlan &h, int32_t items, bool mixed, DeviceBuffers *dev,
     dev->work_base = nullptr;
   }
   const size_t ci = static_cast<size_t>(cap);
+  const size_t nv = static_cast<size_t>(VectorStride(h.n));   // per-item vector stride
   size_t bytes = Bytes<double>(ci * h.T * kElemStride) + Bytes<double>(ci * h.sell_size) +
-                 6 * Bytes<double>(ci * h.n) + Bytes<double>(ci * h.N);
+                 6 * Bytes<double>(ci * nv) + Bytes<double>(ci * h.N);
   if (want_mixed)
-    bytes += Bytes<float>(ci * h.sell_size) + 5 * Bytes<float>(ci * h.n);
+    bytes += Bytes<float>(ci * h.sell_size) + 5 * Bytes<float>(ci * nv);
   if (want_blocks)
-    bytes += Bytes<double>(ci * kMaxBlock * h.n) +
-             (want_mixed ? Bytes<float>(ci * kMaxBlock * h.n) : 0);
+    bytes += Bytes<double>(ci * kMaxBlock * nv) +
+             (want_mixed ? Bytes<float>(ci * kMaxBlock * nv) : 0);
   bytes += Bytes<double>(ci * 2 * kMaxPartitions * 4) + Bytes<uint32_t>(ci * 2) +
            Bytes<int32_t>(ci);
   void *base = nullptr;
@@ -539,25 +540,25 @@ int EnsureWork(const HostPlan &h, int32_t items, bool mixed, DeviceBuffers *dev,
   w.capacity = cap;
   w.elem = Carve<double>(cur, ci * h.T * kElemStride);
   w.vals = Carve<double>(cur, ci * h.sell_size);
-  w.rhs = Carve<double>(cur, ci * h.n);
-  w.inv = Carve<double>(cur, ci * h.n);
-  w.x = Carve<double>(cur, ci * h.n);
-  w.r = Carve<double>(cur, ci * h.n);
-  w.p = Carve<double>(cur, ci * h.n);
-  w.ap = Carve<double>(cur, ci * h.n);
+  w.rhs = Carve<double>(cur, ci * nv);
+  w.inv = Carve<double>(cur, ci * nv);
+  w.x = Carve<double>(cur, ci * nv);
+  w.r = Carve<double>(cur, ci * nv);
+  w.p = Carve<double>(cur, ci * nv);
+  w.ap = Carve<double>(cur, ci * nv);
   w.trial = Carve<double>(cur, ci * h.N);
   if (want_mixed) {
     w.vals32 = Carve<float>(cur, ci * h.sell_size);
-    w.r32 = Carve<float>(cur, ci * h.n);
-    w.p32 = Carve<float>(cur, ci * h.n);
-    w.ap32 = Carve<float>(cur, ci * h.n);
-    w.d32 = Carve<float>(cur, ci * h.n);
-    w.inv32 = Carve<float>(cur, ci * h.n);
+    w.r32 = Carve<float>(cur, ci * nv);
+    w.p32 = Carve<float>(cur, ci * nv);
+    w.ap32 = Carve<float>(cur, ci * nv);
+    w.d32 = Carve<float>(cur, ci * nv);
+    w.inv32 = Carve<float>(cur, ci * nv);
   }
   if (want_blocks) {
-    w.bjac = Carve<double>(cur, ci * kMaxBlock * h.n);
+    w.bjac = Carve<double>(cur, ci * kMaxBlock * nv);
     if (want_mixed)
-      w.bjac32 = Carve<float>(cur, ci * kMaxBlock * h.n);
+      w.bjac32 = Carve<float>(cur, ci * kMaxBlock * nv);
   }
   w.team_partials = Carve<double>(cur, ci * 2 * kMaxPartitions * 4);
   w.team_barrier = Carve<uint32_t>(cur, ci * 2);

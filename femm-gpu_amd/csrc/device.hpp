@@ -56,6 +56,10 @@ struct DeviceWork {
 constexpr int kMaxPartitions = 256;   // workgroups per item in a partitioned solve
 constexpr int kMaxBlock = 8;          // largest block-Jacobi block (rows)
 
+// Per-item stride of the workspace's free-row vectors: n rounded up to 64 rows, so vector
+// passes may load 16-byte chunks (the rows past n stay zero).
+__host__ __device__ constexpr int VectorStride(int n) { return (n + 63) / 64 * 64; }
+
 // Per-item material variants on the plan's mesh: every array is [count][stride] with the
 // plan's sizes (T triangles, `mats` materials, `knots` B-H knots, n free rows, C circuits).
 struct DeviceVariants {

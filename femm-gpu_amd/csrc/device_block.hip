@@ -26,9 +26,9 @@ __global__ __launch_bounds__(kBlockThreads) void BlockPcgKernel(DevicePlan P, De
   R.Sync();
   BuildBlockJacobi(P, R.S, w.vals, w.inv, block_size, w.bjac, nullptr);
   R.Sync();
-  const BlockPrec<double> B{w.bjac, block_size, P.n};
+  const BlockPrec<double> B{w.bjac, block_size, VectorStride(P.n)};
   const LinearInfo li =
-      Pcg64<true, true>(P, w.vals, w.inv, B, w.rhs, w.x, w.r, w.p, w.ap, tol, maxit, false, R);
+      Pcg64<true, 1>(P, w.vals, w.inv, B, w.rhs, w.x, w.r, w.p, w.ap, tol, maxit, false, R);
   if (threadIdx.x == 0) {
     info[3 * item] = li.status;
     info[3 * item + 1] = li.iterations;

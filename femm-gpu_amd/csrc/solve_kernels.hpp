@@ -551,9 +551,46 @@ __device__ double Spmv(const DevicePlan &P, const Span &S, const T *__restrict__
   return local;
 }
 
-// Vector passes keep kVecUnroll independent rows in flight per thread (all
-// loads of a step are issued before any arithmetic).
-constexpr int kVecUnroll = 4;
+// The hot PCG vector passes move 16 bytes per lane per load (two doubles or four floats,
+// consecutive rows): the per-CU rate of vector memory instructions, not HBM, bounds
+// one-workgroup-per-item passes with 4-byte loads.  Per-item vectors are padded to
+// VectorStride rows (zero past n), so every 16-byte chunk is aligned and chunks past
+// the last row only ever hold zeros.
+constexpr int kChunkUnroll = 2;   // chunks per thread in flight per step
+
+template <typename T>
+struct Chunk {
+  static constexpr int N = 16 / sizeof(T);
+  T v[N];
+};
+
+template <typename T>
+__device__ inline Chunk<T> LoadChunk(const T *p)
+{
+  Chunk<T> c;
+  if constexpr (sizeof(T) == 8) {
+    const double2 t = *reinterpret_cast<const double2 *>(p);
+    c.v[0] = t.x;
+    c.v[1] = t.y;
+  } else {
+    const float4 t = *reinterpret_cast<const float4 *>(p);
+    c.v[0] = t.x;
+    c.v[1] = t.y;
+    c.v[2] = t.z;
+    c.v[3] = t.w;
+  }
+  return c;
+}
+
+template <typename T>
+__device__ inline void StoreChunk(T *p, const Chunk<T> &c)
+{
+  if constexpr (sizeof(T) == 8) {
+    *reinterpret_cast<double2 *>(p) = make_double2(c.v[0], c.v[1]);
+  } else {
+    *reinterpret_cast<float4 *>(p) = make_float4(c.v[0], c.v[1], c.v[2], c.v[3]);
+  }
+}
 
 // ----------------------------------------------------------------------------- block-Jacobi
 
@@ -563,9 +600,9 @@ constexpr int kVecUnroll = 4;
 // residual of row j of i's block: z_i = sum_j planes[j][i] r_(first + j), in j order.
 template <typename T>
 struct BlockPrec {
-  const T *planes;   // [kMaxBlock][n] of this item, or nullptr
+  const T *planes;   // [kMaxBlock][VectorStride(n)] of this item, or nullptr
   int bs;            // 1: point Jacobi by the inverse diagonal
-  int n;
+  int n;             // plane stride (VectorStride of the free-row count)
 };
 
 // All LDS accesses of this wave issued so far are complete before any later one
@@ -598,6 +635,67 @@ __device__ inline T BlockApply(const BlockPrec<T> &B, int i, bool valid, T ri)
   return z;
 }
 
+// Blocks of 4 rows held whole by one thread: the thread's 4 consecutive rows arrive as
+// 16-byte chunks (one float4, or two double2), the block's 4 planes likewise (plane j's
+// chunk is column j of the block inverse), so z = M^-1 r needs no exchange between lanes.
+// Rows past the span's last row are zero padding in every vector and plane.
+template <typename T>
+struct Rows4 {
+  T v[4];
+};
+
+template <typename T>
+__device__ inline Rows4<T> LoadRows4(const T *p)
+{
+  Rows4<T> q;
+  if constexpr (sizeof(T) == 4) {
+    const float4 t = *reinterpret_cast<const float4 *>(p);
+    q.v[0] = t.x;
+    q.v[1] = t.y;
+    q.v[2] = t.z;
+    q.v[3] = t.w;
+  } else {
+    const double2 a = *reinterpret_cast<const double2 *>(p);
+    const double2 b = *reinterpret_cast<const double2 *>(p + 2);
+    q.v[0] = a.x;
+    q.v[1] = a.y;
+    q.v[2] = b.x;
+    q.v[3] = b.y;
+  }
+  return q;
+}
+
+template <typename T>
+__device__ inline void StoreRows4(T *p, const Rows4<T> &q)
+{
+  if constexpr (sizeof(T) == 4) {
+    *reinterpret_cast<float4 *>(p) = make_float4(q.v[0], q.v[1], q.v[2], q.v[3]);
+  } else {
+    *reinterpret_cast<double2 *>(p) = make_double2(q.v[0], q.v[1]);
+    *reinterpret_cast<double2 *>(p + 2) = make_double2(q.v[2], q.v[3]);
+  }
+}
+
+// z for one block of 4 rows from its planes: z_k = sum_j Minv[k][j] r_j (j in order).
+template <typename T>
+__device__ inline Rows4<T> Block4Apply(const BlockPrec<T> &B, int row, const Rows4<T> &r)
+{
+  Rows4<T> pl[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    pl[j] = LoadRows4(B.planes + static_cast<size_t>(j) * B.n + row);
+  Rows4<T> z;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    T s = T(0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      s += pl[j].v[k] * r.v[j];
+    z.v[k] = s;
+  }
+  return z;
+}
+
 // BuildBlockInverse (solvers/block_jacobi_preconditioner.cu:73-168) for the rows of span S:
 // each block's principal submatrix, scaled by sqrt(diag) on both sides, must have finite
 // reciprocal diagonals >= DBL_MIN and be symmetric to 1e-12 relative to max(1, |a|, |b|);
@@ -605,7 +703,7 @@ __device__ inline T BlockApply(const BlockPrec<T> &B, int i, bool valid, T ri)
 // inverse by forward / back substitution and unscales it.  An unusable block becomes the
 // point Jacobi rows (the reference's NaN-marked fallback, pcg_device.cuh:56-72).  Lane k
 // writes plane k of its block's rows (column k of the inverse).
-__device__ __noinline__ void BuildBlockJacobi(const DevicePlan &P, const Span &S, const double *vals,
+__device__ void BuildBlockJacobi(const DevicePlan &P, const Span &S, const double *vals,
                                  const double *inv, int bs, double *planes, float *planes32)
 {
   __shared__ double blk[kWaves * (64 * kMaxBlock + 128)];
@@ -615,7 +713,7 @@ __device__ __noinline__ void BuildBlockJacobi(const DevicePlan &P, const Span &S
   double *flag = sc + 64;                      // per block: 1 = unusable
   const int bl = lane / bs, k = lane - bl * bs, lead = bl * bs;
   double *F = F_all + bl * bs * bs;
-  const size_t n = P.n;
+  const size_t n = VectorStride(P.n);   // plane stride
   for (int base = S.row0 + wave * 64; base < S.row1; base += kBlockThreads) {
     const int i = base + lane, first = base + lead;
     const bool valid = i < S.row1;
@@ -729,12 +827,16 @@ struct LinearInfo {
 // the recomputed true residual with restart otherwise.  With warm != 0 the
 // iteration starts from the x already in `x` (unscaled) instead of zero; the
 // acceptance test ||b - A x|| <= tol ||b|| is unchanged.
-template <bool kPipeline = true, bool kBlock = false, class Team = BlockTeam>
+// kBlock: 0 = point Jacobi; 1 = block-Jacobi of B.bs rows (4 in-thread, others by lane
+// exchange); 2 = block-Jacobi of 4 rows only (the lane-exchange code is not compiled).
+template <bool kPipeline = true, int kBlock = 0, class Team = BlockTeam>
 __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const double *inv,
                             const BlockPrec<double> &B, const double *rhs, double *x, double *r,
                             double *p, double *ap, double tol, int maxit, bool warm, Team &R)
 {
-  constexpr bool block = kBlock;
+  constexpr bool block = kBlock != 0;
+  constexpr bool only4 = kBlock == 2;
+  const bool fast4 = only4 || B.bs == 4;
   const int lane = threadIdx.x & 63;
   const int wave_row0 = R.S.row0 + (threadIdx.x & ~63);   // block mode: wave-uniform row loops
   LinearInfo info{FEMM_INTERNAL_ERROR, 0, INFINITY};
@@ -776,7 +878,30 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
     R.Sync();
   }
   double s0[3] = {0.0, 0.0, 0.0};
-  if constexpr (block) {
+  if (block && fast4) {
+    const int q1 = (R.S.row1 + 3) / 4;
+    for (int q = R.S.row0 / 4 + threadIdx.x; q < q1; q += kBlockThreads) {
+      const int row = 4 * q;
+      const Rows4<double> bv = LoadRows4(rhs + row);
+      const Rows4<double> av = warm ? LoadRows4(ap + row) : Rows4<double>{};
+      Rows4<double> rv;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const double b = bv.v[e] / scale;
+        rv.v[e] = warm ? b - av.v[e] : b;
+        s0[0] += rv.v[e] * rv.v[e];
+        s0[2] += b * b;
+      }
+      const Rows4<double> zv = Block4Apply(B, row, rv);
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        s0[1] += rv.v[e] * zv.v[e];
+      if (!warm)
+        StoreRows4(x + row, Rows4<double>{});
+      StoreRows4(r + row, rv);
+      StoreRows4(p + row, zv);
+    }
+  } else if constexpr (block && !only4) {
     for (int base = wave_row0; base < R.S.row1; base += kBlockThreads) {
       const int i = base + lane;
       const bool valid = i < R.S.row1;
@@ -842,45 +967,76 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
     double s1[2] = {0.0, 0.0};
     if constexpr (block) {
       // x, r updated; z = M^-1 r written over Ap (dead until the next product).
-      for (int base = wave_row0; base < R.S.row1; base += kBlockThreads) {
-        const int i = base + lane;
-        const bool valid = i < R.S.row1;
-        double ri = 0.0;
-        if (valid) {
-          x[i] = x[i] + alpha * p[i];
-          ri = r[i] - alpha * ap[i];
-          r[i] = ri;
-          s1[0] += ri * ri;
+      if (fast4) {
+        const int q1 = (R.S.row1 + 3) / 4;
+        for (int q = R.S.row0 / 4 + threadIdx.x; q < q1; q += kBlockThreads) {
+          const int row = 4 * q;
+          Rows4<double> xv = LoadRows4(x + row), pv = LoadRows4(p + row),
+                        rv = LoadRows4(r + row), av = LoadRows4(ap + row);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            xv.v[e] = xv.v[e] + alpha * pv.v[e];
+            rv.v[e] = rv.v[e] - alpha * av.v[e];
+            s1[0] += rv.v[e] * rv.v[e];
+          }
+          const Rows4<double> zv = Block4Apply(B, row, rv);
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            s1[1] += rv.v[e] * zv.v[e];
+          StoreRows4(x + row, xv);
+          StoreRows4(r + row, rv);
+          StoreRows4(ap + row, zv);
         }
-        const double zi = BlockApply(B, i, valid, ri);
-        if (valid) {
-          ap[i] = zi;
-          s1[1] += ri * zi;
+      } else if constexpr (!only4) {
+        for (int base = wave_row0; base < R.S.row1; base += kBlockThreads) {
+          const int i = base + lane;
+          const bool valid = i < R.S.row1;
+          double ri = 0.0;
+          if (valid) {
+            x[i] = x[i] + alpha * p[i];
+            ri = r[i] - alpha * ap[i];
+            r[i] = ri;
+            s1[0] += ri * ri;
+          }
+          const double zi = BlockApply(B, i, valid, ri);
+          if (valid) {
+            ap[i] = zi;
+            s1[1] += ri * zi;
+          }
         }
       }
-    } else
-    for (int i0 = R.S.row0 + threadIdx.x; i0 < R.S.row1; i0 += kBlockThreads * kVecUnroll) {
-      double xv[kVecUnroll], pv[kVecUnroll], rv[kVecUnroll], av[kVecUnroll], iv[kVecUnroll];
+    } else {
+      constexpr int V = Chunk<double>::N;
+      const int c1 = (R.S.row1 + V - 1) / V;
+      for (int c0 = R.S.row0 / V + threadIdx.x; c0 < c1; c0 += kBlockThreads * kChunkUnroll) {
+        Chunk<double> xv[kChunkUnroll], pv[kChunkUnroll], rv[kChunkUnroll], av[kChunkUnroll],
+            iv[kChunkUnroll];
 #pragma unroll
-      for (int u = 0; u < kVecUnroll; ++u) {
-        const int i = i0 + u * kBlockThreads;
-        if (i < R.S.row1) {
-          xv[u] = x[i];
-          pv[u] = p[i];
-          rv[u] = r[i];
-          av[u] = ap[i];
-          iv[u] = inv[i];
+        for (int u = 0; u < kChunkUnroll; ++u) {
+          const int c = c0 + u * kBlockThreads;
+          if (c < c1) {
+            xv[u] = LoadChunk(x + c * V);
+            pv[u] = LoadChunk(p + c * V);
+            rv[u] = LoadChunk(r + c * V);
+            av[u] = LoadChunk(ap + c * V);
+            iv[u] = LoadChunk(inv + c * V);
+          }
         }
-      }
 #pragma unroll
-      for (int u = 0; u < kVecUnroll; ++u) {
-        const int i = i0 + u * kBlockThreads;
-        if (i < R.S.row1) {
-          x[i] = xv[u] + alpha * pv[u];
-          const double ri = rv[u] - alpha * av[u];
-          r[i] = ri;
-          s1[0] += ri * ri;
-          s1[1] += ri * (ri * iv[u]);
+        for (int u = 0; u < kChunkUnroll; ++u) {
+          const int c = c0 + u * kBlockThreads;
+          if (c < c1) {
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+              xv[u].v[e] = xv[u].v[e] + alpha * pv[u].v[e];
+              const double ri = rv[u].v[e] - alpha * av[u].v[e];
+              rv[u].v[e] = ri;
+              s1[0] += ri * ri;
+              s1[1] += ri * (ri * iv[u].v[e]);
+            }
+            StoreChunk(x + c * V, xv[u]);
+            StoreChunk(r + c * V, rv[u]);
+          }
         }
       }
     }
@@ -919,7 +1075,18 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
         return info;
       }
       double rs[1] = {0.0};
-      if constexpr (block) {
+      if (block && fast4) {
+        const int q1 = (R.S.row1 + 3) / 4;
+        for (int q = R.S.row0 / 4 + threadIdx.x; q < q1; q += kBlockThreads) {
+          const int row = 4 * q;
+          const Rows4<double> rv = LoadRows4(r + row);
+          const Rows4<double> zv = Block4Apply(B, row, rv);
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            rs[0] += rv.v[e] * zv.v[e];
+          StoreRows4(p + row, zv);
+        }
+      } else if constexpr (block && !only4) {
         for (int base = wave_row0; base < R.S.row1; base += kBlockThreads) {
           const int i = base + lane;
           const bool valid = i < R.S.row1;
@@ -951,25 +1118,40 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
     }
     const double beta = s1[1] / rho;
     if constexpr (block) {
-      for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
-        p[i] = ap[i] + beta * p[i];
-    } else
-    for (int i0 = R.S.row0 + threadIdx.x; i0 < R.S.row1; i0 += kBlockThreads * kVecUnroll) {
-      double rv[kVecUnroll], iv[kVecUnroll], pv[kVecUnroll];
+      constexpr int V = Chunk<double>::N;
+      const int c1 = (R.S.row1 + V - 1) / V;
+      for (int c = R.S.row0 / V + threadIdx.x; c < c1; c += kBlockThreads) {
+        const Chunk<double> zv = LoadChunk(ap + c * V);
+        Chunk<double> pv = LoadChunk(p + c * V);
 #pragma unroll
-      for (int u = 0; u < kVecUnroll; ++u) {
-        const int i = i0 + u * kBlockThreads;
-        if (i < R.S.row1) {
-          rv[u] = r[i];
-          iv[u] = inv[i];
-          pv[u] = p[i];
-        }
+        for (int e = 0; e < V; ++e)
+          pv.v[e] = zv.v[e] + beta * pv.v[e];
+        StoreChunk(p + c * V, pv);
       }
+    } else {
+      constexpr int V = Chunk<double>::N;
+      const int c1 = (R.S.row1 + V - 1) / V;
+      for (int c0 = R.S.row0 / V + threadIdx.x; c0 < c1; c0 += kBlockThreads * kChunkUnroll) {
+        Chunk<double> rv[kChunkUnroll], iv[kChunkUnroll], pv[kChunkUnroll];
 #pragma unroll
-      for (int u = 0; u < kVecUnroll; ++u) {
-        const int i = i0 + u * kBlockThreads;
-        if (i < R.S.row1)
-          p[i] = rv[u] * iv[u] + beta * pv[u];
+        for (int u = 0; u < kChunkUnroll; ++u) {
+          const int c = c0 + u * kBlockThreads;
+          if (c < c1) {
+            rv[u] = LoadChunk(r + c * V);
+            iv[u] = LoadChunk(inv + c * V);
+            pv[u] = LoadChunk(p + c * V);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kChunkUnroll; ++u) {
+          const int c = c0 + u * kBlockThreads;
+          if (c < c1) {
+#pragma unroll
+            for (int e = 0; e < V; ++e)
+              pv[u].v[e] = rv[u].v[e] * iv[u].v[e] + beta * pv[u].v[e];
+            StoreChunk(p + c * V, pv[u]);
+          }
+        }
       }
     }
     rho = s1[1];
@@ -1043,8 +1225,6 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
 {
   RefinedInfo out{{FEMM_INTERNAL_ERROR, 0, INFINITY}, 0};
   constexpr bool block = kBlock;
-  const int lane = threadIdx.x & 63;
-  const int wave_row0 = R.S.row0 + (threadIdx.x & ~63);   // block mode: wave-uniform row loops
   if (maxit < 0 || !(tol > 0.0) || !(tol < 1.0) || !isfinite(tol)) {
     out.linear.status = FEMM_INVALID_ARGUMENT;
     return out;
@@ -1090,18 +1270,24 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
   {
     double s0[2] = {0.0, 0.0};
     if constexpr (block) {
-      for (int base = wave_row0; base < R.S.row1; base += kBlockThreads) {
-        const int i = base + lane;
-        const bool valid = i < R.S.row1;
-        const float ri = valid ? static_cast<float>(r[i] / sigma) : 0.0f;
-        const float zi = BlockApply(B32, i, valid, ri);
-        if (valid) {
-          r32[i] = ri;
-          p32[i] = zi;
-          d32[i] = 0.0f;
-          s0[0] += static_cast<double>(ri) * ri;
-          s0[1] += static_cast<double>(ri) * zi;
+      const int q1 = (R.S.row1 + 3) / 4;
+      for (int q = R.S.row0 / 4 + threadIdx.x; q < q1; q += kBlockThreads) {
+        const int row = 4 * q;
+        const Rows4<double> r64 = LoadRows4(r + row);
+        Rows4<float> rv, dv;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          rv.v[e] = static_cast<float>(r64.v[e] / sigma);
+          dv.v[e] = 0.0f;
+          s0[0] += static_cast<double>(rv.v[e]) * rv.v[e];
         }
+        const Rows4<float> zv = Block4Apply(B32, row, rv);
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          s0[1] += static_cast<double>(rv.v[e]) * zv.v[e];
+        StoreRows4(r32 + row, rv);
+        StoreRows4(p32 + row, zv);
+        StoreRows4(d32 + row, dv);
       }
     } else {
       for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
@@ -1133,45 +1319,57 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
     double s1[2] = {0.0, 0.0};
     if constexpr (block) {
       // d, r updated; z = M^-1 r written over Ap (dead until the next product).
-      for (int base = wave_row0; base < R.S.row1; base += kBlockThreads) {
-        const int i = base + lane;
-        const bool valid = i < R.S.row1;
-        float ri = 0.0f;
-        if (valid) {
-          d32[i] = d32[i] + alpha * p32[i];
-          ri = r32[i] - alpha * ap32[i];
-          r32[i] = ri;
-          s1[0] += static_cast<double>(ri) * ri;
-        }
-        const float zi = BlockApply(B32, i, valid, ri);
-        if (valid) {
-          ap32[i] = zi;
-          s1[1] += static_cast<double>(ri) * zi;
-        }
-      }
-    } else
-    for (int i0 = R.S.row0 + threadIdx.x; i0 < R.S.row1; i0 += kBlockThreads * kVecUnroll) {
-      float dv[kVecUnroll], pv[kVecUnroll], rv[kVecUnroll], av[kVecUnroll], iv[kVecUnroll];
+      const int q1 = (R.S.row1 + 3) / 4;
+      for (int q = R.S.row0 / 4 + threadIdx.x; q < q1; q += kBlockThreads) {
+        const int row = 4 * q;
+        Rows4<float> dv = LoadRows4(d32 + row), pv = LoadRows4(p32 + row),
+                     rv = LoadRows4(r32 + row), av = LoadRows4(ap32 + row);
 #pragma unroll
-      for (int u = 0; u < kVecUnroll; ++u) {
-        const int i = i0 + u * kBlockThreads;
-        if (i < R.S.row1) {
-          dv[u] = d32[i];
-          pv[u] = p32[i];
-          rv[u] = r32[i];
-          av[u] = ap32[i];
-          iv[u] = inv32[i];
+        for (int e = 0; e < 4; ++e) {
+          dv.v[e] = dv.v[e] + alpha * pv.v[e];
+          rv.v[e] = rv.v[e] - alpha * av.v[e];
+          s1[0] += static_cast<double>(rv.v[e]) * rv.v[e];
         }
-      }
+        const Rows4<float> zv = Block4Apply(B32, row, rv);
 #pragma unroll
-      for (int u = 0; u < kVecUnroll; ++u) {
-        const int i = i0 + u * kBlockThreads;
-        if (i < R.S.row1) {
-          d32[i] = dv[u] + alpha * pv[u];
-          const float ri = rv[u] - alpha * av[u];
-          r32[i] = ri;
-          s1[0] += static_cast<double>(ri) * ri;
-          s1[1] += static_cast<double>(ri) * (ri * iv[u]);
+        for (int e = 0; e < 4; ++e)
+          s1[1] += static_cast<double>(rv.v[e]) * zv.v[e];
+        StoreRows4(d32 + row, dv);
+        StoreRows4(r32 + row, rv);
+        StoreRows4(ap32 + row, zv);
+      }
+    } else {
+      constexpr int V = Chunk<float>::N;
+      const int c1 = (R.S.row1 + V - 1) / V;
+      for (int c0 = R.S.row0 / V + threadIdx.x; c0 < c1; c0 += kBlockThreads * kChunkUnroll) {
+        Chunk<float> dv[kChunkUnroll], pv[kChunkUnroll], rv[kChunkUnroll], av[kChunkUnroll],
+            iv[kChunkUnroll];
+#pragma unroll
+        for (int u = 0; u < kChunkUnroll; ++u) {
+          const int c = c0 + u * kBlockThreads;
+          if (c < c1) {
+            dv[u] = LoadChunk(d32 + c * V);
+            pv[u] = LoadChunk(p32 + c * V);
+            rv[u] = LoadChunk(r32 + c * V);
+            av[u] = LoadChunk(ap32 + c * V);
+            iv[u] = LoadChunk(inv32 + c * V);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kChunkUnroll; ++u) {
+          const int c = c0 + u * kBlockThreads;
+          if (c < c1) {
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+              dv[u].v[e] = dv[u].v[e] + alpha * pv[u].v[e];
+              const float ri = rv[u].v[e] - alpha * av[u].v[e];
+              rv[u].v[e] = ri;
+              s1[0] += static_cast<double>(ri) * ri;
+              s1[1] += static_cast<double>(ri) * (ri * iv[u].v[e]);
+            }
+            StoreChunk(d32 + c * V, dv[u]);
+            StoreChunk(r32 + c * V, rv[u]);
+          }
         }
       }
     }
@@ -1213,21 +1411,24 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
       sigma = rmax;
       double s2[2] = {0.0, 0.0};
       if constexpr (block) {
-        for (int base = wave_row0; base < R.S.row1; base += kBlockThreads) {
-          const int i = base + lane;
-          const bool valid = i < R.S.row1;
-          float ri = 0.0f;
-          if (valid) {
-            ri = static_cast<float>(r[i] / sigma);
-            r32[i] = ri;
-            p32[i] = static_cast<float>(q * static_cast<double>(p32[i]));
-            s2[0] += static_cast<double>(ri) * ri;
+        const int q1 = (R.S.row1 + 3) / 4;
+        for (int c = R.S.row0 / 4 + threadIdx.x; c < q1; c += kBlockThreads) {
+          const int row = 4 * c;
+          const Rows4<double> r64 = LoadRows4(r + row);
+          Rows4<float> rv, pv = LoadRows4(p32 + row);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            rv.v[e] = static_cast<float>(r64.v[e] / sigma);
+            pv.v[e] = static_cast<float>(q * static_cast<double>(pv.v[e]));
+            s2[0] += static_cast<double>(rv.v[e]) * rv.v[e];
           }
-          const float zi = BlockApply(B32, i, valid, ri);
-          if (valid) {
-            ap32[i] = zi;
-            s2[1] += static_cast<double>(ri) * zi;
-          }
+          const Rows4<float> zv = Block4Apply(B32, row, rv);
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            s2[1] += static_cast<double>(rv.v[e]) * zv.v[e];
+          StoreRows4(r32 + row, rv);
+          StoreRows4(p32 + row, pv);
+          StoreRows4(ap32 + row, zv);
         }
       } else {
         for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
@@ -1249,25 +1450,40 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
     }
     const float beta = static_cast<float>(rho_next / rho);
     if constexpr (block) {
-      for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
-        p32[i] = ap32[i] + beta * p32[i];
-    } else
-    for (int i0 = R.S.row0 + threadIdx.x; i0 < R.S.row1; i0 += kBlockThreads * kVecUnroll) {
-      float rv[kVecUnroll], iv[kVecUnroll], pv[kVecUnroll];
+      constexpr int V = Chunk<float>::N;
+      const int c1 = (R.S.row1 + V - 1) / V;
+      for (int c = R.S.row0 / V + threadIdx.x; c < c1; c += kBlockThreads) {
+        const Chunk<float> zv = LoadChunk(ap32 + c * V);
+        Chunk<float> pv = LoadChunk(p32 + c * V);
 #pragma unroll
-      for (int u = 0; u < kVecUnroll; ++u) {
-        const int i = i0 + u * kBlockThreads;
-        if (i < R.S.row1) {
-          rv[u] = r32[i];
-          iv[u] = inv32[i];
-          pv[u] = p32[i];
-        }
+        for (int e = 0; e < V; ++e)
+          pv.v[e] = zv.v[e] + beta * pv.v[e];
+        StoreChunk(p32 + c * V, pv);
       }
+    } else {
+      constexpr int V = Chunk<float>::N;
+      const int c1 = (R.S.row1 + V - 1) / V;
+      for (int c0 = R.S.row0 / V + threadIdx.x; c0 < c1; c0 += kBlockThreads * kChunkUnroll) {
+        Chunk<float> rv[kChunkUnroll], iv[kChunkUnroll], pv[kChunkUnroll];
 #pragma unroll
-      for (int u = 0; u < kVecUnroll; ++u) {
-        const int i = i0 + u * kBlockThreads;
-        if (i < R.S.row1)
-          p32[i] = rv[u] * iv[u] + beta * pv[u];
+        for (int u = 0; u < kChunkUnroll; ++u) {
+          const int c = c0 + u * kBlockThreads;
+          if (c < c1) {
+            rv[u] = LoadChunk(r32 + c * V);
+            iv[u] = LoadChunk(inv32 + c * V);
+            pv[u] = LoadChunk(p32 + c * V);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kChunkUnroll; ++u) {
+          const int c = c0 + u * kBlockThreads;
+          if (c < c1) {
+#pragma unroll
+            for (int e = 0; e < V; ++e)
+              pv[u].v[e] = rv[u].v[e] * iv[u].v[e] + beta * pv[u].v[e];
+            StoreChunk(p32 + c * V, pv[u]);
+          }
+        }
       }
     }
     rho = rho_next;
@@ -1278,8 +1494,10 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
     x[i] += sigma * static_cast<double>(d32[i]);
   R.Sync();
   out.linear.iterations = it;
-  const LinearInfo li = Pcg64<false, kBlock>(P, vals, inv, B64, rhs, x, r, p, ap, tol,
-                                     fallback ? max(0, maxit - it) : 0, true, R);
+  // FP64 finish from the current x (a breakdown or stall of the FP32 iteration, or only the
+  // final true-residual check).  With block-Jacobi the mixed form supports 4-row blocks.
+  const LinearInfo li = Pcg64<false, kBlock ? 2 : 0>(P, vals, inv, B64, rhs, x, r, p, ap, tol,
+                                                     fallback ? max(0, maxit - it) : 0, true, R);
   out.linear.iterations += li.iterations;
   out.linear.residual = li.residual;
   out.linear.status = li.status;
@@ -1301,7 +1519,7 @@ struct ItemPointers {
 __device__ ItemPointers ItemWork(const DevicePlan &P, const DeviceWork &W, int item, bool mixed)
 {
   ItemPointers ip;
-  const size_t n = P.n, s = P.sell_size;
+  const size_t n = VectorStride(P.n), s = P.sell_size;
   ip.elem = W.elem + static_cast<size_t>(item) * P.T * kElemStride;
   ip.vals = W.vals + item * s;
   ip.rhs = W.rhs + item * n;
@@ -1459,9 +1677,9 @@ __device__ LinearInfo LinearSolve(const DevicePlan &P, const ItemPointers &w,
     BuildBlockJacobi(P, R.S, w.vals, w.inv, bs, w.bjac, kMixed ? w.bjac32 : nullptr);
     R.Sync();
   }
-  const BlockPrec<double> B64{w.bjac, bs, P.n};
+  const BlockPrec<double> B64{w.bjac, bs, VectorStride(P.n)};
   if constexpr (kMixed) {
-    const BlockPrec<float> B32{w.bjac32, bs, P.n};
+    const BlockPrec<float> B32{w.bjac32, bs, VectorStride(P.n)};
     const RefinedInfo ri = ReliableSolve<kBlock>(P, w.vals, w.inv, w.vals32, w.inv32, B64, B32, w.rhs, w.x,
                                          w.r, w.p, w.ap, w.d32, w.r32, w.p32, w.ap32, tol, maxit,
                                          cfg.pcg_warm_start != 0, cfg.max_refinements,
@@ -1469,7 +1687,7 @@ __device__ LinearInfo LinearSolve(const DevicePlan &P, const ItemPointers &w,
     li = ri.linear;
     res.refinements += ri.refinements;
   } else {
-    li = Pcg64<true, kBlock>(P, w.vals, w.inv, B64, w.rhs, w.x, w.r, w.p, w.ap, tol, maxit,
+    li = Pcg64<true, kBlock ? 1 : 0>(P, w.vals, w.inv, B64, w.rhs, w.x, w.r, w.p, w.ap, tol, maxit,
                              cfg.pcg_warm_start != 0, R);
   }
   R.Sync();

@@ -44,6 +44,10 @@ def main(argv=None):
     ap.add_argument("--max-cells", type=int, default=512)
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--repeats", type=int, default=2)
+    ap.add_argument("--max-linear", type=int, default=16384,
+                    help="PCG iteration cap per Newton step (16,384: ProductionSampleOptions)")
+    ap.add_argument("--block-size", type=int, default=1,
+                    help="1 = point Jacobi (the reference default), 4 = block-Jacobi")
     ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)
 
@@ -67,10 +71,8 @@ def main(argv=None):
     nodes = [p.N for p in plans]
     _log(f"# {len(plans)} meshes, cells {cells.min()}..{cells.max()}, nodes {min(nodes)}.."
          f"{max(nodes)} (total {sum(nodes)}), setup {time.perf_counter() - t0:.1f} s")
-    # The production cap of 16,384 PCG iterations per Newton step is too small for the
-    # largest meshes' Jacobi PCG (~20k iterations at 390 x 390 cells); both forms get more.
-    opts = accepted_options(max_linear_iterations=200000)
-    cfg = reference_config()
+    opts = accepted_options(max_linear_iterations=args.max_linear)
+    cfg = reference_config(block_size=args.block_size)
     stream = torch.cuda.current_stream()
     bufs = []
     for p, cur in zip(plans, currents):
@@ -100,14 +102,15 @@ def main(argv=None):
 
     # per_mesh
     t = time.perf_counter()
-    parts_single = [p.autotune(1, reference_config(partitions=0)) for p in plans]
+    parts_single = [p.autotune(1, reference_config(partitions=0, block_size=args.block_size))
+                    for p in plans]
     _log(f"# per-mesh autotune {time.perf_counter() - t:.1f} s: partitions {parts_single}")
 
     def per_mesh():
         for p, b, k in zip(plans, bufs, parts_single):
             p.solve_batch_device(b["cur"].data_ptr(), 1, b["a"].data_ptr(), b["flux"].data_ptr(),
                                  b["res"].data_ptr(), stream.cuda_stream, opts,
-                                 reference_config(partitions=k))
+                                 reference_config(partitions=k, block_size=args.block_size))
 
     ms = timed(per_mesh)
     base = [read(b) for b in bufs]
