@@ -153,9 +153,11 @@ int ProbeSolve(femm_plan *plan, int32_t items, const femm_solve_config_t &config
 
 bool ConfigValid(const femm_solve_config_t &c)
 {
-  return (c.precision == FEMM_PRECISION_FP64 || c.precision == FEMM_PRECISION_MIXED) &&
-         (c.block_size == 0 || c.block_size == 1 || c.block_size == 2 || c.block_size == 4 ||
-          c.block_size == 8);
+  // The mixed form's FP32 block-Jacobi keeps a 4-row block in one thread's registers.
+  const bool block_ok = c.block_size == 0 || c.block_size == 1 || c.block_size == 4 ||
+                        (c.precision == FEMM_PRECISION_FP64 &&
+                         (c.block_size == 2 || c.block_size == 8));
+  return (c.precision == FEMM_PRECISION_FP64 || c.precision == FEMM_PRECISION_MIXED) && block_ok;
 }
 
 femm_newton_options_t DefaultNewtonOptions() { return DefaultOptions(); }

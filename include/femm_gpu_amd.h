@@ -43,7 +43,7 @@ typedef struct femm_solve_config {
                                   default); 2, 4 or 8 = block-Jacobi on blocks of that many
                                   consecutive free rows (PcgPreconditioner::kBlockJacobi,
                                   solvers/options.h:8-10, block_jacobi_preconditioner.cu),
-                                  FP32 planes in the mixed form */
+                                  FP32 planes in the mixed form, which takes 4 only */
 } femm_solve_config_t;
 
 /* Per-item outcome (reference SolveInfo plus the Newton counters of

@@ -122,3 +122,14 @@ def test_block_partitioned_matches_oracle(form):
     again = plan.solve_batch(currents, opts, cfg(block_size=4, partitions=8))
     assert np.array_equal(again["a"], out["a"])
     plan.close()
+
+
+def test_mixed_block_size_other_than_four_is_rejected():
+    from femm_gpu_amd.solver import FemmError
+    model, _ = meshes.motor_sector(angular_cells=160)
+    plan = Plan(model)
+    with pytest.raises(FemmError) as err:
+        plan.solve_batch(meshes.motor_currents(1, seed=1), accepted_options(),
+                         mixed_config(block_size=8))
+    assert err.value.code == 2
+    plan.close()
