@@ -467,6 +467,81 @@ __device__ inline void LoadSlice(const DevicePlan &P, const T *__restrict__ vals
   }
 }
 
+// Generic (unpipelined) product of one slice wider than kUnrolledWidth.
+template <typename T, typename V>
+__device__ inline T WideSlice(const DevicePlan &P, const T *__restrict__ vals,
+                              const V *__restrict__ v, int s, int lane)
+{
+  const int base = P.slice_ptr[s] + lane, width = P.slice_width[s];
+  T sum = 0;
+  for (int k = 0; k < width; ++k) {
+    const int pos = base + k * kSliceRows;
+    sum += vals[pos] * static_cast<T>(v[P.sell_cols[pos]]);
+  }
+  return sum;
+}
+
+// Two slices per wave per step (slices s and s + kWaves): twice the gathers in flight per
+// round trip for the FP32 iterations, whose loads take half the registers of FP64 ones.
+template <typename T, typename V>
+__device__ double Spmv2(const DevicePlan &P, const Span &S, const T *__restrict__ vals,
+                        const V *__restrict__ v, V *__restrict__ y, const V *__restrict__ w)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double local = 0.0;
+  const int first = S.slice0 + wave;
+  if (first >= S.slice1)
+    return local;
+  auto clamp = [&](int s) { return s < S.slice1 ? s : first; };
+  SliceLoads<T> c0, c1;
+  LoadSlice(P, vals, first, lane, c0);
+  LoadSlice(P, vals, clamp(first + kWaves), lane, c1);
+  for (int s = first; s < S.slice1; s += 2 * kWaves) {
+    const int s1 = s + kWaves;
+    T sum0 = 0, sum1 = 0;
+    if (c0.width <= kUnrolledWidth && c1.width <= kUnrolledWidth) {
+      V g0[kUnrolledWidth], g1[kUnrolledWidth];
+#pragma unroll
+      for (int k = 0; k < kUnrolledWidth; ++k)
+        g0[k] = v[c0.c[k]];
+#pragma unroll
+      for (int k = 0; k < kUnrolledWidth; ++k)
+        g1[k] = v[c1.c[k]];
+      SliceLoads<T> n0, n1;
+      // Past the end, the first slice is loaded again (in bounds, never used).
+      LoadSlice(P, vals, clamp(s + 2 * kWaves), lane, n0);
+      LoadSlice(P, vals, clamp(s + 3 * kWaves), lane, n1);
+#pragma unroll
+      for (int k = 0; k < kUnrolledWidth; ++k)
+        sum0 += c0.a[k] * static_cast<T>(g0[k]);
+#pragma unroll
+      for (int k = 0; k < kUnrolledWidth; ++k)
+        sum1 += c1.a[k] * static_cast<T>(g1[k]);
+      c0 = n0;
+      c1 = n1;
+    } else {
+      sum0 = WideSlice(P, vals, v, s, lane);
+      if (s1 < S.slice1)
+        sum1 = WideSlice(P, vals, v, s1, lane);
+      LoadSlice(P, vals, clamp(s + 2 * kWaves), lane, c0);
+      LoadSlice(P, vals, clamp(s + 3 * kWaves), lane, c1);
+    }
+    const int row0 = s * kSliceRows + lane;
+    if (row0 < P.n) {
+      y[row0] = static_cast<V>(sum0);
+      if (w)
+        local += static_cast<double>(w[row0]) * static_cast<double>(sum0);
+    }
+    const int row1 = s1 * kSliceRows + lane;
+    if (s1 < S.slice1 && row1 < P.n) {
+      y[row1] = static_cast<V>(sum1);
+      if (w)
+        local += static_cast<double>(w[row1]) * static_cast<double>(sum1);
+    }
+  }
+  return local;
+}
+
 // kPipeline = false: the plain two-round-trip loop, for rarely used products (FP64
 // residuals inside the mixed form, equation residuals) where registers matter more.
 template <typename T, typename V, bool kPipeline = true>
@@ -1309,7 +1384,7 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
   bool fallback = false;
   int it = 0;
   while (it < maxit) {
-    double pap[1] = {Spmv(P, R.S, vals32, p32, ap32, p32)};
+    double pap[1] = {Spmv2(P, R.S, vals32, p32, ap32, p32)};
     R.Sum(pap);
     if (!(pap[0] > 0.0) || !(rho > 0.0) || !isfinite(pap[0]) || !isfinite(rho)) {
       fallback = true;
