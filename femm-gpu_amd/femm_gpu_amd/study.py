@@ -39,6 +39,8 @@ def main(argv=None):
     ap.add_argument("--modes", type=int, default=64)
     ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--warm-budget", type=int, default=16)
+    ap.add_argument("--block-size", type=int, default=4,
+                    help="mixed-form PCG preconditioner: 4 = FP32 block-Jacobi, 1 = point")
     ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)
 
@@ -97,7 +99,8 @@ def main(argv=None):
     a_train, _, res_train, train_ms = run_chunks(
         x_train, lambda idx, cur, B, a, flux, res, lo: plan.solve_batch_variants_device(
             idx.data_ptr(), cur.data_ptr(), B, a.data_ptr(), flux.data_ptr(), res.data_ptr(),
-            stream.cuda_stream, opts, mixed_config()), "train", args.train_chunk)
+            stream.cuda_stream, opts, mixed_config(block_size=args.block_size)), "train",
+        args.train_chunk)
     ok = _passing(res_train)
     feats = torch.from_numpy(np.ascontiguousarray(x_train[ok])).to("cuda")
     states = a_train[torch.from_numpy(np.flatnonzero(ok)).to("cuda")].contiguous()
@@ -120,7 +123,8 @@ def main(argv=None):
     a_learn, flux_learn, res_learn, learn_ms = run_chunks(
         x_test, lambda idx, cur, B, a, flux, res, lo: ws.solve_batch_device(
             B, feats_test[lo:lo + B].data_ptr(), cur.data_ptr(), a.data_ptr(), flux.data_ptr(),
-            res.data_ptr(), stream.cuda_stream, opts, mixed_config(), args.warm_budget,
+            res.data_ptr(), stream.cuda_stream, opts, mixed_config(block_size=args.block_size),
+            args.warm_budget,
             variant_ptr=idx.data_ptr()), "learned", args.chunk)
     da = ((a_learn - a_plain).norm(dim=1) / a_plain.norm(dim=1).clamp_min(1e-300)).cpu().numpy()
     dflux = np.abs(flux_learn - flux_plain).max(1) / np.maximum(np.abs(flux_plain).max(1), 1e-300)
@@ -136,6 +140,7 @@ def main(argv=None):
                           gpu_s=train_ms / 1e3, wall_s=time.perf_counter() - t0 - fit_s,
                           fit_s=fit_s, report=fit),
                test_seed=args.test_seed, warm_budget=args.warm_budget,
+               block_size=args.block_size,
                plain_fp64=summary(res_plain, plain_ms),
                learned_mixed=summary(res_learn, learn_ms),
                max_A_rel_l2=float(da.max()), mean_A_rel_l2=float(da.mean()),
