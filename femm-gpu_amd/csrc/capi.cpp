@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <thread>
 #include <atomic>
 #include <cstdio>
@@ -609,6 +610,89 @@ int femm_newton_batch_variants(femm_plan *plan, int32_t items, const int32_t *va
     return FEMM_INVALID_ARGUMENT;
   return SolveHost(plan, items, variant_of_item, currents, options, config, warm_starts, a,
                    nullptr, nullptr, flux, results);
+}
+
+int femm_tangent_multi_rhs(femm_plan *plan, const double *base_a, const double *base_currents,
+                           int32_t directions, const double *direction_currents,
+                           const femm_newton_options_t *options, double *da, double *dbx,
+                           double *dby, double *dflux, femm_item_result_t *results)
+{
+  if (plan == nullptr || base_a == nullptr || base_currents == nullptr || directions <= 0 ||
+      direction_currents == nullptr || options == nullptr || da == nullptr || dflux == nullptr ||
+      results == nullptr)
+    return FEMM_INVALID_ARGUMENT;
+  const femm::HostPlan &h = plan->host;
+  const size_t D = static_cast<size_t>(directions), C = static_cast<size_t>(h.C);
+  femm_item_result_t invalid{};
+  invalid.status = FEMM_INVALID_ARGUMENT;
+  bool ok = options->linear_relative_tolerance > 0.0 && options->max_linear_iterations >= 0;
+  for (size_t i = 0; i < D * C; ++i)
+    ok = ok && std::isfinite(direction_currents[i]);
+  if (!ok) {
+    // SolveTangentMultiRhs marks every direction invalid (nonlinear_solver.cu:90-104).
+    for (size_t d = 0; d < D; ++d)
+      results[d] = invalid;
+    return FEMM_OK;
+  }
+  int st = EnsurePlanDevice(plan);
+  if (st == FEMM_OK)
+    st = femm::EnsureWork(h, directions, false, &plan->dev);
+  if (st != FEMM_OK)
+    return st;
+  const size_t sa = Round(sizeof(double) * h.N), sc = Round(sizeof(double) * std::max<size_t>(C, 1));
+  const size_t sd = Round(sizeof(double) * D * std::max<size_t>(C, 1));
+  const size_t sda = Round(sizeof(double) * D * h.N), sb = Round(sizeof(double) * D * h.T);
+  const size_t sr = Round(sizeof(femm_item_result_t) * D), sv = Round(sizeof(int32_t));
+  const bool fields = dbx != nullptr && dby != nullptr;
+  if ((st = EnsureIo(plan, sa + sc + sd + sda + (fields ? 2 * sb : 0) + sd + sr + sv)) != FEMM_OK)
+    return st;
+  char *io = static_cast<char *>(plan->io);
+  double *d_a = reinterpret_cast<double *>(io);
+  double *d_cur = reinterpret_cast<double *>(io + sa);
+  double *d_dir = reinterpret_cast<double *>(io + sa + sc);
+  double *d_da = reinterpret_cast<double *>(io + sa + sc + sd);
+  char *rest = io + sa + sc + sd + sda;
+  double *d_bx = fields ? reinterpret_cast<double *>(rest) : nullptr;
+  double *d_by = fields ? reinterpret_cast<double *>(rest + sb) : nullptr;
+  rest += fields ? 2 * sb : 0;
+  double *d_flux = reinterpret_cast<double *>(rest);
+  auto *d_res = reinterpret_cast<femm_item_result_t *>(rest + sd);
+  auto *d_valid = reinterpret_cast<int32_t *>(rest + sd + sr);
+  if (hipMemcpy(d_a, base_a, sizeof(double) * h.N, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d_cur, base_currents, sizeof(double) * C, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d_dir, direction_currents, sizeof(double) * D * C, hipMemcpyHostToDevice) !=
+          hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  // The Newton operator at the base state (AssembleNonlinearNewton with use_newton).
+  if ((st = femm::LaunchAssemble(plan->dev, 1, d_a, d_cur, 1, d_valid, nullptr)) != FEMM_OK)
+    return st;
+  int32_t valid = 0;
+  if (hipMemcpy(&valid, d_valid, sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  if (!valid) {
+    femm_item_result_t failed{};
+    failed.status = FEMM_ASSEMBLY_FAILED;
+    for (size_t d = 0; d < D; ++d)
+      results[d] = failed;
+    return FEMM_OK;
+  }
+  if ((st = femm::LaunchTangent(plan->dev, directions, d_dir, options->linear_relative_tolerance,
+                                options->max_linear_iterations, d_da, d_res, nullptr)) !=
+          FEMM_OK ||
+      (st = femm::LaunchFinalize(plan->dev, directions, d_da, d_bx, d_by, d_flux, nullptr)) !=
+          FEMM_OK)
+    return st;
+  if (hipDeviceSynchronize() != hipSuccess ||
+      hipMemcpy(da, d_da, sizeof(double) * D * h.N, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(dflux, d_flux, sizeof(double) * D * C, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(results, d_res, sizeof(femm_item_result_t) * D, hipMemcpyDeviceToHost) !=
+          hipSuccess ||
+      (fields && (hipMemcpy(dbx, d_bx, sizeof(double) * D * h.T, hipMemcpyDeviceToHost) !=
+                      hipSuccess ||
+                  hipMemcpy(dby, d_by, sizeof(double) * D * h.T, hipMemcpyDeviceToHost) !=
+                      hipSuccess)))
+    return FEMM_INTERNAL_ERROR;
+  return FEMM_OK;
 }
 
 int femm_finalize_batch(femm_plan *plan, int32_t items, const double *states, double *bx,

@@ -150,6 +150,11 @@ int LaunchBlockSolve(int form, int mode, const DeviceBuffers &dev, int32_t items
                      hipStream_t stream, const LaunchExtras &x);
 // The group kernel of a block-Jacobi form (reference driver or safeguarded recovery).
 const void *BlockGroupKernel(int form, bool recovery);
+// Tangent solves (tangent.hip): one PCG per direction on the Newton operator assembled into
+// item 0 (LaunchAssemble at the base state); da[directions][N], results[directions].
+int LaunchTangent(const DeviceBuffers &dev, int32_t directions, const double *direction_currents,
+                  double tol, int32_t maxit, double *da, femm_item_result_t *results,
+                  hipStream_t stream);
 // PCG with block-Jacobi on work.vals / work.rhs (x0 = 0); info[items][3].
 int LaunchBlockPcg(const DeviceBuffers &dev, int32_t items, double tol, int32_t maxit,
                    int32_t block_size, double *info, hipStream_t stream);

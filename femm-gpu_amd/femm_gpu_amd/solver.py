@@ -302,6 +302,29 @@ class Plan:
             ctypes.c_void_p(radial_ptr), ctypes.c_void_p(stream_ptr)),
             "femm_postprocess_batch_device")
 
+    def tangent_multi_rhs(self, base_a, base_currents, directions, options: CNewtonOptions = None,
+                          fields=False):
+        """Shared-operator tangent solves dA/dI along circuit-current directions at a
+        converged state (femm_tangent_multi_rhs; SolveTangentMultiRhs)."""
+        base_a = np.ascontiguousarray(base_a, np.float64).reshape(self.N)
+        base_currents = np.ascontiguousarray(base_currents, np.float64).reshape(self.C)
+        directions = np.ascontiguousarray(directions, np.float64).reshape(-1, self.C)
+        D = directions.shape[0]
+        options = options or production_options()
+        da, dflux = np.zeros((D, self.N)), np.zeros((D, self.C))
+        dbx = np.zeros((D, self.T)) if fields else None
+        dby = np.zeros((D, self.T)) if fields else None
+        results = np.zeros(D, RESULT_DTYPE)
+        _check(self._lib.femm_tangent_multi_rhs(
+            self._handle, _dp(base_a), _dp(base_currents), D, _dp(directions),
+            ctypes.byref(options), _dp(da), None if dbx is None else _dp(dbx),
+            None if dby is None else _dp(dby), _dp(dflux),
+            results.ctypes.data_as(ctypes.c_void_p)), "femm_tangent_multi_rhs")
+        out = dict(da=da, dflux=dflux, results=results)
+        if fields:
+            out.update(dbx=dbx, dby=dby)
+        return out
+
     def finalize_batch(self, states):
         states = np.ascontiguousarray(states, np.float64).reshape(-1, self.N)
         B = states.shape[0]

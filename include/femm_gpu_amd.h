@@ -197,6 +197,23 @@ int femm_newton_batch_variants_device(femm_plan *plan, int32_t items,
                                       const double *warm_starts, double *a, double *flux,
                                       femm_item_result_t *results, void *stream);
 
+/* Shared-operator tangent solves: NonlinearP1Solver::SolveTangentMultiRhs
+ * (solvers/nonlinear_solver.cu:85-172) with GpuCsrSolver::SolveCurrentOperatorBatch
+ * (solvers/csr_solver.cu:296-390).  At a converged state base_a[N] with circuit currents
+ * base_currents[C], the Newton operator J = K + C is assembled once and
+ * J dA_d = sum_c direction_currents[d][c] s_c is solved for each of `directions` right-hand
+ * sides (s_c: circuit c's per-ampere source load), by Jacobi PCG from x0 = 0 with
+ * options->linear_relative_tolerance / max_linear_iterations.  Outputs per direction: the
+ * field derivative da[d][N] (zero on prescribed nodes), its element B dbx/dby[d][T] (may be
+ * NULL) and flux derivative dflux[d][C] (FinalizeResult of the derivative; with a unit
+ * direction, a column of the differential inductance per unit depth), and results[d]
+ * (status, linear_iterations, linear_residual).  A non-finite direction makes every
+ * result INVALID_ARGUMENT, as in the reference. */
+int femm_tangent_multi_rhs(femm_plan *plan, const double *base_a, const double *base_currents,
+                           int32_t directions, const double *direction_currents,
+                           const femm_newton_options_t *options, double *da, double *dbx,
+                           double *dby, double *dflux, femm_item_result_t *results);
+
 /* Element B and circuit flux for given states (FinalizeResult,
  * solvers/nonlinear_solver.cu:16-83) on the device. */
 int femm_finalize_batch(femm_plan *plan, int32_t items, const double *states, double *bx,

@@ -456,6 +456,9 @@ struct Assembly {
   std::vector<int32_t> rows, cols, free_nodes, node_to_free;
   std::vector<double> values, diag, rhs_per_amp, rhs_offset, boundary;
   std::vector<int8_t> node_sign;
+  // Per-ampere coil load of each circuit on every node (source_load_per_circuit,
+  // physics/nonlinear_model.cpp:471-546).
+  std::vector<std::vector<double>> source_per_circuit;
 };
 
 // physics/nonlinear_model.cpp:321-394.  Row sign times column sign; fixed
@@ -535,6 +538,8 @@ int Assemble(const Model &m, const std::vector<double> &a_old, const std::vector
   const size_t nn = m.nodes.size();
   Rows jac(nn);
   std::vector<double> rhs(nn, 0.0);
+  std::vector<std::vector<double>> source(static_cast<size_t>(m.circuits),
+                                          std::vector<double>(nn, 0.0));
   for (size_t e = 0; e < m.tri.size(); ++e) {
     const auto &t = m.tri[e];
     const femm_material_t &mt = m.mat[m.tri_mat[e]];
@@ -556,8 +561,10 @@ int Assemble(const Model &m, const std::vector<double> &a_old, const std::vector
       for (int i = 0; i < 3; ++i) {
         const double coil = tm.area * mt.source_j_per_a / 3.0;
         const double pm = tm.area * (hcx * tm.bx[i] + hcy * tm.by[i]);
-        if (mt.circuit >= 0)
+        if (mt.circuit >= 0) {
+          source[mt.circuit][t[i]] += coil;
           rhs[t[i]] += cur[mt.circuit] * coil;
+        }
         rhs[t[i]] += pm;
         const double bi = bx * tm.bx[i] + by * tm.by[i];
         for (int j = 0; j < 3; ++j) {
@@ -589,8 +596,10 @@ int Assemble(const Model &m, const std::vector<double> &a_old, const std::vector
         for (int i = 0; i < 3; ++i) {
           const double coil = vw * mt.source_j_per_a * q.shape[i];
           const double pm = vw * (hcx * q.br[i] + hcy * q.bz[i]);
-          if (mt.circuit >= 0)
+          if (mt.circuit >= 0) {
+            source[mt.circuit][t[i]] += coil;
             rhs[t[i]] += cur[mt.circuit] * coil;
+          }
           rhs[t[i]] += pm;
           const double bi = br * q.br[i] + bz * q.bz[i];
           for (int j = 0; j < 3; ++j) {
@@ -610,7 +619,9 @@ int Assemble(const Model &m, const std::vector<double> &a_old, const std::vector
     for (const auto &e : row)
       if (!std::isfinite(e.second))
         return FEMM_NUMERICAL_NONFINITE;
-  return Reduce(m, jac, rhs, out);
+  const int st = Reduce(m, jac, rhs, out);
+  out->source_per_circuit = std::move(source);
+  return st;
 }
 
 std::vector<double> Expand(const Assembly &a, const std::vector<double> &free_solution)
@@ -1583,6 +1594,64 @@ struct oracle_symbolic {
   Symbolic s;
 };
 
+// NonlinearP1Solver::SolveTangentMultiRhs (solvers/nonlinear_solver.cu:85-172): the Newton
+// operator at the base state, one right-hand side per direction built circuit by circuit and
+// node by node (rhs[row] += dir_c * sign * source_c[node]), Jacobi PCG from x0 = 0, the
+// derivative expanded with signs and zero on prescribed nodes
+// (ExpandAssemblyDerivative, physics/nonlinear_model.cpp:413-426), then FinalizeResult.
+int Tangent(const Model &m, const std::vector<double> &a, const std::vector<double> &cur,
+            int directions, const double *dirs, const femm_newton_options_t &o, double *da,
+            double *dflux, oracle_result_t *res)
+{
+  for (int d = 0; d < directions; ++d) {
+    res[d] = oracle_result_t{};
+    res[d].status = FEMM_INVALID_ARGUMENT;
+  }
+  if (!(o.linear_relative_tolerance > 0.0) || o.max_linear_iterations < 0)
+    return FEMM_OK;
+  for (int i = 0; i < directions * m.circuits; ++i)
+    if (!std::isfinite(dirs[i]))
+      return FEMM_OK;
+  Assembly tg;
+  const int st = Assemble(m, a, cur, true, &tg);
+  if (st != FEMM_OK) {
+    for (int d = 0; d < directions; ++d)
+      res[d].status = st;
+    return FEMM_OK;
+  }
+  const size_t nf = tg.free_nodes.size(), nn = m.nodes.size();
+  for (int d = 0; d < directions; ++d) {
+    std::vector<double> rhs(nf, 0.0), sol(nf, 0.0);
+    for (int c = 0; c < m.circuits; ++c) {
+      const double scale = dirs[static_cast<size_t>(d) * m.circuits + c];
+      if (scale == 0.0)
+        continue;
+      for (size_t v = 0; v < nn; ++v) {
+        const int32_t row = tg.node_to_free[v];
+        if (row >= 0)
+          rhs[row] += scale * static_cast<double>(tg.node_sign[v]) * tg.source_per_circuit[c][v];
+      }
+    }
+    double info[3];
+    Pcg(static_cast<int>(nf), tg.rows.data(), tg.cols.data(), tg.values.data(), tg.diag.data(),
+        rhs.data(), o.linear_relative_tolerance, o.max_linear_iterations, sol.data(), info, 1);
+    res[d].status = static_cast<int>(info[0]);
+    res[d].linear_iterations = static_cast<int32_t>(info[1]);
+    std::vector<double> expanded(nn, 0.0);
+    if (res[d].status == FEMM_OK)
+      for (size_t v = 0; v < nn; ++v) {
+        const int32_t row = tg.node_to_free[v];
+        if (row >= 0)
+          expanded[v] = static_cast<double>(tg.node_sign[v]) * sol[row];
+      }
+    std::copy(expanded.begin(), expanded.end(), da + static_cast<size_t>(d) * nn);
+    std::vector<double> bx(m.tri.size()), by(m.tri.size()), fl(std::max(m.circuits, 1));
+    if (Finalize(m, expanded, bx.data(), by.data(), fl.data()) == FEMM_OK)
+      std::copy(fl.begin(), fl.begin() + m.circuits, dflux + static_cast<size_t>(d) * m.circuits);
+  }
+  return FEMM_OK;
+}
+
 extern "C" {
 
 int oracle_bh_spline(const double *b, const double *h, int n, double *out_b, double *out_h,
@@ -1748,6 +1817,20 @@ void oracle_block_jacobi(int n, const int32_t *row_offsets, const int32_t *colum
 }
 
 void oracle_set_block_size(int block_size) { g_block_size = std::max(block_size, 1); }
+
+int oracle_tangent_multi_rhs(const femm_model_t *model, const double *base_a,
+                             const double *base_currents, int directions,
+                             const double *direction_currents,
+                             const femm_newton_options_t *options, double *da, double *dflux,
+                             oracle_result_t *results)
+{
+  Model m;
+  if (!LoadModel(model, &m) || directions <= 0)
+    return FEMM_INVALID_ARGUMENT;
+  return Tangent(m, std::vector<double>(base_a, base_a + m.nodes.size()),
+                 std::vector<double>(base_currents, base_currents + m.circuits), directions,
+                 direction_currents, *options, da, dflux, results);
+}
 
 int oracle_newton_solve(const femm_model_t *model, const double *currents,
                         const femm_newton_options_t *options, const double *warm_start, double *a,

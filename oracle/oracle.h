@@ -133,6 +133,15 @@ int oracle_weighted_stress(const femm_model_t *model, const double *a, const dou
                            const double *angles_rad, int angle_count, double out[3],
                            double *radial_b, double *mask_out);
 
+/* Shared-operator tangent solves (NonlinearP1Solver::SolveTangentMultiRhs): at base_a with
+ * base_currents, da[d][node_count] and dflux[d][circuit_count] for each of `directions`
+ * circuit-current directions; results[d] carries status and linear_iterations. */
+int oracle_tangent_multi_rhs(const femm_model_t *model, const double *base_a,
+                             const double *base_currents, int directions,
+                             const double *direction_currents,
+                             const femm_newton_options_t *options, double *da, double *dflux,
+                             oracle_result_t *results);
+
 #ifdef __cplusplus
 }
 #endif

@@ -222,6 +222,22 @@ def safeguarded_solve(model, currents, options, warm_start=None):
                 step=res.relative_step, eq=res.equation_residual)
 
 
+def tangent(model, base_a, base_currents, directions, options):
+    """SolveTangentMultiRhs restated: da[D][N], dflux[D][C], per-direction status / iterations."""
+    N, C = model.node_count, model.circuit_count
+    directions = np.ascontiguousarray(directions, np.float64).reshape(-1, C)
+    D = directions.shape[0]
+    da, dflux = np.zeros((D, N)), np.zeros((D, max(1, C)))
+    res = (OracleResult * D)()
+    base_a = np.ascontiguousarray(base_a, np.float64)
+    base_currents = np.ascontiguousarray(base_currents, np.float64)
+    st = oracle().oracle_tangent_multi_rhs(ctypes.byref(model.c_struct()), _dp(base_a),
+                                           _dp(base_currents), D, _dp(directions),
+                                           ctypes.byref(options), _dp(da), _dp(dflux), res)
+    return dict(status=st, da=da, dflux=dflux[:, :C],
+                results=[dict(status=r.status, linear=r.linear_iterations) for r in res])
+
+
 def newton_batch(model, currents, options, threads):
     currents = np.ascontiguousarray(currents, np.float64)
     B = currents.shape[0]

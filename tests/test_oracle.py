@@ -405,6 +405,73 @@ def test_block_jacobi_newton_reaches_the_frozen_femm_answer():
         assert out["linear"] < point["linear"]
 
 
+# ------------------------------------------------ tangent multi-RHS (SolveTangentMultiRhs)
+
+def _tight(**kw):
+    from femm_gpu_amd.model import plain_options
+    values = dict(relative_tolerance=1e-12, linear_relative_tolerance=1e-13,
+                  max_newton_iterations=200, max_linear_iterations=20000)
+    values.update(kw)
+    return plain_options(**values)
+
+
+def test_tangent_of_a_linear_model_is_the_unit_current_solution():
+    """Linear materials, no PM, zero Dirichlet: J = K and dA/dI = A(I = 1) exactly
+    (superposition), so the tangent equals an independent Newton solve at unit current."""
+    from femm_gpu_amd import meshes
+    model, cur = meshes.square_coil_ring(cells=16, seed=3)
+    o = _tight()
+    base = ol.newton_solve(model, cur, o)
+    unit = ol.newton_solve(model, [1.0], o)
+    assert base["status"] == 0 and unit["status"] == 0
+    t = ol.tangent(model, base["a"], cur, [[1.0], [-2.5]], o)
+    assert [r["status"] for r in t["results"]] == [0, 0]
+    for d, scale in enumerate((1.0, -2.5)):
+        ref = scale * unit["a"]
+        assert np.linalg.norm(t["da"][d] - ref) <= 1e-10 * np.linalg.norm(ref)
+        assert abs(t["dflux"][d][0] - scale * unit["flux"][0]) <= 1e-10 * abs(unit["flux"][0])
+
+
+@pytest.mark.parametrize("case", ["pm_coil", "motor_small"])
+def test_tangent_matches_central_differences_of_newton_solutions(case):
+    """Nonlinear (saturating B-H, PM): the tangent flux derivative equals the central
+    difference (phi(I + h e) - phi(I - h e)) / 2h of two independent Newton solves within
+    the O(h^2) truncation and the solves' tolerance."""
+    from femm_gpu_amd import meshes
+    if case == "pm_coil":
+        model, _, current, _ = ff.nonlinear_pm_coil()
+        cur = np.array([current])
+        dirs = np.array([[1.0]])
+    else:
+        model, _ = meshes.motor_sector(angular_cells=160)
+        cur = meshes.motor_currents(1, seed=12)[0]
+        dirs = np.eye(3)
+    o = _tight() if case == "pm_coil" else _tight(relative_tolerance=1e-10,
+                                                  linear_relative_tolerance=1e-11,
+                                                  max_linear_iterations=100000)
+    base = ol.newton_solve(model, cur, o)
+    assert base["status"] == 0
+    t = ol.tangent(model, base["a"], cur, dirs, o)
+    for d, e in enumerate(dirs):
+        assert t["results"][d]["status"] == 0
+        h = 1e-4 * max(np.abs(cur).max(), 1.0)
+        up = ol.newton_solve(model, cur + h * e, o)
+        dn = ol.newton_solve(model, cur - h * e, o)
+        fd = (up["flux"] - dn["flux"]) / (2 * h)
+        rel = np.abs(t["dflux"][d] - fd).max() / np.abs(fd).max()
+        assert rel <= 2e-5, (case, d, rel)
+        fd_a = (up["a"] - dn["a"]) / (2 * h)
+        assert np.linalg.norm(t["da"][d] - fd_a) <= 2e-5 * np.linalg.norm(fd_a)
+
+
+def test_tangent_rejects_non_finite_directions():
+    model, _, current, _ = ff.nonlinear_pm_coil()
+    o = _tight()
+    base = ol.newton_solve(model, [current], o)
+    t = ol.tangent(model, base["a"], [current], [[1.0], [float("nan")]], o)
+    assert [r["status"] for r in t["results"]] == [2, 2]
+
+
 # ---------------------------------------------------------------- signed reduction (a5, a6)
 
 def _ap_model():
