@@ -47,8 +47,13 @@ __global__ __launch_bounds__(kBlockThreads) void SpmvKernel(DevicePlan P, Device
     vals = W.vals32 + static_cast<size_t>(item) * P.sell_size;
   const size_t n = P.n;
   for (int rep = 0; rep < reps; ++rep) {
-    Spmv<T, T, true>(P, WholeItem(P), vals, x + item * n, y + item * n,
-                     static_cast<const T *>(nullptr));
+    // The PCG's own product: two slices per step in FP32, one in FP64.
+    if constexpr (sizeof(T) == 4)
+      Spmv2<T, T>(P, WholeItem(P), vals, x + item * n, y + item * n,
+                  static_cast<const T *>(nullptr));
+    else
+      Spmv<T, T, true>(P, WholeItem(P), vals, x + item * n, y + item * n,
+                       static_cast<const T *>(nullptr));
     __syncthreads();
   }
 }
