@@ -318,8 +318,9 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(f"learned_mixed block {args.block_size}",
                                                  workload),
-                     "kernel": f"SolveKernel<{3 if args.block_size > 1 else 1}, 2> (learned "
-                               "start + mixed form, one launch)",
+                     "kernel": f"SolveKernel<{3 if args.block_size > 1 else 1}, 1> (mixed "
+                               "form, safeguarded attempt from the learned start; the plain-"
+                               "start redo launch of rejected items is timed with it)",
                      "pcg_iteration_bytes_per_item": chosen["pcg_iter_bytes"],
                      "spmv_alone_fp32": chosen["spmv"],
                      "spmv_alone_fp64": plain["spmv"]},

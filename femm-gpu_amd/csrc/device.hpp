@@ -165,9 +165,9 @@ int LaunchNewton(const DeviceBuffers &dev, int32_t items, const double *currents
                  const femm_newton_options_t &options, const femm_solve_config_t &config,
                  const double *warm, double *a, double *flux, femm_item_result_t *results,
                  hipStream_t stream, const LaunchExtras &extras = LaunchExtras{});
-// Learned-start solve in one launch: per item, the safeguarded Newton from `warm` with
-// `warm_options` (its iteration budget), and if that attempt is not accepted, the
-// reference driver from the plain start with `plain_options` in the same workgroup.
+// Learned-start solve, two launches on `stream`: per item, the safeguarded Newton from
+// `warm` with `warm_options` (its iteration budget), then, for the items that attempt did
+// not accept, the reference driver from the plain start with `plain_options`.
 int LaunchLearned(const DeviceBuffers &dev, int32_t items, const double *currents,
                   const femm_newton_options_t &warm_options,
                   const femm_newton_options_t &plain_options, const femm_solve_config_t &config,

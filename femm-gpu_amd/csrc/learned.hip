@@ -687,9 +687,10 @@ int femm_newton_batch_learned_device(femm_plan *plan, femm_warm_start *ws, int32
   hipStream_t s = static_cast<hipStream_t>(stream);
   if ((st = Predict(ws, items, features, s)) != FEMM_OK)
     return st;
-  // One launch: per item, the safeguarded Newton (full steps, Armijo on the equation
-  // residual, both acceptance gates) from the predicted start within the warm budget, and
-  // if not accepted, the reference driver from the plain start in the same workgroup.
+  // Per item, the safeguarded Newton (full steps, Armijo on the equation residual, both
+  // acceptance gates) from the predicted start within the warm budget; then, in a second
+  // launch on the same stream, the reference driver from the plain start for the items
+  // that attempt did not accept.
   femm_newton_options_t warm_opts = *options;
   warm_opts.require_equation_residual = 1;
   warm_opts.max_newton_iterations = std::min(warm_newton_iterations, options->max_newton_iterations);

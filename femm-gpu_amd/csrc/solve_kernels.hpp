@@ -1792,8 +1792,7 @@ __device__ LinearInfo LinearSolve(const DevicePlan &P, const ItemPointers &w,
 //
 // Both policies scale the equation residual by its norm at the Dirichlet-expanded plain
 // start (nonlinear_solver.cu:565-576), independent of any warm start.  The two policies
-// share one body so that a kernel running both (the learned mode) holds one copy of the
-// assembly and the linear solve.
+// share one body, the assembly and the linear solve.
 template <int kForm, class Team>
 __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers &w,
                                         const double *cur, double *a, bool inputs_ok,
@@ -1979,8 +1978,8 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
 enum SolveMode : int32_t {
   kModeNewton = 0,       // the reference driver
   kModeSafeguarded = 1,  // SolveSafeguarded
-  kModeLearned = 2       // safeguarded from `warm` within a budget, then the driver from the
-                         // plain start in the same workgroup if that attempt is not accepted
+  kModeLearned = 2       // launch pair: safeguarded from `warm` within a budget, then the
+                         // driver from the plain start for the items that attempt left
 };
 
 // The nonlinear solve of `kMode` for one item by `R`'s team, results of workgroup 0
@@ -2002,25 +2001,10 @@ __device__ __attribute__((always_inline)) void SolveItem(const DevicePlan &P, co
   for (int c = 0; c < P.C; ++c)
     inputs_ok = inputs_ok && isfinite(cur[c]);
   const double *warm_item = warm ? warm + static_cast<size_t>(item) * P.N : nullptr;
-  // One call site of ItemSolve (the learned mode loops over its two attempts) keeps a
-  // single inlined copy of the assembly and the linear solve.
-  constexpr int kAttempts = kMode == kModeLearned ? 2 : 1;
-  femm_item_result_t res{}, spent{};
-#pragma clang loop unroll(disable)
-  for (int attempt = 0; attempt < kAttempts; ++attempt) {
-    const bool first = attempt == 0;
-    const bool safeguarded = kMode == kModeSafeguarded || (kMode == kModeLearned && first);
-    const femm_newton_options_t &o =
-        (kMode == kModeLearned && first) ? args.warm_options : args.options;
-    res = ItemSolve<kForm>(P, w, cur, a, inputs_ok, o, args.config, first ? warm_item : nullptr,
-                            safeguarded, R);
-    res.newton_iterations += spent.newton_iterations;
-    res.linear_iterations += spent.linear_iterations;
-    res.refinements += spent.refinements;
-    if (res.status == FEMM_OK)
-      break;
-    spent = res;
-  }
+  static_assert(kMode == kModeNewton || kMode == kModeSafeguarded,
+                "the learned mode is a pair of launches, not a kernel");
+  femm_item_result_t res = ItemSolve<kForm>(P, w, cur, a, inputs_ok, args.options, args.config,
+                                            warm_item, kMode == kModeSafeguarded, R);
   WriteFlux(P, a, flux_out + static_cast<size_t>(item) * P.C, R, lead);
   if (prior != nullptr) {
     res.newton_iterations += earlier.newton_iterations;
@@ -2211,41 +2195,43 @@ int WithForm(int form, F &&f)
 }
 
 
-// One solve form's launch: `partitions` > 1 runs the team kernel cooperatively (the learned
-// mode as a warm attempt and a plain-start redo of the failed items), otherwise one
-// workgroup per item.
+// One solve form's launch: `partitions` > 1 runs the team kernel cooperatively, otherwise
+// one workgroup per item.  The learned mode is two launches of the other modes: the
+// safeguarded attempt from `warm` with the warm options over every item, then the driver
+// from the plain start over the items that attempt did not accept, their counters added to
+// the attempt's.  (One kernel holding both attempts per item needs more registers than a
+// 512-thread workgroup has per lane and spills inside the PCG loop.)
 template <int kForm, int kMode>
 int LaunchFormSolve(const DeviceBuffers &dev, int32_t items, int32_t partitions,
                     const double *currents, const NewtonArgs &args, const double *warm, double *a,
                     double *flux, femm_item_result_t *results, hipStream_t stream,
                     const LaunchExtras &x)
 {
-  if (partitions > 1) {
-    if constexpr (kMode == kModeLearned) {
-      NewtonArgs warm_args = args;
-      warm_args.options = args.warm_options;
-      const int st = LaunchTeams<kForm, kModeSafeguarded>(dev, items, partitions, currents,
-                                                          warm_args, warm, a, flux, results,
-                                                          stream, x);
-      if (st != FEMM_OK)
-        return st;
-      LaunchExtras redo = x;
-      redo.prior = results;
-      NewtonArgs plain_args = args;
-      plain_args.prior_mode = kPriorRedoFailed;
-      return LaunchTeams<kForm, kModeNewton>(dev, items, partitions, currents, plain_args,
-                                             nullptr, a, flux, results, stream, redo);
-    } else {
+  if constexpr (kMode == kModeLearned) {
+    NewtonArgs warm_args = args;
+    warm_args.options = args.warm_options;
+    const int st = LaunchFormSolve<kForm, kModeSafeguarded>(dev, items, partitions, currents,
+                                                            warm_args, warm, a, flux, results,
+                                                            stream, x);
+    if (st != FEMM_OK)
+      return st;
+    LaunchExtras redo = x;
+    redo.prior = results;
+    NewtonArgs plain_args = args;
+    plain_args.prior_mode = kPriorRedoFailed;
+    return LaunchFormSolve<kForm, kModeNewton>(dev, items, partitions, currents, plain_args,
+                                               nullptr, a, flux, results, stream, redo);
+  } else {
+    if (partitions > 1)
       return LaunchTeams<kForm, kMode>(dev, items, partitions, currents, args, warm, a, flux,
                                        results, stream, x);
-    }
+    if (partitions < 0)
+      return FEMM_INVALID_ARGUMENT;
+    hipLaunchKernelGGL((SolveKernel<kForm, kMode>), dim3(items), dim3(kBlockThreads), 0, stream,
+                       dev.plan, dev.work, currents, args, warm, a, flux, results, x.prior,
+                       dev.variants, x.variant_of_item);
+    return hipGetLastError() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
   }
-  if (partitions < 0)
-    return FEMM_INVALID_ARGUMENT;
-  hipLaunchKernelGGL((SolveKernel<kForm, kMode>), dim3(items), dim3(kBlockThreads), 0, stream,
-                     dev.plan, dev.work, currents, args, warm, a, flux, results, x.prior,
-                     dev.variants, x.variant_of_item);
-  return hipGetLastError() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
 }
 
 // Runs one of the three solve modes of form kForm.
