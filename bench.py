@@ -125,7 +125,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--items", type=int, default=256)
     ap.add_argument("--seed", type=int, default=20251203)
-    ap.add_argument("--train-samples", type=int, default=256)
+    ap.add_argument("--train-samples", type=int, default=1024,
+                    help="untimed warm-start training solves (disjoint seed)")
+    ap.add_argument("--train-steps", type=int, default=24000,
+                    help="Adam steps of the warm-start network fit")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--angular-cells", type=int, default=480)
     ap.add_argument("--forms", default="plain_fp64,learned_mixed",
@@ -133,6 +136,10 @@ def main():
     ap.add_argument("--block-size", type=int, default=4,
                     help="PCG preconditioner of the mixed forms: 4 = block-Jacobi of 4 rows "
                          "(FP32, the chosen form), 1 = point Jacobi")
+    ap.add_argument("--split-newton", type=int, default=0,
+                    help="learned form: warm-attempt iterations per item in the first launch "
+                         "before the items still iterating resume as partitioned teams "
+                         "(0 = one launch)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first, as the known "
                          "reference dispatch for FETCH_SIZE / WRITE_SIZE scaling")
@@ -187,6 +194,7 @@ def main():
                     solves_per_s=float(passing.sum()) / (ms * 1e-3),
                     newton_mean=float(r["newton_iterations"].mean()),
                     pcg_mean=float(r["linear_iterations"].mean()),
+                    pcg_max=int(r["linear_iterations"].max()),
                     achieved_gbs=bytes_total / (ms * 1e-3) / 1e9, pcg_iter_bytes=pcg_iter_bytes,
                     a=a.clone(), flux=flux.clone())
 
@@ -208,7 +216,7 @@ def main():
 
     def report(name, f):
         print(f"# {name}: {f['ms']:.1f} ms/step, pass {f['passing']}/{B}, newton "
-              f"{f['newton_mean']:.2f}, pcg/item {f['pcg_mean']:.0f}, "
+              f"{f['newton_mean']:.2f}, pcg/item {f['pcg_mean']:.0f} (max {f['pcg_max']}), "
               f"{f['achieved_gbs']:.0f} GB/s algorithmic; newton counts {f['newton_hist']}",
               file=sys.stderr, flush=True)
 
@@ -239,20 +247,22 @@ def main():
         tf = torch.zeros((S, plan.C), dtype=torch.float64, device="cuda")
         tr = torch.zeros(S * RESULT_BYTES, dtype=torch.uint8, device="cuda")
         plan.solve_batch_device(train_cur.data_ptr(), S, ta.data_ptr(), tf.data_ptr(),
-                                tr.data_ptr(), stream, options, mixed_config())
+                                tr.data_ptr(), stream, options,
+                                mixed_config(block_size=args.block_size))
         torch.cuda.synchronize()
         solve_s = time.perf_counter() - t0
         ws = WarmStart(plan, inputs=3, modes=64, hidden=256, seed=12044)
         fit = ws.fit_device(S, train_cur.data_ptr(), ta.data_ptr(), stream,
-                            train_settings(steps=4000, batch=128, learning_rate=2e-3))
+                            train_settings(steps=args.train_steps, batch=128,
+                                           learning_rate=2e-3))
         torch.cuda.synchronize()
-        training = dict(samples=S, seed=TRAIN_SEED, solve_s=solve_s,
+        training = dict(samples=S, seed=TRAIN_SEED, steps=args.train_steps, solve_s=solve_s,
                         fit_s=time.perf_counter() - t0 - solve_s, report=fit)
         print(f"# training: {S} solves {solve_s:.1f} s, fit {training['fit_s']:.1f} s, "
               f"POD energy {fit['pod_energy_fraction']:.8f}, loss {fit['first_loss']:.3e}"
               f" -> {fit['final_loss']:.3e}", file=sys.stderr, flush=True)
         del ta, tf, tr, train_cur
-        cfg = mixed_config(block_size=args.block_size)
+        cfg = mixed_config(block_size=args.block_size, split_newton_iterations=args.split_newton)
         ms, wall = timer.run(lambda: ws.solve_batch_device(
             B, currents_dev.data_ptr(), currents_dev.data_ptr(), a.data_ptr(), flux.data_ptr(),
             res.data_ptr(), stream, options, cfg, WARM_BUDGET), args.steps, args.warmup)
@@ -305,10 +315,12 @@ def main():
                            f"fallback after {WARM_BUDGET} iterations",
                    "items_passing": chosen["passing"],
                    "newton_mean": chosen["newton_mean"], "pcg_iterations_mean": chosen["pcg_mean"],
+                   "pcg_iterations_max": chosen["pcg_max"],
                    "plain_fp64": {"solves_per_s": plain["solves_per_s"], "ms_per_step": plain["ms"],
                                   "items_passing": plain["passing"],
                                   "newton_mean": plain["newton_mean"],
                                   "pcg_iterations_mean": plain["pcg_mean"],
+                                  "pcg_iterations_max": plain["pcg_max"],
                                   "achieved_GBps": plain["achieved_gbs"]},
                    "speedup_vs_plain_fp64": value / max(plain["solves_per_s"], 1e-30),
                    "max_A_rel_l2_vs_plain_fp64": rel_l2,

@@ -75,6 +75,7 @@ femm_solve_config_t DefaultConfig()
   c.inner_relative_tolerance = 0.1;
   c.partitions = 1;
   c.block_size = 1;
+  c.split_newton_iterations = 0;
   return c;
 }
 
@@ -158,7 +159,8 @@ bool ConfigValid(const femm_solve_config_t &c)
   const bool block_ok = c.block_size == 0 || c.block_size == 1 || c.block_size == 4 ||
                         (c.precision == FEMM_PRECISION_FP64 &&
                          (c.block_size == 2 || c.block_size == 8));
-  return (c.precision == FEMM_PRECISION_FP64 || c.precision == FEMM_PRECISION_MIXED) && block_ok;
+  return (c.precision == FEMM_PRECISION_FP64 || c.precision == FEMM_PRECISION_MIXED) && block_ok &&
+         c.split_newton_iterations >= 0;
 }
 
 femm_newton_options_t DefaultNewtonOptions() { return DefaultOptions(); }

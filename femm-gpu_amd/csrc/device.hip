@@ -530,7 +530,7 @@ int EnsureWork(const HostPlan &h, int32_t items, bool mixed, DeviceBuffers *dev,
     bytes += Bytes<double>(ci * kMaxBlock * nv) +
              (want_mixed ? Bytes<float>(ci * kMaxBlock * nv) : 0);
   bytes += Bytes<double>(ci * 2 * kMaxPartitions * 4) + Bytes<uint32_t>(ci * 2) +
-           Bytes<int32_t>(ci);
+           Bytes<int32_t>(ci) + Bytes<int32_t>(ci + 1);
   void *base = nullptr;
   if (hipMalloc(&base, bytes) != hipSuccess)
     return FEMM_GPU_ALLOCATION_FAILED;
@@ -568,6 +568,7 @@ int EnsureWork(const HostPlan &h, int32_t items, bool mixed, DeviceBuffers *dev,
   w.team_partials = Carve<double>(cur, ci * 2 * kMaxPartitions * 4);
   w.team_barrier = Carve<uint32_t>(cur, ci * 2);
   w.team_abort = Carve<int32_t>(cur, ci);
+  w.cont = Carve<int32_t>(cur, ci + 1);
   dev->work_base = base;
   dev->work_bytes = bytes;
   return hipDeviceSynchronize() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;

@@ -51,6 +51,9 @@ struct DeviceWork {
   double *team_partials;
   uint32_t *team_barrier;
   int32_t *team_abort;
+  // Items a budget-capped launch left to a continuation: [0] = count, then the items in
+  // increasing order ([1 + items]).
+  int32_t *cont;
 };
 
 constexpr int kMaxPartitions = 256;   // workgroups per item in a partitioned solve

@@ -24,7 +24,8 @@ __global__ __launch_bounds__(kBlockThreads) void BlockPcgKernel(DevicePlan P, De
     w.inv[row] = (isfinite(d) && d > 0.0) ? 1.0 / d : 0.0;
   }
   R.Sync();
-  BuildBlockJacobi(P, R.S, w.vals, w.inv, block_size, w.bjac, nullptr);
+  BuildBlockJacobi(P.n, P.slice_ptr, P.slice_width, P.sell_cols, P.diag_sell, R.S.row0, R.S.row1,
+                   w.vals, w.inv, block_size, w.bjac, nullptr);
   R.Sync();
   const BlockPrec<double> B{w.bjac, block_size, VectorStride(P.n)};
   const LinearInfo li =

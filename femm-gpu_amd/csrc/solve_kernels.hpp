@@ -58,7 +58,7 @@ struct BlockReducer {
   int buf;
 
   template <int K>
-  __device__ void Sum(double (&v)[K])
+  __device__ __attribute__((always_inline)) void Sum(double (&v)[K])
   {
     static_assert(K <= 4, "at most four simultaneous sums");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -88,7 +88,7 @@ struct BlockReducer {
     buf ^= 1;
   }
 
-  __device__ double Max(double v)
+  __device__ __attribute__((always_inline)) double Max(double v)
   {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -106,7 +106,7 @@ struct BlockReducer {
     return Uniform(s);
   }
 
-  __device__ int AnyTrue(bool local)
+  __device__ __attribute__((always_inline)) int AnyTrue(bool local)
   {
     double v = local ? 1.0 : 0.0;
     return Max(v) > 0.0;
@@ -132,13 +132,13 @@ struct BlockTeam {
   BlockReducer R;
   Span S;
   template <int K>
-  __device__ void Sum(double (&v)[K])
+  __device__ __attribute__((always_inline)) void Sum(double (&v)[K])
   {
     R.Sum(v);
   }
-  __device__ double Max(double v) { return R.Max(v); }
-  __device__ int AnyTrue(bool local) { return R.AnyTrue(local); }
-  __device__ void Sync() { __syncthreads(); }
+  __device__ __attribute__((always_inline)) double Max(double v) { return R.Max(v); }
+  __device__ __attribute__((always_inline)) int AnyTrue(bool local) { return R.AnyTrue(local); }
+  __device__ __attribute__((always_inline)) void Sync() { __syncthreads(); }
 };
 
 
@@ -190,7 +190,7 @@ struct GridTeam {
   int32_t *abort;
   int G, g, buf;
 
-  __device__ void Sync()
+  __device__ __attribute__((always_inline)) void Sync()
   {
     __syncthreads();
     if (threadIdx.x == 0)
@@ -199,7 +199,7 @@ struct GridTeam {
   }
 
   template <int K>
-  __device__ void Sum(double (&v)[K])
+  __device__ __attribute__((always_inline)) void Sum(double (&v)[K])
   {
     R.Sum(v);
     double *mine = partials + (static_cast<size_t>(buf) * G + g) * 4;
@@ -220,7 +220,7 @@ struct GridTeam {
     buf ^= 1;
   }
 
-  __device__ double Max(double v)
+  __device__ __attribute__((always_inline)) double Max(double v)
   {
     v = R.Max(v);
     double *mine = partials + (static_cast<size_t>(buf) * G + g) * 4;
@@ -235,7 +235,7 @@ struct GridTeam {
     return Uniform(m);
   }
 
-  __device__ int AnyTrue(bool local) { return Max(local ? 1.0 : 0.0) > 0.0; }
+  __device__ __attribute__((always_inline)) int AnyTrue(bool local) { return Max(local ? 1.0 : 0.0) > 0.0; }
 };
 
 __device__ inline int UniformInt(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -484,7 +484,7 @@ __device__ inline T WideSlice(const DevicePlan &P, const T *__restrict__ vals,
 // Two slices per wave per step (slices s and s + kWaves): twice the gathers in flight per
 // round trip for the FP32 iterations, whose loads take half the registers of FP64 ones.
 template <typename T, typename V>
-__device__ double Spmv2(const DevicePlan &P, const Span &S, const T *__restrict__ vals,
+__device__ __attribute__((always_inline)) double Spmv2(const DevicePlan &P, const Span &S, const T *__restrict__ vals,
                         const V *__restrict__ v, V *__restrict__ y, const V *__restrict__ w)
 {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -545,7 +545,7 @@ __device__ double Spmv2(const DevicePlan &P, const Span &S, const T *__restrict_
 // kPipeline = false: the plain two-round-trip loop, for rarely used products (FP64
 // residuals inside the mixed form, equation residuals) where registers matter more.
 template <typename T, typename V, bool kPipeline = true>
-__device__ double Spmv(const DevicePlan &P, const Span &S, const T *__restrict__ vals,
+__device__ __attribute__((always_inline)) double Spmv(const DevicePlan &P, const Span &S, const T *__restrict__ vals,
                        const V *__restrict__ v, V *__restrict__ y, const V *__restrict__ w)
 {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -778,8 +778,13 @@ __device__ inline Rows4<T> Block4Apply(const BlockPrec<T> &B, int row, const Row
 // inverse by forward / back substitution and unscales it.  An unusable block becomes the
 // point Jacobi rows (the reference's NaN-marked fallback, pcg_device.cuh:56-72).  Lane k
 // writes plane k of its block's rows (column k of the inverse).
-__device__ void BuildBlockJacobi(const DevicePlan &P, const Span &S, const double *vals,
-                                 const double *inv, int bs, double *planes, float *planes32)
+// The plan arrays and the row range come as scalars: this out-of-line build must not take
+// the address of the caller's plan or team, which would keep those in scratch memory for
+// the whole solve.
+__device__ void BuildBlockJacobi(int n_rows, const int32_t *slice_ptr, const int32_t *slice_width,
+                                 const int32_t *sell_cols, const int32_t *diag_sell, int row0,
+                                 int row1, const double *vals, const double *inv, int bs,
+                                 double *planes, float *planes32)
 {
   __shared__ double blk[kWaves * (64 * kMaxBlock + 128)];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -788,12 +793,12 @@ __device__ void BuildBlockJacobi(const DevicePlan &P, const Span &S, const doubl
   double *flag = sc + 64;                      // per block: 1 = unusable
   const int bl = lane / bs, k = lane - bl * bs, lead = bl * bs;
   double *F = F_all + bl * bs * bs;
-  const size_t n = VectorStride(P.n);   // plane stride
-  for (int base = S.row0 + wave * 64; base < S.row1; base += kBlockThreads) {
+  const size_t n = VectorStride(n_rows);   // plane stride
+  for (int base = row0 + wave * 64; base < row1; base += kBlockThreads) {
     const int i = base + lane, first = base + lead;
-    const bool valid = i < S.row1;
-    const int width = max(0, min(bs, S.row1 - first));
-    const double d = valid ? vals[P.diag_sell[i]] : 1.0;
+    const bool valid = i < row1;
+    const int width = max(0, min(bs, row1 - first));
+    const double d = valid ? vals[diag_sell[i]] : 1.0;
     sc[lane] = valid ? sqrt(d) : 1.0;
     for (int c = 0; c < bs; ++c)
       F[k * bs + c] = 0.0;
@@ -803,10 +808,10 @@ __device__ void BuildBlockJacobi(const DevicePlan &P, const Span &S, const doubl
       const double reciprocal = 1.0 / d;
       bad_row = !isfinite(reciprocal) || reciprocal < DBL_MIN;
       const int slice = i / kSliceRows;
-      const int pos0 = P.slice_ptr[slice] + lane, entries = P.slice_width[slice];
+      const int pos0 = slice_ptr[slice] + lane, entries = slice_width[slice];
       for (int e = 0; e < entries; ++e) {
         const int pos = pos0 + e * kSliceRows;
-        const int c = P.sell_cols[pos] - first;
+        const int c = sell_cols[pos] - first;
         if (c >= 0 && c < width)
           F[k * bs + c] += vals[pos] / sc[lane] / sc[lead + c];
       }
@@ -1676,8 +1681,14 @@ __device__ bool ApplyVariant(DevicePlan &P, const DeviceVariants &V, const int32
   return true;
 }
 
+// Item `item`'s row of a [items][N] warm-start array (null: the plain start).
+__device__ inline const double *WarmItem(const DevicePlan &P, const double *warm, int item)
+{
+  return warm ? warm + static_cast<size_t>(item) * P.N : nullptr;
+}
+
 // True when a launch with prior results leaves this item as the earlier attempt left it.
-__device__ bool KeepPrior(const femm_item_result_t &earlier, int32_t mode)
+__device__ __attribute__((always_inline)) bool KeepPrior(const femm_item_result_t &earlier, int32_t mode)
 {
   if (mode == kPriorRedoFailed)
     return earlier.status == FEMM_OK;
@@ -1749,7 +1760,8 @@ __device__ LinearInfo LinearSolve(const DevicePlan &P, const ItemPointers &w,
   // BlockJacobiPlan::Prepare, block_jacobi_preconditioner.cu:188-258).
   const int bs = kBlock ? cfg.block_size : 1;
   if constexpr (kBlock) {
-    BuildBlockJacobi(P, R.S, w.vals, w.inv, bs, w.bjac, kMixed ? w.bjac32 : nullptr);
+    BuildBlockJacobi(P.n, P.slice_ptr, P.slice_width, P.sell_cols, P.diag_sell, R.S.row0,
+                     R.S.row1, w.vals, w.inv, bs, w.bjac, kMixed ? w.bjac32 : nullptr);
     R.Sync();
   }
   const BlockPrec<double> B64{w.bjac, bs, VectorStride(P.n)};
@@ -1988,7 +2000,7 @@ enum SolveMode : int32_t {
 template <int kForm, int kMode, class Team>
 __device__ __attribute__((always_inline)) void SolveItem(const DevicePlan &P, const DeviceWork &W, int item, bool lead,
                           bool inputs_ok, const double *currents, const NewtonArgs &args,
-                          const double *warm, double *A, double *flux_out,
+                          const double *warm_item, double *A, double *flux_out,
                           femm_item_result_t *results, const femm_item_result_t *prior,
                           Team &R)
 {
@@ -2000,7 +2012,6 @@ __device__ __attribute__((always_inline)) void SolveItem(const DevicePlan &P, co
   const double *cur = currents + static_cast<size_t>(item) * P.C;
   for (int c = 0; c < P.C; ++c)
     inputs_ok = inputs_ok && isfinite(cur[c]);
-  const double *warm_item = warm ? warm + static_cast<size_t>(item) * P.N : nullptr;
   static_assert(kMode == kModeNewton || kMode == kModeSafeguarded,
                 "the learned mode is a pair of launches, not a kernel");
   femm_item_result_t res = ItemSolve<kForm>(P, w, cur, a, inputs_ok, args.options, args.config,
@@ -2029,8 +2040,8 @@ __global__ __launch_bounds__(kBlockThreads) void SolveKernel(
   DevicePlan P = base;
   const bool inputs_ok = ApplyVariant(P, VS, variant_of_item, item);
   BlockTeam R{BlockReducer{lds, 0}, WholeItem(P)};
-  SolveItem<kForm, kMode>(P, W, item, true, inputs_ok, currents, args, warm, A, flux_out, results,
-                           prior, R);
+  SolveItem<kForm, kMode>(P, W, item, true, inputs_ok, currents, args, WarmItem(P, warm, item), A,
+                           flux_out, results, prior, R);
 }
 
 // `partitions` workgroups per batch item (grid = items x partitions, cooperative launch).
@@ -2056,12 +2067,97 @@ __global__ __launch_bounds__(kBlockThreads) void TeamSolveKernel(
              partitions,
              g,
              0};
-  SolveItem<kForm, kMode>(P, W, item, g == 0, inputs_ok, currents, args, warm, A, flux_out,
-                           results, prior, R);
+  SolveItem<kForm, kMode>(P, W, item, g == 0, inputs_ok, currents, args, WarmItem(P, warm, item),
+                           A, flux_out, results, prior, R);
   R.Sync();
   if (g == 0 && threadIdx.x == 0 &&
       __hip_atomic_load(R.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
     results[item].status = FEMM_INTERNAL_ERROR;
+}
+
+// Lists, in increasing order, the items a later team launch takes: with cap > 0 those a
+// safeguarded launch capped at `cap` Newton iterations left unconverged at the cap, with
+// cap = 0 those whose status is not FEMM_OK.  cont[0] = count, cont[1 + k] = k-th item.
+// One wave; a ballot per 64 items keeps the order.
+__global__ __launch_bounds__(64) void ListItemsKernel(const femm_item_result_t *results,
+                                                      int items, int cap, int32_t *cont)
+{
+  int count = 0;
+  for (int base = 0; base < items; base += 64) {
+    const int i = base + static_cast<int>(threadIdx.x);
+    bool go = false;
+    if (i < items)
+      go = cap > 0 ? results[i].status == FEMM_NONLINEAR_SOLVE_NOT_CONVERGED &&
+                         results[i].newton_iterations >= cap
+                   : results[i].status != FEMM_OK;
+    const unsigned long long mask = __ballot(go);
+    if (go)
+      cont[1 + count + __popcll(mask & ((1ull << threadIdx.x) - 1ull))] = i;
+    count += __popcll(mask);
+  }
+  if (threadIdx.x == 0)
+    cont[0] = count;
+}
+
+// Workgroups per listed item when `grid` workgroups share `count` items: as many as divide
+// evenly, at most kMaxPartitions and at most one per kTeamRowSlices row slices (a team's
+// barriers cost more than they save below about 7k rows per workgroup).
+constexpr int kTeamRowSlices = 112;
+__host__ __device__ inline int ListTeams(int grid, int count, int nslices)
+{
+  int p = grid / (count > 0 ? count : 1);
+  p = p < kMaxPartitions ? p : kMaxPartitions;
+  const int by_rows = nslices / kTeamRowSlices;
+  p = p < by_rows ? p : by_rows;
+  return p > 1 ? p : 1;
+}
+
+// Solves the items ListItemsKernel listed, the launch's workgroups shared out evenly
+// (ListTeams per item; with more items than teams, each team takes items t, t + teams, ...),
+// counters added to the earlier launch's.  kModeSafeguarded resumes each item from the state
+// the capped launch left in A with the options' budget (the rest of the capped attempt's):
+// the state is copied to the item's trial buffer first, because the solve rewrites A with
+// the plain start (for the residual scale) before it takes the warm start, and it uses the
+// trial buffer only after that.  kModeNewton solves each item again from the plain start.
+template <int kForm, int kMode>
+__global__ __launch_bounds__(kBlockThreads) void TeamListKernel(
+    DevicePlan base, DeviceWork W, const double *currents, NewtonArgs args, double *A,
+    double *flux_out, femm_item_result_t *results, DeviceVariants VS,
+    const int32_t *variant_of_item)
+{
+  __shared__ double lds[2 * kWaves * 4];
+  const int count = UniformInt(W.cont[0]);
+  const int p = UniformInt(ListTeams(static_cast<int>(gridDim.x), count, base.nslices));
+  const int teams = static_cast<int>(gridDim.x) / p;
+  const int g = UniformInt(static_cast<int>(blockIdx.x) % p);
+  for (int t = UniformInt(static_cast<int>(blockIdx.x) / p); t < teams && t < count;
+       t += teams) {
+    const int item = UniformInt(W.cont[1 + t]);
+    DevicePlan P = base;
+    const bool inputs_ok = ApplyVariant(P, VS, variant_of_item, item);
+    GridTeam R{BlockReducer{lds, 0},
+               Partition(P, p, g),
+               W.team_partials + static_cast<size_t>(item) * 2 * kMaxPartitions * 4,
+               W.team_barrier + 2 * static_cast<size_t>(item),
+               W.team_abort + item,
+               p,
+               g,
+               0};
+    const double *start = nullptr;
+    if constexpr (kMode == kModeSafeguarded) {
+      const double *a = A + static_cast<size_t>(item) * P.N;
+      double *keep = W.trial + static_cast<size_t>(item) * P.N;
+      for (int v = R.S.node0 + threadIdx.x; v < R.S.node1; v += kBlockThreads)
+        keep[v] = a[v];
+      start = keep;
+    }
+    SolveItem<kForm, kMode>(P, W, item, g == 0, inputs_ok, currents, args, start, A, flux_out,
+                            results, results, R);
+    R.Sync();
+    if (g == 0 && threadIdx.x == 0 &&
+        __hip_atomic_load(R.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+      results[item].status = FEMM_INTERNAL_ERROR;
+  }
 }
 
 // Several plans (meshes of uneven size) in one cooperative launch: entry e covers
@@ -2177,6 +2273,36 @@ int LaunchTeams(const DeviceBuffers &dev, int32_t items, int32_t partitions,
              : FEMM_INTERNAL_ERROR;
 }
 
+// Lists the items (ListItemsKernel with `cap`) and solves them in teams over every
+// co-resident workgroup (TeamListKernel<kForm, kMode>, cooperative launch) with `args`.
+template <int kForm, int kMode>
+int LaunchList(const DeviceBuffers &dev, int32_t items, int32_t cap, const double *currents,
+               const NewtonArgs &args, double *a, double *flux, femm_item_result_t *results,
+               hipStream_t stream, const LaunchExtras &x)
+{
+  const void *kernel = reinterpret_cast<const void *>(&TeamListKernel<kForm, kMode>);
+  const long grid = std::min<long>(CoResidentBlocks(kernel),
+                                   static_cast<long>(items) * kMaxPartitions);
+  if (grid <= 0)
+    return FEMM_INTERNAL_ERROR;
+  ListItemsKernel<<<1, 64, 0, stream>>>(results, items, cap, dev.work.cont);
+  ClearTeamKernel<<<(items + 255) / 256, 256, 0, stream>>>(dev.work.team_barrier,
+                                                           dev.work.team_abort, items);
+  if (hipGetLastError() != hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  DevicePlan plan = dev.plan;
+  DeviceWork work = dev.work;
+  NewtonArgs launch_args = args;
+  DeviceVariants variants = dev.variants;
+  const int32_t *variant_of_item = x.variant_of_item;
+  void *params[] = {&plan, &work, &currents, &launch_args, &a, &flux, &results, &variants,
+                    &variant_of_item};
+  return hipLaunchCooperativeKernel(kernel, dim3(static_cast<unsigned>(grid)), dim3(kBlockThreads),
+                                    params, 0, stream) == hipSuccess
+             ? FEMM_OK
+             : FEMM_INTERNAL_ERROR;
+}
+
 // Runs f(std::integral_constant<int, form>{}) for a runtime solve form (kFormMixed |
 // kFormBlock bits).
 template <class F>
@@ -2196,11 +2322,18 @@ int WithForm(int form, F &&f)
 
 
 // One solve form's launch: `partitions` > 1 runs the team kernel cooperatively, otherwise
-// one workgroup per item.  The learned mode is two launches of the other modes: the
+// one workgroup per item.  The learned mode is launches of the other modes: the
 // safeguarded attempt from `warm` with the warm options over every item, then the driver
 // from the plain start over the items that attempt did not accept, their counters added to
 // the attempt's.  (One kernel holding both attempts per item needs more registers than a
-// 512-thread workgroup has per lane and spills inside the PCG loop.)
+// 512-thread workgroup has per lane and spills inside the PCG loop.)  With
+// one workgroup per item, the plain-start redo runs the rejected items only, in teams that
+// share the whole GPU (TeamListKernel), so a few fallbacks do not run on a few compute units
+// while the rest idle.  With config.split_newton_iterations = k (below the warm budget),
+// the attempt itself is two launches: at most k iterations per item, then TeamListKernel
+// resumes the items still iterating with the rest of the budget.  Which items a team
+// launch takes, and their team size, depend only on the items' own iterations, so repeated
+// runs stay bitwise identical.
 template <int kForm, int kMode>
 int LaunchFormSolve(const DeviceBuffers &dev, int32_t items, int32_t partitions,
                     const double *currents, const NewtonArgs &args, const double *warm, double *a,
@@ -2210,11 +2343,27 @@ int LaunchFormSolve(const DeviceBuffers &dev, int32_t items, int32_t partitions,
   if constexpr (kMode == kModeLearned) {
     NewtonArgs warm_args = args;
     warm_args.options = args.warm_options;
-    const int st = LaunchFormSolve<kForm, kModeSafeguarded>(dev, items, partitions, currents,
-                                                            warm_args, warm, a, flux, results,
-                                                            stream, x);
+    const int budget = args.warm_options.max_newton_iterations;
+    const int split = args.config.split_newton_iterations;
+    const bool resume = split > 0 && split < budget && partitions <= 1;
+    if (resume)
+      warm_args.options.max_newton_iterations = split;
+    int st = LaunchFormSolve<kForm, kModeSafeguarded>(dev, items, partitions, currents,
+                                                      warm_args, warm, a, flux, results, stream,
+                                                      x);
     if (st != FEMM_OK)
       return st;
+    if (resume) {
+      NewtonArgs rest = warm_args;
+      rest.options.max_newton_iterations = budget - split;
+      st = LaunchList<kForm, kModeSafeguarded>(dev, items, split, currents, rest, a, flux,
+                                               results, stream, x);
+      if (st != FEMM_OK)
+        return st;
+    }
+    if (partitions <= 1)
+      return LaunchList<kForm, kModeNewton>(dev, items, 0, currents, args, a, flux, results,
+                                            stream, x);
     LaunchExtras redo = x;
     redo.prior = results;
     NewtonArgs plain_args = args;

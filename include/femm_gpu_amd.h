@@ -44,6 +44,13 @@ typedef struct femm_solve_config {
                                   consecutive free rows (PcgPreconditioner::kBlockJacobi,
                                   solvers/options.h:8-10, block_jacobi_preconditioner.cu),
                                   FP32 planes in the mixed form, which takes 4 only */
+  int32_t split_newton_iterations; /* learned solve, one workgroup per item: 0 = the warm
+                                  attempt runs to its budget in one launch.  k > 0 = every
+                                  item runs at most k safeguarded iterations in that launch;
+                                  the items still iterating then continue (same state, the
+                                  rest of the budget) as partitioned teams that share the
+                                  whole GPU, so the batch's slowest items do not leave most
+                                  compute units idle.  No counterpart in the reference. */
 } femm_solve_config_t;
 
 /* Per-item outcome (reference SolveInfo plus the Newton counters of
