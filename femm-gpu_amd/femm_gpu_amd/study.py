@@ -37,7 +37,7 @@ def main(argv=None):
     ap.add_argument("--train-seed", type=int, default=5001)
     ap.add_argument("--test-seed", type=int, default=9001)
     ap.add_argument("--modes", type=int, default=64)
-    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--steps", type=int, default=24000)
     ap.add_argument("--warm-budget", type=int, default=16)
     ap.add_argument("--block-size", type=int, default=4,
                     help="mixed-form PCG preconditioner: 4 = FP32 block-Jacobi, 1 = point")
