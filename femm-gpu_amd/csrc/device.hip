@@ -529,7 +529,7 @@ int EnsureWork(const HostPlan &h, int32_t items, bool mixed, DeviceBuffers *dev,
   if (want_blocks)
     bytes += Bytes<double>(ci * kMaxBlock * nv) +
              (want_mixed ? Bytes<float>(ci * kMaxBlock * nv) : 0);
-  bytes += Bytes<double>(ci * 2 * kMaxPartitions * 4) + Bytes<uint32_t>(ci * 2) +
+  bytes += Bytes<double>(ci * 2 * kMaxPartitions * 4) + Bytes<uint32_t>(ci * kBarrierWords) +
            Bytes<int32_t>(ci) + Bytes<int32_t>(ci + 1);
   void *base = nullptr;
   if (hipMalloc(&base, bytes) != hipSuccess)
@@ -566,7 +566,7 @@ int EnsureWork(const HostPlan &h, int32_t items, bool mixed, DeviceBuffers *dev,
       w.bjac32 = Carve<float>(cur, ci * kMaxBlock * nv);
   }
   w.team_partials = Carve<double>(cur, ci * 2 * kMaxPartitions * 4);
-  w.team_barrier = Carve<uint32_t>(cur, ci * 2);
+  w.team_barrier = Carve<uint32_t>(cur, ci * kBarrierWords);
   w.team_abort = Carve<int32_t>(cur, ci);
   w.cont = Carve<int32_t>(cur, ci + 1);
   dev->work_base = base;

@@ -47,7 +47,8 @@ struct DeviceWork {
   double *bjac;      // [items][kMaxBlock][n] block-Jacobi planes (block_size > 1 only)
   float *bjac32;     // [items][kMaxBlock][n] (mixed precision with block-Jacobi)
   // Partitioned solves (several workgroups per item): per-item reduction partials
-  // [items][2][kMaxPartitions][4], barrier words [items][2] and abort flags [items].
+  // [items][2][kMaxPartitions][4], barrier words [items][kBarrierWords] and abort flags
+  // [items].
   double *team_partials;
   uint32_t *team_barrier;
   int32_t *team_abort;
@@ -57,6 +58,10 @@ struct DeviceWork {
 };
 
 constexpr int kMaxPartitions = 256;   // workgroups per item in a partitioned solve
+// Team barrier words per item: one arrival word per workgroup, then the release word in a
+// cache line of its own.
+constexpr int kBarrierRelease = kMaxPartitions;
+constexpr int kBarrierWords = kMaxPartitions + 32;
 constexpr int kMaxBlock = 8;          // largest block-Jacobi block (rows)
 
 // Per-item stride of the workspace's free-row vectors: n rounded up to 64 rows, so vector

@@ -147,54 +147,84 @@ struct BlockTeam {
 // the nodes and triangles.  A reduction turns each workgroup's block sum into a partial in
 // global memory; after a team barrier every workgroup adds the G partials in workgroup
 // order, so the result is identical in every workgroup and repeatable for a fixed G.  The
-// barrier is sense-reversing on two words per item (arrivals, generation) with agent-scope
-// release / acquire, so each workgroup's global writes are visible to the others after
-// it.  All workgroups of a launch are co-resident (cooperative launch); a wait that still
-// exceeds kMaxBarrierPolls (seconds) marks the team aborted, and every later barrier of
-// the team returns at once, so no workgroup can wait forever.
+// barrier is flat: workgroup g publishes its writes (agent-scope release) and stores the
+// barrier's number in its own arrival word; workgroup 0 watches all G arrival words with
+// one wave (one load per 64 workgroups), then acquires and stores the number in the release
+// word, which the others watch before they acquire.  No read-modify-write is contended and
+// each workgroup fences twice per barrier.  All workgroups of a launch are co-resident
+// (cooperative launch); a wait that still exceeds kMaxBarrierPolls (seconds) marks the
+// team aborted, and every later barrier of the team returns at once, so no workgroup can
+// wait forever.
 constexpr long kMaxBarrierPolls = 1L << 27;
 
-// Thread 0's part of the team barrier; one out-of-line copy keeps the spin loop from
-// being expanded at every barrier of the solver.
-__device__ __noinline__ void TeamArrive(uint32_t *bar, int32_t *abort, int G)
+// Wave 0's part of team barrier number `target` for workgroup g of G; one out-of-line copy
+// keeps the wait loops from being expanded at every barrier of the solver.
+__device__ __noinline__ void TeamBarrier(uint32_t *bar, int32_t *abort, int G, int g,
+                                        uint32_t target)
 {
-  __threadfence();   // the workgroup's writes (ordered before by the block barrier)
   if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
     return;
-  const uint32_t gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-  const uint32_t arrived =
-      __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-  if (arrived == static_cast<uint32_t>(G - 1)) {
-    __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  } else {
-    // Relaxed polls (no cache invalidation per poll); the fence below acquires once.
-    for (long polls = 0;
-         __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen; ++polls) {
+  const int lane = static_cast<int>(threadIdx.x);
+  // The workgroup's writes (completed to the L2 by the block barrier before), then arrival.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  if (lane == 0)
+    __hip_atomic_store(&bar[g], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  bool aborted = false;
+  if (g == 0) {
+    for (long polls = 0;; ++polls) {
+      bool here = true;
+      for (int h = lane; h < G; h += 64)
+        here = here && static_cast<int32_t>(__hip_atomic_load(&bar[h], __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_AGENT) -
+                                            target) >= 0;
+      if (__all(here))
+        break;
       if (polls > kMaxBarrierPolls ||
           __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-        __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        aborted = true;
         break;
       }
-      __builtin_amdgcn_s_sleep(2);
+      __builtin_amdgcn_s_sleep(1);
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // every member's writes
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");   // passed on with the release word
+    if (!aborted && lane == 0)
+      __hip_atomic_store(&bar[kBarrierRelease], target, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    for (long polls = 0;
+         static_cast<int32_t>(__hip_atomic_load(&bar[kBarrierRelease], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT) -
+                              target) < 0;
+         ++polls) {
+      if (polls > kMaxBarrierPolls ||
+          __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        aborted = true;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // the other workgroups' writes
   }
-  __threadfence();   // acquire: the other workgroups' writes, for the loads after the barrier
+  if (aborted && lane == 0)
+    __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 struct GridTeam {
   BlockReducer R;
   Span S;
   double *partials;   // [2][G][4]
-  uint32_t *bar;      // [2]: arrivals, generation
+  uint32_t *bar;      // [kBarrierWords]: arrival word per workgroup, release word
   int32_t *abort;
   int G, g, buf;
+  uint32_t passed = 0;   // barriers passed so far (the same count in every member)
 
   __device__ __attribute__((always_inline)) void Sync()
   {
     __syncthreads();
-    if (threadIdx.x == 0)
-      TeamArrive(bar, abort, G);
+    ++passed;
+    if (threadIdx.x < 64)
+      TeamBarrier(bar, abort, G, g, passed);
     __syncthreads();
   }
 
@@ -2062,7 +2092,7 @@ __global__ __launch_bounds__(kBlockThreads) void TeamSolveKernel(
   GridTeam R{BlockReducer{lds, 0},
              Partition(P, partitions, g),
              W.team_partials + static_cast<size_t>(item) * 2 * kMaxPartitions * 4,
-             W.team_barrier + 2 * static_cast<size_t>(item),
+             W.team_barrier + static_cast<size_t>(item) * kBarrierWords,
              W.team_abort + item,
              partitions,
              g,
@@ -2138,7 +2168,7 @@ __global__ __launch_bounds__(kBlockThreads) void TeamListKernel(
     GridTeam R{BlockReducer{lds, 0},
                Partition(P, p, g),
                W.team_partials + static_cast<size_t>(item) * 2 * kMaxPartitions * 4,
-               W.team_barrier + 2 * static_cast<size_t>(item),
+               W.team_barrier + static_cast<size_t>(item) * kBarrierWords,
                W.team_abort + item,
                p,
                g,
@@ -2193,7 +2223,7 @@ __global__ __launch_bounds__(kBlockThreads) void GroupSolveKernel(const GroupEnt
   GridTeam R{BlockReducer{lds, 0},
              Partition(P, G, g),
              W.team_partials + static_cast<size_t>(item) * 2 * kMaxPartitions * 4,
-             W.team_barrier + 2 * static_cast<size_t>(item),
+             W.team_barrier + static_cast<size_t>(item) * kBarrierWords,
              W.team_abort + item,
              G,
              g,
@@ -2206,14 +2236,16 @@ __global__ __launch_bounds__(kBlockThreads) void GroupSolveKernel(const GroupEnt
     E.results[item].status = FEMM_INTERNAL_ERROR;
 }
 
-// Fresh team barrier words (two per item) and abort flags before a partitioned launch; a
-// kernel rather than a memset so the pair can sit in a captured graph.
+// Fresh team barrier words (kBarrierWords per item) and abort flags before a partitioned
+// launch; a kernel rather than a memset so the pair can sit in a captured graph.
 __global__ void ClearTeamKernel(uint32_t *barrier, int32_t *abort, int items)
 {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < items; i += gridDim.x * blockDim.x) {
-    barrier[2 * i] = 0u;
-    barrier[2 * i + 1] = 0u;
-    abort[i] = 0;
+  const long words = static_cast<long>(items) * kBarrierWords;
+  for (long k = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; k < words;
+       k += static_cast<long>(gridDim.x) * blockDim.x) {
+    barrier[k] = 0u;
+    if (k % kBarrierWords == 0)
+      abort[k / kBarrierWords] = 0;
   }
 }
 
@@ -2221,12 +2253,13 @@ __global__ void ClearTeamKernel(uint32_t *barrier, int32_t *abort, int items)
 __global__ void ClearGroupKernel(const GroupEntry *entries, int count)
 {
   const GroupEntry &E = entries[blockIdx.x];
-  if (E.partitions > 1)
-    for (int i = threadIdx.x; i < E.items; i += blockDim.x) {
-      E.work.team_barrier[2 * i] = 0u;
-      E.work.team_barrier[2 * i + 1] = 0u;
+  if (E.partitions > 1) {
+    const long words = static_cast<long>(E.items) * kBarrierWords;
+    for (long k = threadIdx.x; k < words; k += blockDim.x)
+      E.work.team_barrier[k] = 0u;
+    for (int i = threadIdx.x; i < E.items; i += blockDim.x)
       E.work.team_abort[i] = 0;
-    }
+  }
 }
 
 // ----------------------------------------------------------------------------- launches
