@@ -2131,8 +2131,8 @@ __global__ __launch_bounds__(64) void ListItemsKernel(const femm_item_result_t *
 
 // Workgroups per listed item when `grid` workgroups share `count` items: as many as divide
 // evenly, at most kMaxPartitions and at most one per kTeamRowSlices row slices (a team's
-// barriers cost more than they save below about 7k rows per workgroup).
-constexpr int kTeamRowSlices = 112;
+// barriers cost more than they save below about 3.5k rows per workgroup).
+constexpr int kTeamRowSlices = 56;
 __host__ __device__ inline int ListTeams(int grid, int count, int nslices)
 {
   int p = grid / (count > 0 ? count : 1);
