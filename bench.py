@@ -136,6 +136,9 @@ def main():
     ap.add_argument("--block-size", type=int, default=4,
                     help="PCG preconditioner of the mixed forms: 4 = block-Jacobi of 4 rows "
                          "(FP32, the chosen form), 1 = point Jacobi")
+    ap.add_argument("--forcing", type=float, default=0.01,
+                    help="learned form: inexact-Newton forcing of the safeguarded warm attempt "
+                         "(linear tolerance = forcing x relative equation residual; 0 = exact)")
     ap.add_argument("--split-newton", type=int, default=0,
                     help="learned form: warm-attempt iterations per item in the first launch "
                          "before the items still iterating resume as partitioned teams "
@@ -262,7 +265,8 @@ def main():
               f"POD energy {fit['pod_energy_fraction']:.8f}, loss {fit['first_loss']:.3e}"
               f" -> {fit['final_loss']:.3e}", file=sys.stderr, flush=True)
         del ta, tf, tr, train_cur
-        cfg = mixed_config(block_size=args.block_size, split_newton_iterations=args.split_newton)
+        cfg = mixed_config(block_size=args.block_size, split_newton_iterations=args.split_newton,
+                           inexact_forcing=args.forcing)
         ms, wall = timer.run(lambda: ws.solve_batch_device(
             B, currents_dev.data_ptr(), currents_dev.data_ptr(), a.data_ptr(), flux.data_ptr(),
             res.data_ptr(), stream, options, cfg, WARM_BUDGET), args.steps, args.warmup)
@@ -308,7 +312,9 @@ def main():
         "config": {"workload": f"motor_sector 12-slot/4-pole anti-periodic, N={plan.N} nodes, "
                                f"T={plan.T} triangles, batch {B}, U(-8e6,8e6) A/m^2 phase currents",
                    "form": "learned warm start (POD 64 + linear + MLP 3-256-256-64 bf16) -> "
-                           "safeguarded Newton, FP32 PCG "
+                           "safeguarded Newton"
+                           + (f" (inexact, forcing {args.forcing:g})" if args.forcing > 0 else "")
+                           + ", FP32 PCG "
                            + (f"(block-Jacobi, {args.block_size}-row blocks) "
                               if args.block_size > 1 else "(point Jacobi) ")
                            + "with FP64 reliable updates from the current state, plain-start "

@@ -1727,6 +1727,19 @@ __device__ __attribute__((always_inline)) bool KeepPrior(const femm_item_result_
   return false;
 }
 
+// Linear tolerance of a safeguarded Newton step: the options' exact tolerance, or with
+// cfg.inexact_forcing = c > 0 an inexact-Newton forcing term c x the relative equation
+// residual (Eisenstat & Walker, SIAM J. Sci. Comput. 17, 1996) within [exact, 1e-3].
+__device__ __attribute__((always_inline)) double ForcingTolerance(const femm_newton_options_t &o,
+                                                                   const femm_solve_config_t &cfg,
+                                                                   double rel_residual)
+{
+  const double exact = o.linear_relative_tolerance;
+  if (!(cfg.inexact_forcing > 0.0))
+    return exact;
+  return fmax(exact, fmin(1e-3, cfg.inexact_forcing * rel_residual));
+}
+
 // Inexact-Newton inner tolerance (physics/nonlinear_model.cpp:166-174).
 __device__ double LinearTolerance(const femm_newton_options_t &o, bool adaptive, double prev)
 {
@@ -1814,6 +1827,25 @@ __device__ LinearInfo LinearSolve(const DevicePlan &P, const ItemPointers &w,
   return li;
 }
 
+// Undamped Newton step ||A_cand - A|| / ||A_cand|| over all nodes, A_cand the free-row
+// solution in w.x expanded with signs and the prescribed values (nonlinear_solver.cu:326-335).
+template <class Team>
+__device__ __attribute__((always_inline)) double StepOf(const DevicePlan &P,
+                                                         const ItemPointers &w, const double *a,
+                                                         Team &R)
+{
+  double s[2] = {0.0, 0.0};
+  for (int v = R.S.node0 + threadIdx.x; v < R.S.node1; v += kBlockThreads) {
+    const int f = P.node_to_free[v];
+    const double cand = f >= 0 ? static_cast<double>(P.node_sign[v]) * w.x[f] : P.boundary[v];
+    const double d = cand - a[v];
+    s[0] += d * d;
+    s[1] += cand * cand;
+  }
+  R.Sum(s);
+  return sqrt(s[0]) / fmax(sqrt(s[1]), DBL_MIN);
+}
+
 // One item's nonlinear solve; leaves the state in `a`.
 //
 // safeguarded = false: the reference Newton driver (NonlinearP1Solver::Solve / SolveBatch,
@@ -1882,6 +1914,7 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
 
   double omega = 1.0, prev = INFINITY;
   bool pending = false;   // step gate passed; equation residual not yet checked
+  bool exact_next = false; // safeguarded: re-solve the current linearization exactly
   bool eq_current = false;
   for (int it = 0; status == FEMM_NONLINEAR_SOLVE_NOT_CONVERGED; ++it) {
     if (safeguarded) {
@@ -1914,7 +1947,10 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
           break;
       }
     }
-    const double tol = safeguarded ? o.linear_relative_tolerance : LinearTolerance(o, adaptive, prev);
+    const double tol =
+        safeguarded ? (exact_next ? o.linear_relative_tolerance : ForcingTolerance(o, cfg, norm / scale))
+                    : LinearTolerance(o, adaptive, prev);
+    exact_next = false;
     if (cfg.pcg_warm_start) {
       for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
         w.x[i] = a[P.free_nodes[i]];
@@ -1926,16 +1962,7 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
       break;
     }
     // Undamped step ||A_cand - A|| / ||A_cand|| over all nodes (nonlinear_solver.cu:326-335).
-    double s[2] = {0.0, 0.0};
-    for (int v = R.S.node0 + threadIdx.x; v < R.S.node1; v += kBlockThreads) {
-      const int f = P.node_to_free[v];
-      const double cand = f >= 0 ? static_cast<double>(P.node_sign[v]) * w.x[f] : P.boundary[v];
-      const double d = cand - a[v];
-      s[0] += d * d;
-      s[1] += cand * cand;
-    }
-    R.Sum(s);
-    const double step = sqrt(s[0]) / fmax(sqrt(s[1]), DBL_MIN);
+    double step = StepOf(P, w, a, R);
     res.newton_iterations = it + 1;
     res.relative_step = step;
     if (!isfinite(step)) {
@@ -1943,10 +1970,20 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
       break;
     }
     if (safeguarded) {
-      if (step < 100.0 * o.relative_tolerance && norm / scale <= o.relative_tolerance) {
+      const bool gates = step < 100.0 * o.relative_tolerance && norm / scale <= o.relative_tolerance;
+      if (gates && tol > o.linear_relative_tolerance) {
+        // Both gates met on an inexact solve: solve this linearization again to the exact
+        // tolerance (the assembly is still the current state's) before accepting.
+        exact_next = true;
+        continue;
+      }
+      if (gates && !(cfg.inexact_forcing > 0.0)) {
         status = FEMM_OK;
         break;
       }
+      // Armijo backtracking.  With inexact forcing, a step that met both gates is tried at
+      // alpha = 1 only: the item is accepted at that candidate if its own equation residual
+      // passes, else at the current state.
       bool accepted = false;
       double alpha = 1.0;
       for (int bt = 0; bt < 20; ++bt, alpha *= 0.5) {
@@ -1958,20 +1995,28 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
         }
         R.Sync();
         if (!AssembleItem(P, w, w.trial, cur, true, R)) {
-          status = FEMM_ASSEMBLY_FAILED;
+          if (!gates)
+            status = FEMM_ASSEMBLY_FAILED;
           break;
         }
         R.Sync();
         const double tn = EquationNorm(P, w, w.trial, R, w.p);
-        if (isfinite(tn) &&
-            (tn <= (1.0 - 1e-4 * alpha) * norm || tn / scale <= o.relative_tolerance)) {
+        const bool take = gates ? isfinite(tn) && tn / scale <= o.relative_tolerance
+                                : isfinite(tn) && (tn <= (1.0 - 1e-4 * alpha) * norm ||
+                                                   tn / scale <= o.relative_tolerance);
+        if (take) {
           for (int v = R.S.node0 + threadIdx.x; v < R.S.node1; v += kBlockThreads)
             a[v] = w.trial[v];
           R.Sync();
           norm = tn;
           accepted = true;
-          break;
         }
+        if (take || gates)
+          break;
+      }
+      if (gates) {
+        status = FEMM_OK;
+        break;
       }
       if (!accepted)
         break;

@@ -32,7 +32,8 @@ class SolveConfig(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("pcg_warm_start", ctypes.c_int32),
                 ("verify_equation_residual", ctypes.c_int32), ("max_refinements", ctypes.c_int32),
                 ("inner_relative_tolerance", ctypes.c_double), ("partitions", ctypes.c_int32),
-                ("block_size", ctypes.c_int32), ("split_newton_iterations", ctypes.c_int32)]
+                ("block_size", ctypes.c_int32), ("split_newton_iterations", ctypes.c_int32),
+                ("inexact_forcing", ctypes.c_double)]
 
 
 class ItemResult(ctypes.Structure):

@@ -39,6 +39,8 @@ def main(argv=None):
     ap.add_argument("--modes", type=int, default=64)
     ap.add_argument("--steps", type=int, default=24000)
     ap.add_argument("--warm-budget", type=int, default=16)
+    ap.add_argument("--forcing", type=float, default=0.01,
+                    help="inexact-Newton forcing of the warm attempt (0 = exact linear solves)")
     ap.add_argument("--block-size", type=int, default=4,
                     help="mixed-form PCG preconditioner: 4 = FP32 block-Jacobi, 1 = point")
     ap.add_argument("--out", default=None)
@@ -123,7 +125,8 @@ def main(argv=None):
     a_learn, flux_learn, res_learn, learn_ms = run_chunks(
         x_test, lambda idx, cur, B, a, flux, res, lo: ws.solve_batch_device(
             B, feats_test[lo:lo + B].data_ptr(), cur.data_ptr(), a.data_ptr(), flux.data_ptr(),
-            res.data_ptr(), stream.cuda_stream, opts, mixed_config(block_size=args.block_size),
+            res.data_ptr(), stream.cuda_stream, opts,
+            mixed_config(block_size=args.block_size, inexact_forcing=args.forcing),
             args.warm_budget,
             variant_ptr=idx.data_ptr()), "learned", args.chunk)
     da = ((a_learn - a_plain).norm(dim=1) / a_plain.norm(dim=1).clamp_min(1e-300)).cpu().numpy()
@@ -140,7 +143,7 @@ def main(argv=None):
                           gpu_s=train_ms / 1e3, wall_s=time.perf_counter() - t0 - fit_s,
                           fit_s=fit_s, report=fit),
                test_seed=args.test_seed, warm_budget=args.warm_budget,
-               block_size=args.block_size,
+               block_size=args.block_size, forcing=args.forcing,
                plain_fp64=summary(res_plain, plain_ms),
                learned_mixed=summary(res_learn, learn_ms),
                max_A_rel_l2=float(da.max()), mean_A_rel_l2=float(da.mean()),

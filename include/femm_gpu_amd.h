@@ -51,6 +51,15 @@ typedef struct femm_solve_config {
                                   rest of the budget) as partitioned teams that share the
                                   whole GPU, so the batch's slowest items do not leave most
                                   compute units idle.  No counterpart in the reference. */
+  double inexact_forcing;      /* safeguarded Newton (learned warm attempt, recovery): 0 = every
+                                  linear solve to linear_relative_tolerance (the reference's
+                                  SolveSafeguarded, nonlinear_solver.cu:534-659); c > 0 =
+                                  inexact Newton (Eisenstat & Walker forcing terms): linear
+                                  tolerance c x the relative equation residual within
+                                  [linear_relative_tolerance, 1e-3], and a step that meets
+                                  both gates is solved again to the exact tolerance before the
+                                  item is accepted (at the exact candidate when its own
+                                  equation residual passes). */
 } femm_solve_config_t;
 
 /* Per-item outcome (reference SolveInfo plus the Newton counters of

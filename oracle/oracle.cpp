@@ -1091,13 +1091,32 @@ double EquationResidual(const Model &m, const std::vector<double> &a,
   return EquationNorm(s1, a) / scale;
 }
 
+// Linear tolerance of a safeguarded Newton step.  forcing = 0: the options' exact tolerance
+// (the reference's SolveSafeguarded).  forcing > 0: inexact Newton with a forcing term
+// proportional to the current relative equation residual (Eisenstat & Walker, "Choosing the
+// forcing terms in an inexact Newton method", SIAM J. Sci. Comput. 17, 1996), kept within
+// [exact, 1e-3].
+double ForcingTolerance(const femm_newton_options_t &o, double forcing, double rel_residual)
+{
+  const double exact = o.linear_relative_tolerance;
+  if (!(forcing > 0.0))
+    return exact;
+  return std::fmax(exact, std::fmin(1e-3, forcing * rel_residual));
+}
+
 // nonlinear_solver.cu:534-659: host-assembled Newton with Armijo backtracking
 // on the equation residual.
 // With `warm`, the iteration starts from the warm state (free nodes taken from it, the
 // prescribed values kept) while the residual scale stays the norm at the plain start:
 // the warm-start hook of Solve (nonlinear_solver.cu:667-698) combined with this loop.
+// With forcing > 0 the linear solves are inexact (ForcingTolerance); a step that meets both
+// gates on an inexact solve is solved again (same linearization) to the exact tolerance,
+// which counts as a Newton iteration, and a step that meets both gates on an exact solve is
+// tried at alpha = 1: the item is accepted at that candidate when the candidate's own
+// equation residual passes, else at the current state.
 int Safeguarded(const Model &m, const std::vector<double> &cur, const femm_newton_options_t &o,
-                std::vector<double> *state_out, oracle_result_t *res, const double *warm = nullptr)
+                std::vector<double> *state_out, oracle_result_t *res, const double *warm = nullptr,
+                double forcing = 0.0)
 {
   std::vector<double> state;
   if (!StartState(m, &state))
@@ -1117,7 +1136,9 @@ int Safeguarded(const Model &m, const std::vector<double> &cur, const femm_newto
       return st;
     norm = EquationNorm(as, state);
   }
+  const double exact = o.linear_relative_tolerance;
   int status = FEMM_NONLINEAR_SOLVE_NOT_CONVERGED;
+  bool exact_next = false;
   for (int it = 0; it < o.max_newton_iterations; ++it) {
     if (!std::isfinite(norm)) {
       status = FEMM_NUMERICAL_NONFINITE;
@@ -1127,9 +1148,11 @@ int Safeguarded(const Model &m, const std::vector<double> &cur, const femm_newto
     std::vector<double> rhs(nf), sol(nf);
     for (size_t r = 0; r < nf; ++r)
       rhs[r] = as.rhs_per_amp[r] + as.rhs_offset[r];
+    const double tol = exact_next ? exact : ForcingTolerance(o, forcing, norm / scale);
+    exact_next = false;
     double info[3];
     Pcg(static_cast<int>(nf), as.rows.data(), as.cols.data(), as.values.data(), as.diag.data(),
-        rhs.data(), o.linear_relative_tolerance, o.max_linear_iterations, sol.data(), info);
+        rhs.data(), tol, o.max_linear_iterations, sol.data(), info);
     res->linear_iterations += static_cast<int32_t>(info[1]);
     if (static_cast<int>(info[0]) != FEMM_OK) {
       status = static_cast<int>(info[0]);
@@ -1148,7 +1171,12 @@ int Safeguarded(const Model &m, const std::vector<double> &cur, const femm_newto
       status = FEMM_NUMERICAL_NONFINITE;
       break;
     }
-    if (step < 100.0 * o.relative_tolerance && norm / scale <= o.relative_tolerance) {
+    const bool gates = step < 100.0 * o.relative_tolerance && norm / scale <= o.relative_tolerance;
+    if (gates && tol > exact) {
+      exact_next = true;
+      continue;
+    }
+    if (gates && !(forcing > 0.0)) {
       status = FEMM_OK;
       break;
     }
@@ -1161,17 +1189,26 @@ int Safeguarded(const Model &m, const std::vector<double> &cur, const femm_newto
       Assembly ta;
       st = Assemble(m, trial, cur, true, &ta);
       if (st != FEMM_OK) {
-        status = st;
+        if (!gates)
+          status = st;
         break;
       }
       const double tn = EquationNorm(ta, trial);
-      if (std::isfinite(tn) && (tn <= (1.0 - 1e-4 * alpha) * norm || tn / scale <= o.relative_tolerance)) {
+      const bool take = gates ? std::isfinite(tn) && tn / scale <= o.relative_tolerance
+                              : std::isfinite(tn) && (tn <= (1.0 - 1e-4 * alpha) * norm ||
+                                                      tn / scale <= o.relative_tolerance);
+      if (take) {
         state = std::move(trial);
         as = std::move(ta);
         norm = tn;
         accepted = true;
-        break;
       }
+      if (take || gates)
+        break;
+    }
+    if (gates) {
+      status = FEMM_OK;
+      break;
     }
     if (!accepted)
       break;
@@ -1851,9 +1888,10 @@ int oracle_newton_solve(const femm_model_t *model, const double *currents,
   return st;
 }
 
-int oracle_safeguarded_solve(const femm_model_t *model, const double *currents,
-                             const femm_newton_options_t *options, const double *warm_start,
-                             double *a, oracle_result_t *result)
+int oracle_safeguarded_solve_forcing(const femm_model_t *model, const double *currents,
+                                     const femm_newton_options_t *options, double forcing,
+                                     const double *warm_start, double *a,
+                                     oracle_result_t *result)
 {
   *result = oracle_result_t{};
   result->status = FEMM_INVALID_ARGUMENT;
@@ -1874,10 +1912,17 @@ int oracle_safeguarded_solve(const femm_model_t *model, const double *currents,
     if (!std::isfinite(c))
       return result->status;
   std::vector<double> state;
-  const int st = Safeguarded(m, cur, o, &state, result, warm_start);
+  const int st = Safeguarded(m, cur, o, &state, result, warm_start, forcing);
   if (state.size() == m.nodes.size())
     std::copy(state.begin(), state.end(), a);
   return result->status = st;
+}
+
+int oracle_safeguarded_solve(const femm_model_t *model, const double *currents,
+                             const femm_newton_options_t *options, const double *warm_start,
+                             double *a, oracle_result_t *result)
+{
+  return oracle_safeguarded_solve_forcing(model, currents, options, 0.0, warm_start, a, result);
 }
 
 int oracle_newton_batch(const femm_model_t *model, int items, const double *currents,

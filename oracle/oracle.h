@@ -107,6 +107,12 @@ int oracle_newton_solve(const femm_model_t *model, const double *currents,
 int oracle_safeguarded_solve(const femm_model_t *model, const double *currents,
                              const femm_newton_options_t *options, const double *warm_start,
                              double *a, oracle_result_t *result);
+/* The same with inexact linear solves: tolerance forcing x (relative equation residual)
+ * within [linear_relative_tolerance, 1e-3]; forcing = 0 is oracle_safeguarded_solve. */
+int oracle_safeguarded_solve_forcing(const femm_model_t *model, const double *currents,
+                                     const femm_newton_options_t *options, double forcing,
+                                     const double *warm_start, double *a,
+                                     oracle_result_t *result);
 
 /* Batch of independent items on `threads` host threads (one item per thread at a
  * time, as the reference's host-assembly pool, nonlinear_solver.cu:371-396). */

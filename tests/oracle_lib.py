@@ -208,16 +208,17 @@ def newton_solve(model, currents, options, warm_start=None):
                 eq=res.equation_residual)
 
 
-def safeguarded_solve(model, currents, options, warm_start=None):
-    """SolveSafeguarded alone (optionally from a warm start)."""
+def safeguarded_solve(model, currents, options, warm_start=None, forcing=0.0):
+    """SolveSafeguarded alone (optionally from a warm start); forcing > 0 makes the linear
+    solves inexact (tolerance forcing x relative equation residual, exact before acceptance)."""
     a = np.zeros(model.node_count)
     res = OracleResult()
     currents = np.ascontiguousarray(currents, np.float64)
     warm = None if warm_start is None else np.ascontiguousarray(warm_start, np.float64)
-    st = oracle().oracle_safeguarded_solve(ctypes.byref(model.c_struct()), _dp(currents),
-                                           ctypes.byref(options),
-                                           None if warm is None else _dp(warm), _dp(a),
-                                           ctypes.byref(res))
+    st = oracle().oracle_safeguarded_solve_forcing(ctypes.byref(model.c_struct()), _dp(currents),
+                                                   ctypes.byref(options), c_d(forcing),
+                                                   None if warm is None else _dp(warm), _dp(a),
+                                                   ctypes.byref(res))
     return dict(status=st, a=a, newton=res.newton_iterations, linear=res.linear_iterations,
                 step=res.relative_step, eq=res.equation_residual)
 

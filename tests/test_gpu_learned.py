@@ -351,3 +351,55 @@ def test_learned_split_resumes_items_in_teams():
         assert np.array_equal(part["flux"], again["flux"])
     ws.close()
     plan.close()
+
+
+def test_inexact_forcing_matches_oracle_and_the_plain_solve():
+    """Safeguarded warm attempt with inexact linear solves (forcing 0.01 x relative equation
+    residual): in the FP64 form from x0 = 0 it reaches the oracle's state under the same
+    policy (A within 1e-9; the linear tolerance follows the computed residual, so last-bit
+    summation differences may move a PCG stopping point: Newton counts within 2, PCG counts
+    within 20 %); in the chosen mixed block-Jacobi form every
+    item passes both gates, agrees with the plain FP64 solve at the gates' resolution
+    (5e-8 / 1e-7) and needs fewer PCG iterations than exact solves from the same starts."""
+    say("case inexact forcing: motor_small, FP64 vs oracle (2 items), mixed block-4 (6 items)")
+    model, _ = meshes.motor_sector(angular_cells=160)
+    plan = Plan(model)
+    train_cur, train = _small_motor_data(plan, 64, seed=31)
+    ws = WarmStart(plan, inputs=3, modes=16, hidden=64, seed=4)
+    ws.fit(train_cur, train["a"], train_settings(steps=800, batch=32))
+    opts = accepted_options()
+    cur = meshes.motor_currents(2, seed=41)
+    a0 = ws.predict(cur)
+    out = ws.solve_batch(cur, cur, opts, reference_config(inexact_forcing=0.01),
+                         warm_iterations=32)
+    for b in range(2):
+        ref = ol.safeguarded_solve(model, cur[b], opts, warm_start=a0[b], forcing=0.01)
+        r = out["results"][b]
+        say(f"  FP64 item {b}: newton {r['newton_iterations']} (oracle {ref['newton']}) pcg "
+            f"{r['linear_iterations']} (oracle {ref['linear']}) dA {_rel_l2(out['a'][b], ref['a']):.2e}")
+        assert r["status"] == ref["status"] == 0
+        assert abs(int(r["newton_iterations"]) - ref["newton"]) <= 2
+        assert abs(int(r["linear_iterations"]) - ref["linear"]) <= 0.2 * ref["linear"]
+        assert _rel_l2(out["a"][b], ref["a"]) <= 1e-9
+    cur = meshes.motor_currents(6, seed=53)
+    plain = plan.solve_batch(cur, opts, reference_config())
+    exact = ws.solve_batch(cur, cur, opts, mixed_config(block_size=4), warm_iterations=16)
+    inexact = ws.solve_batch(cur, cur, opts, mixed_config(block_size=4, inexact_forcing=0.01),
+                             warm_iterations=16)
+    for b in range(6):
+        r = inexact["results"][b]
+        say(f"  mixed item {b}: newton {r['newton_iterations']} pcg {r['linear_iterations']} "
+            f"(exact {exact['results'][b]['linear_iterations']}) "
+            f"dA {_rel_l2(inexact['a'][b], plain['a'][b]):.2e}")
+        assert r["status"] == 0
+        assert r["relative_step"] < 1e-6 and r["equation_residual"] <= 1e-8
+        assert _rel_l2(inexact["a"][b], plain["a"][b]) <= 5e-8
+        assert np.abs(inexact["flux"][b] - plain["flux"][b]).max() <= \
+            1e-7 * np.abs(plain["flux"][b]).max()
+    assert inexact["results"]["linear_iterations"].sum() < \
+        exact["results"]["linear_iterations"].sum()
+    again = ws.solve_batch(cur, cur, opts, mixed_config(block_size=4, inexact_forcing=0.01),
+                           warm_iterations=16)
+    assert np.array_equal(again["a"], inexact["a"])
+    ws.close()
+    plan.close()

@@ -590,3 +590,32 @@ def test_equation_residual_definition():
     assert ol.equation_residual(model, np.zeros(model.node_count), [current]) == 1.0
     # A planted sign error in the state breaks it by orders of magnitude.
     assert ol.equation_residual(model, -r["a"], [current]) > 1e-2
+
+
+@pytest.mark.parametrize("start", ["plain", "perturbed"])
+def test_inexact_safeguarded_newton_reaches_the_frozen_femm_answer(start):
+    """Safeguarded Newton with inexact linear solves (forcing 0.01 x relative equation
+    residual, exact solve before acceptance): the frozen FEMM answer of nonlinear_pm_coil
+    (reference_commands.cu:99-142 bars), both gates at the returned state (the equation
+    residual recomputed on its own) and the exact policy's answer at the gate resolution.
+    forcing = 0 is the exact policy bit for bit.  (Whether it saves PCG iterations depends on
+    the PCG start: with x0 = 0, as here, this small mesh needs more in total; the GPU form
+    starts each PCG from the current state, where the bench measures the saving.)"""
+    model, a_exp, current, flux = ff.nonlinear_pm_coil()
+    o = production_options(require_equation_residual=1)
+    warm = None
+    if start == "perturbed":
+        rng = np.random.default_rng(7)
+        warm = a_exp * (1.0 + 1e-3 * rng.standard_normal(a_exp.size))
+    exact = ol.safeguarded_solve(model, [current], o, warm_start=warm)
+    zero = ol.safeguarded_solve(model, [current], o, warm_start=warm, forcing=0.0)
+    inexact = ol.safeguarded_solve(model, [current], o, warm_start=warm, forcing=0.01)
+    assert np.array_equal(zero["a"], exact["a"]) and zero["linear"] == exact["linear"]
+    assert exact["status"] == 0 and inexact["status"] == 0
+    assert inexact["step"] < 1e-6
+    assert ol.equation_residual(model, inexact["a"], [current]) <= 1e-8
+    assert np.abs(inexact["a"] - a_exp).max() <= 1e-9
+    st, _, _, fl = ol.finalize(model, inexact["a"])
+    assert st == 0 and near_mixed(fl[0], flux, 1e-10, 1e-6)
+    assert np.linalg.norm(inexact["a"] - exact["a"]) <= 5e-8 * np.linalg.norm(exact["a"])
+
