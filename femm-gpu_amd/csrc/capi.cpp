@@ -174,12 +174,13 @@ using femm::EnsurePlanDevice;
 namespace
 {
 
-// Batched Newton on device pointers, optionally per-item material variants, followed by the
-// bounded recovery when options ask for it.
+// Batched Newton on device pointers, optionally per-item material variants: the reference
+// driver followed by the bounded recovery when options ask for it, or (safeguarded) the
+// safeguarded Newton alone.
 int SolveDevice(femm_plan *plan, int32_t items, const int32_t *variant_of_item,
                 const double *currents, const femm_newton_options_t *options,
                 const femm_solve_config_t *config, const double *warm_starts, double *a,
-                double *flux, femm_item_result_t *results, void *stream)
+                double *flux, femm_item_result_t *results, void *stream, bool safeguarded = false)
 {
   if (plan == nullptr || items <= 0 || currents == nullptr || a == nullptr || flux == nullptr ||
       results == nullptr)
@@ -197,6 +198,9 @@ int SolveDevice(femm_plan *plan, int32_t items, const int32_t *variant_of_item,
   const hipStream_t s = static_cast<hipStream_t>(stream);
   femm::LaunchExtras x;
   x.variant_of_item = variant_of_item;
+  if (safeguarded)
+    return femm::LaunchSafeguarded(plan->dev, items, currents, o, c, warm_starts, a, flux,
+                                   results, s, x);
   st = femm::LaunchNewton(plan->dev, items, currents, o, c, warm_starts, a, flux, results, s, x);
   if (st != FEMM_OK || !o.recover_nonconvergence || o.max_newton_iterations <= 0)
     return st;
@@ -437,6 +441,16 @@ int femm_newton_batch_device(femm_plan *plan, int32_t items, const double *curre
                      results, stream);
 }
 
+int femm_newton_batch_safeguarded_device(femm_plan *plan, int32_t items, const double *currents,
+                                         const femm_newton_options_t *options,
+                                         const femm_solve_config_t *config,
+                                         const double *warm_starts, double *a, double *flux,
+                                         femm_item_result_t *results, void *stream)
+{
+  return SolveDevice(plan, items, nullptr, currents, options, config, warm_starts, a, flux,
+                     results, stream, true);
+}
+
 int femm_newton_batch_variants_device(femm_plan *plan, int32_t items,
                                       const int32_t *variant_of_item, const double *currents,
                                       const femm_newton_options_t *options,
@@ -542,7 +556,8 @@ namespace
 int SolveHost(femm_plan *plan, int32_t items, const int32_t *variant_of_item,
               const double *currents, const femm_newton_options_t *options,
               const femm_solve_config_t *config, const double *warm_starts, double *a,
-              double *bx, double *by, double *flux, femm_item_result_t *results)
+              double *bx, double *by, double *flux, femm_item_result_t *results,
+              bool safeguarded = false)
 {
   if (plan == nullptr || items <= 0 || currents == nullptr || a == nullptr || flux == nullptr ||
       results == nullptr)
@@ -574,7 +589,8 @@ int SolveHost(femm_plan *plan, int32_t items, const int32_t *variant_of_item,
       (d_var && hipMemcpy(d_var, variant_of_item, sizeof(int32_t) * B, hipMemcpyHostToDevice) !=
                     hipSuccess))
     return FEMM_INTERNAL_ERROR;
-  st = SolveDevice(plan, items, d_var, d_cur, options, config, d_warm, d_a, d_flux, d_res, nullptr);
+  st = SolveDevice(plan, items, d_var, d_cur, options, config, d_warm, d_a, d_flux, d_res, nullptr,
+                   safeguarded);
   if (st != FEMM_OK)
     return st;
   if (sb && (st = femm::LaunchFinalize(plan->dev, items, d_a, d_bx, d_by, nullptr, nullptr)) !=
@@ -602,6 +618,16 @@ int femm_newton_batch(femm_plan *plan, int32_t items, const double *currents,
 {
   return SolveHost(plan, items, nullptr, currents, options, config, warm_starts, a, bx, by, flux,
                    results);
+}
+
+int femm_newton_batch_safeguarded(femm_plan *plan, int32_t items, const double *currents,
+                                  const femm_newton_options_t *options,
+                                  const femm_solve_config_t *config, const double *warm_starts,
+                                  double *a, double *bx, double *by, double *flux,
+                                  femm_item_result_t *results)
+{
+  return SolveHost(plan, items, nullptr, currents, options, config, warm_starts, a, bx, by, flux,
+                   results, true);
 }
 
 int femm_newton_batch_variants(femm_plan *plan, int32_t items, const int32_t *variant_of_item,

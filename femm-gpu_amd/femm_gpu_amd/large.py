@@ -7,7 +7,9 @@ Cases: the C-core with a 2 mm gap (NdFeB block, two coils at +-5e5 A/m^2, B-H ir
 B-H plunger (249,865 nodes, 7-point quadrature, Br / Bz per element).  Each case is solved
 in the FP64 reference form (point Jacobi PCG), in the mixed form (FP32 PCG iterations with
 FP64 reliable updates to the same 1e-8 equation residual) and in the mixed form with
-block-Jacobi (block size 4, FP32 planes), each with the autotuned number of workgroups;
+block-Jacobi (block size 4, FP32 planes), each with the autotuned number of workgroups, and
+the last form once more as the safeguarded Newton with inexact linear solves (forcing 0.01,
+femm_newton_batch_safeguarded) from the plain start;
 reported per form: wall time, Newton and PCG counts, gates, and each mixed form's A relative
 L2 / flux / element-B difference to the FP64 form.
 """
@@ -56,12 +58,15 @@ def main(argv=None):
              f"(mesh + plan {time.perf_counter() - t0:.1f} s)")
         opts = accepted_options(max_newton_iterations=64, max_linear_iterations=200000)
         forms = {}
-        for form, cfg, bs in (("fp64", reference_config, 1), ("mixed", mixed_config, 1),
-                              ("mixed_block4", mixed_config, 4)):
+        for form, cfg, bs, forcing in (("fp64", reference_config, 1, 0.0),
+                                       ("mixed", mixed_config, 1, 0.0),
+                                       ("mixed_block4", mixed_config, 4, 0.0),
+                                       ("mixed_block4_inexact", mixed_config, 4, 0.01)):
             parts = plan.autotune(1, cfg(partitions=0, block_size=bs))
             t = time.perf_counter()
             out = plan.solve_batch(np.asarray(currents)[None, :], opts,
-                                   cfg(partitions=0, block_size=bs), fields=True)
+                                   cfg(partitions=0, block_size=bs, inexact_forcing=forcing),
+                                   fields=True, safeguarded=forcing > 0)
             wall = time.perf_counter() - t
             r = out["results"][0]
             forms[form] = dict(out=out, summary=dict(

@@ -179,8 +179,9 @@ class Plan:
         return x, info
 
     def solve_batch(self, currents, options: CNewtonOptions = None, config: SolveConfig = None,
-                    warm_starts=None, fields=False):
-        """Batched Newton solve (SolveBatch + FinalizeResult).  Returns a dict of arrays."""
+                    warm_starts=None, fields=False, safeguarded=False):
+        """Batched Newton solve (SolveBatch + FinalizeResult; with safeguarded=True the
+        batched SolveSafeguarded, femm_newton_batch_safeguarded).  Returns a dict of arrays."""
         currents = np.ascontiguousarray(currents, np.float64).reshape(-1, self.C)
         B = currents.shape[0]
         options = options or production_options()
@@ -190,11 +191,12 @@ class Plan:
         by = np.zeros((B, self.T)) if fields else None
         results = np.zeros(B, RESULT_DTYPE)
         warm = None if warm_starts is None else np.ascontiguousarray(warm_starts, np.float64)
-        _check(self._lib.femm_newton_batch(
+        name = "femm_newton_batch_safeguarded" if safeguarded else "femm_newton_batch"
+        _check(getattr(self._lib, name)(
             self._handle, B, _dp(currents), ctypes.byref(options), ctypes.byref(config),
             None if warm is None else _dp(warm), _dp(a), None if bx is None else _dp(bx),
             None if by is None else _dp(by), _dp(flux),
-            results.ctypes.data_as(ctypes.c_void_p)), "femm_newton_batch")
+            results.ctypes.data_as(ctypes.c_void_p)), name)
         out = dict(a=a, flux=flux, results=results)
         if fields:
             out.update(bx=bx, by=by)
@@ -202,15 +204,17 @@ class Plan:
 
     def solve_batch_device(self, currents_ptr, items, a_ptr, flux_ptr, results_ptr, stream_ptr,
                            options: CNewtonOptions = None, config: SolveConfig = None,
-                           warm_ptr=None):
-        """Device-pointer entry (femm_newton_batch_device); nothing is synchronised."""
+                           warm_ptr=None, safeguarded=False):
+        """Device-pointer entry (femm_newton_batch[_safeguarded]_device); nothing is
+        synchronised."""
         options = options or production_options()
         config = config or reference_config()
-        _check(self._lib.femm_newton_batch_device(
+        name = "femm_newton_batch_safeguarded_device" if safeguarded else "femm_newton_batch_device"
+        _check(getattr(self._lib, name)(
             self._handle, int(items), ctypes.c_void_p(currents_ptr), ctypes.byref(options),
             ctypes.byref(config), None if warm_ptr is None else ctypes.c_void_p(warm_ptr),
             ctypes.c_void_p(a_ptr), ctypes.c_void_p(flux_ptr), ctypes.c_void_p(results_ptr),
-            ctypes.c_void_p(stream_ptr)), "femm_newton_batch_device")
+            ctypes.c_void_p(stream_ptr)), name)
 
     def spmv_batch_device(self, items, precision, reps, x_ptr, y_ptr, stream_ptr):
         """reps products per item on the workspace operator (femm_spmv_batch_device)."""

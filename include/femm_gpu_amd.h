@@ -133,6 +133,22 @@ int femm_newton_batch_device(femm_plan *plan, int32_t items, const double *curre
                              const femm_solve_config_t *config, const double *warm_starts,
                              double *a, double *flux, femm_item_result_t *results, void *stream);
 
+/* Batched safeguarded Newton.  Replaces NonlinearP1Solver::SolveSafeguarded
+ * (solvers/nonlinear_solver.cu:534-659): full Newton steps from the plain start (or
+ * warm_starts, the hook of Solve, :667-698) with Armijo backtracking on the equation residual
+ * and both acceptance gates, no driver and no recovery; config->inexact_forcing > 0 makes its
+ * linear solves inexact.  Arguments and outputs as femm_newton_batch(_device). */
+int femm_newton_batch_safeguarded(femm_plan *plan, int32_t items, const double *currents,
+                                  const femm_newton_options_t *options,
+                                  const femm_solve_config_t *config, const double *warm_starts,
+                                  double *a, double *bx, double *by, double *flux,
+                                  femm_item_result_t *results);
+int femm_newton_batch_safeguarded_device(femm_plan *plan, int32_t items, const double *currents,
+                                         const femm_newton_options_t *options,
+                                         const femm_solve_config_t *config,
+                                         const double *warm_starts, double *a, double *flux,
+                                         femm_item_result_t *results, void *stream);
+
 /* Measurement entry (no counterpart in the reference): `reps` products y = A x per item with
  * the operator values the last solve or assembly on this plan left in its workspace, by the
  * PCG's own SELL-64 SpMV, enqueued on `stream`.  precision FEMM_PRECISION_FP64: x, y are

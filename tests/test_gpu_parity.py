@@ -119,6 +119,39 @@ def test_newton_matches_frozen_femm_pm_coil():
     plan.close()
 
 
+def test_batched_safeguarded_matches_oracle():
+    """femm_newton_batch_safeguarded (SolveSafeguarded, nonlinear_solver.cu:534-659) from the
+    plain start: per item the oracle's Newton count and state (A within 1e-9, flux 1e-7) with
+    exact linear solves; with inexact forcing 0.01 the oracle's state under the same policy
+    (A within 1e-9; counts within 2 Newton steps / 20 % PCG, the tolerance following the
+    computed residual) and the frozen FEMM pm_coil answer at its bars."""
+    say("case batched safeguarded pm_coil: 3 currents, exact and forcing 0.01")
+    model, a_exp, current, flux = ff.nonlinear_pm_coil()
+    plan = Plan(model)
+    o = production_options(require_equation_residual=1)
+    cur = np.array([[current], [0.5 * current], [1.5 * current]])
+    exact = plan.solve_batch(cur, o, reference_config(), safeguarded=True)
+    inexact = plan.solve_batch(cur, o, reference_config(inexact_forcing=0.01), safeguarded=True)
+    for b in range(3):
+        for out, forcing in ((exact, 0.0), (inexact, 0.01)):
+            ref = ol.safeguarded_solve(model, cur[b], o, forcing=forcing)
+            r = out["results"][b]
+            say(f"  item {b} forcing {forcing}: newton {r['newton_iterations']} (oracle "
+                f"{ref['newton']}) pcg {r['linear_iterations']} (oracle {ref['linear']})")
+            assert r["status"] == ref["status"] == 0
+            assert r["relative_step"] < 1e-6 and r["equation_residual"] <= 1e-8
+            assert np.linalg.norm(out["a"][b] - ref["a"]) <= 1e-9 * np.linalg.norm(ref["a"])
+            if forcing == 0.0:
+                assert r["newton_iterations"] == ref["newton"]
+            else:
+                assert abs(int(r["newton_iterations"]) - ref["newton"]) <= 2
+                assert abs(int(r["linear_iterations"]) - ref["linear"]) <= 0.2 * ref["linear"]
+    for out in (exact, inexact):
+        assert np.abs(out["a"][0] - a_exp).max() <= 1e-9
+        assert abs(out["flux"][0, 0] - flux) <= 1e-10 + 1e-6 * abs(flux)
+    plan.close()
+
+
 @pytest.mark.parametrize("name,problem,pot_l2,flux_rel", [
     ("periodic_strip", 0, 1e-10, 1e-10),
     ("antiperiodic_strip", 0, 1e-10, 1e-10),
