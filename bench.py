@@ -12,10 +12,15 @@ Forms (both timed, same items, same gates):
                  reference relaxation policy from the plain start.
 * learned_mixed (the reported value) - learned warm start (POD + linear path + BF16 MFMA
                  MLP, predicted on the GPU inside the timed step), safeguarded Newton from
-                 it, FP32 PCG with FP64 reliable updates started from the current state,
-                 plain-start fallback for any item not accepted within the warm budget.
-                 The network is trained beforehand, untimed, on solver solutions of a
-                 disjoint seeded sample set; its cost is reported as `training`.
+                 it with inexact linear solves (tolerance forcing x relative equation
+                 residual, exact before acceptance), FP32 PCG with FP64 reliable updates and
+                 4-row block-Jacobi started from the current state, plain-start fallback for
+                 any item not accepted within the warm budget.  The network is trained
+                 beforehand, untimed, on solver solutions of a disjoint seeded sample set;
+                 its cost is reported as `training`.
+Optional forms (--forms): mixed_plain_start (the reference driver in the mixed form) and
+inexact_plain_start (the chosen form without the network: safeguarded inexact Newton from
+the plain start).
 
 Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs on one GPU.
 """
@@ -132,7 +137,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--angular-cells", type=int, default=480)
     ap.add_argument("--forms", default="plain_fp64,learned_mixed",
-                    help="comma-separated subset of plain_fp64,learned_mixed,mixed_plain_start")
+                    help="comma-separated subset of plain_fp64,learned_mixed,mixed_plain_start,"
+                         "inexact_plain_start")
     ap.add_argument("--block-size", type=int, default=4,
                     help="PCG preconditioner of the mixed forms: 4 = block-Jacobi of 4 rows "
                          "(FP32, the chosen form), 1 = point Jacobi")
@@ -241,6 +247,14 @@ def main():
             options, cfg), args.steps, args.warmup)
         forms["mixed_plain_start"] = collect(ms, wall, True, args.block_size)
         report("mixed_plain_start", forms["mixed_plain_start"])
+    if "inexact_plain_start" in wanted:
+        # The chosen form without the network: safeguarded inexact Newton from the plain start.
+        cfg = mixed_config(block_size=args.block_size, inexact_forcing=args.forcing)
+        ms, wall = timer.run(lambda: plan.solve_batch_device(
+            currents_dev.data_ptr(), B, a.data_ptr(), flux.data_ptr(), res.data_ptr(), stream,
+            options, cfg, safeguarded=True), args.steps, args.warmup)
+        forms["inexact_plain_start"] = collect(ms, wall, True, args.block_size)
+        report("inexact_plain_start", forms["inexact_plain_start"])
     if "learned_mixed" in wanted:
         # Untimed training: solver solutions of a disjoint seeded sample set.
         t0 = time.perf_counter()
