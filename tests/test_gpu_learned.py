@@ -398,6 +398,19 @@ def test_inexact_forcing_matches_oracle_and_the_plain_solve():
             1e-7 * np.abs(plain["flux"][b]).max()
     assert inexact["results"]["linear_iterations"].sum() < \
         exact["results"]["linear_iterations"].sum()
+    # The same method without the network (safeguarded inexact Newton from the plain start)
+    # needs more Newton steps and PCG iterations.
+    no_net = plan.solve_batch(cur, opts, mixed_config(block_size=4, inexact_forcing=0.01),
+                              safeguarded=True)
+    assert np.all(no_net["results"]["status"] == 0)
+    say(f"  without the network: newton {no_net['results']['newton_iterations'].sum()} pcg "
+        f"{no_net['results']['linear_iterations'].sum()}; learned: newton "
+        f"{inexact['results']['newton_iterations'].sum()} pcg "
+        f"{inexact['results']['linear_iterations'].sum()}")
+    assert inexact["results"]["newton_iterations"].sum() < \
+        no_net["results"]["newton_iterations"].sum()
+    assert inexact["results"]["linear_iterations"].sum() < \
+        no_net["results"]["linear_iterations"].sum()
     again = ws.solve_batch(cur, cur, opts, mixed_config(block_size=4, inexact_forcing=0.01),
                            warm_iterations=16)
     assert np.array_equal(again["a"], inexact["a"])
