@@ -665,7 +665,7 @@ int GroupCapacity(bool mixed)
 
 int LaunchGroup(const GroupEntry *entries_device, const std::vector<GroupEntry> &entries,
                 const femm_newton_options_t &options, const femm_solve_config_t &config,
-                bool recovery, hipStream_t stream)
+                bool safeguarded, bool prior_results, hipStream_t stream)
 {
   if (entries.empty())
     return FEMM_INVALID_ARGUMENT;
@@ -674,7 +674,7 @@ int LaunchGroup(const GroupEntry *entries_device, const std::vector<GroupEntry> 
   for (const GroupEntry &E : entries)
     if (block && (E.work.bjac == nullptr || (mixed && E.work.bjac32 == nullptr)))
       return FEMM_INVALID_ARGUMENT;
-  const void *kernel = GroupKernel((mixed ? kFormMixed : 0) | (block ? kFormBlock : 0), recovery);
+  const void *kernel = GroupKernel((mixed ? kFormMixed : 0) | (block ? kFormBlock : 0), safeguarded);
   long grid = 0;
   for (const GroupEntry &E : entries) {
     if (E.items <= 0 || E.partitions <= 0 || E.partitions > kMaxPartitions ||
@@ -690,8 +690,8 @@ int LaunchGroup(const GroupEntry *entries_device, const std::vector<GroupEntry> 
     return FEMM_INTERNAL_ERROR;
   const GroupEntry *table = entries_device;
   int entry_count = count;
-  NewtonArgs args{options, config, recovery ? kPriorRedoUnconverged : kPriorNone, options};
-  int prior = recovery ? 1 : 0;
+  NewtonArgs args{options, config, prior_results ? kPriorRedoUnconverged : kPriorNone, options};
+  int prior = prior_results ? 1 : 0;
   void *params[] = {&table, &entry_count, &args, &prior};
   return hipLaunchCooperativeKernel(kernel, dim3(static_cast<unsigned>(grid)), dim3(kBlockThreads),
                                     params, 0, stream) == hipSuccess

@@ -130,12 +130,13 @@ struct GroupEntry {
 };
 
 // The entries (also copied to `entries_device`, same order) in one cooperative launch of the
-// reference driver: entry e's items use blocks [first_block, first_block + items x partitions).
-// recovery: the bounded safeguarded re-solve of the items an earlier group launch left
-// unconverged (results read and rewritten in place).
+// reference driver, or with `safeguarded` of the safeguarded Newton from the plain start:
+// entry e's items use blocks [first_block, first_block + items x partitions).  With
+// `prior_results` only the items an earlier group launch left unconverged are solved again
+// (results read and rewritten in place): the bounded recovery.
 int LaunchGroup(const GroupEntry *entries_device, const std::vector<GroupEntry> &entries,
                 const femm_newton_options_t &options, const femm_solve_config_t &config,
-                bool recovery, hipStream_t stream);
+                bool safeguarded, bool prior_results, hipStream_t stream);
 // Co-resident workgroup capacity of a group launch on the current device.
 int GroupCapacity(bool mixed);
 

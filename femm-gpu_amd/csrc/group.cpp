@@ -193,8 +193,10 @@ int femm_group_autotune(femm_group *group, const femm_solve_config_t *config,
   return FEMM_OK;
 }
 
-int femm_newton_group(femm_group *group, const femm_newton_options_t *options,
-                      const femm_solve_config_t *config, void *stream)
+// One group solve: the reference driver followed by the bounded recovery, or the safeguarded
+// Newton alone.
+static int GroupSolve(femm_group *group, const femm_newton_options_t *options,
+               const femm_solve_config_t *config, void *stream, bool safeguarded)
 {
   if (group == nullptr)
     return FEMM_INVALID_ARGUMENT;
@@ -223,11 +225,23 @@ int femm_newton_group(femm_group *group, const femm_newton_options_t *options,
       return FEMM_INTERNAL_ERROR;
     group->uploaded = table;
   }
-  st = femm::LaunchGroup(group->table, table, o, c, false, s);
-  if (st != FEMM_OK || !o.recover_nonconvergence || o.max_newton_iterations <= 0)
+  st = femm::LaunchGroup(group->table, table, o, c, safeguarded, false, s);
+  if (safeguarded || st != FEMM_OK || !o.recover_nonconvergence || o.max_newton_iterations <= 0)
     return st;
   // Bounded recovery of exhausted items from the plain start, as femm_newton_batch.
-  return femm::LaunchGroup(group->table, table, o, c, true, s);
+  return femm::LaunchGroup(group->table, table, o, c, true, true, s);
+}
+
+int femm_newton_group(femm_group *group, const femm_newton_options_t *options,
+                      const femm_solve_config_t *config, void *stream)
+{
+  return GroupSolve(group, options, config, stream, false);
+}
+
+int femm_newton_group_safeguarded(femm_group *group, const femm_newton_options_t *options,
+                                  const femm_solve_config_t *config, void *stream)
+{
+  return GroupSolve(group, options, config, stream, true);
 }
 
 }  // extern "C"

@@ -394,10 +394,12 @@ class Group:
         self.partitions = [int(v) for v in p]
         return self.partitions, int(blocks.value)
 
-    def solve(self, stream_ptr, options: CNewtonOptions = None, config: SolveConfig = None):
-        """One launch over every entry (femm_newton_group); nothing is synchronised."""
+    def solve(self, stream_ptr, options: CNewtonOptions = None, config: SolveConfig = None,
+              safeguarded=False):
+        """One launch over every entry (femm_newton_group, or with safeguarded=True
+        femm_newton_group_safeguarded); nothing is synchronised."""
         options = options or production_options()
         config = config or reference_config()
-        _check(self._lib.femm_newton_group(self._handle, ctypes.byref(options),
-                                           ctypes.byref(config), ctypes.c_void_p(stream_ptr)),
-               "femm_newton_group")
+        name = "femm_newton_group_safeguarded" if safeguarded else "femm_newton_group"
+        _check(getattr(self._lib, name)(self._handle, ctypes.byref(options),
+                                        ctypes.byref(config), ctypes.c_void_p(stream_ptr)), name)

@@ -205,6 +205,11 @@ int femm_group_autotune(femm_group *group, const femm_solve_config_t *config,
  * share options and config (config->partitions is ignored: the group's own apply). */
 int femm_newton_group(femm_group *group, const femm_newton_options_t *options,
                       const femm_solve_config_t *config, void *stream);
+/* The same launch with the safeguarded Newton (NonlinearP1Solver::SolveSafeguarded,
+ * nonlinear_solver.cu:534-659) from the plain start for every item of every entry, no driver
+ * and no recovery; config->inexact_forcing > 0 makes its linear solves inexact. */
+int femm_newton_group_safeguarded(femm_group *group, const femm_newton_options_t *options,
+                                  const femm_solve_config_t *config, void *stream);
 
 /* Per-item material variants on the plan's mesh.  Each variant is a full model with the
  * plan's nodes, triangles, Dirichlet nodes and values, node constraints, problem type,

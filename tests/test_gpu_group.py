@@ -137,3 +137,33 @@ def test_group_autotune_fits_and_solves():
     group.close()
     for e in entries:
         e.plan.close()
+
+
+@pytest.mark.parametrize("forcing", [0.0, 0.01])
+def test_group_safeguarded_matches_per_plan_solves(forcing):
+    """femm_newton_group_safeguarded: every entry's items bit for bit the per-plan
+    femm_newton_batch_safeguarded solve at the same partition count (one and several
+    workgroups per item), exact and with inexact forcing; every item passes both gates."""
+    say(f"case group safeguarded vs per-plan (forcing {forcing}): rings 8/12/16/24 cells")
+    entries = [_Entry(8, 2, 1, 1), _Entry(12, 1, 2, 2), _Entry(16, 3, 1, 3), _Entry(24, 1, 4, 4)]
+    opts = accepted_options()
+    group = Group([e.spec() for e in entries])
+    group.solve(torch.cuda.current_stream().cuda_stream, opts,
+                reference_config(inexact_forcing=forcing), safeguarded=True)
+    torch.cuda.synchronize()
+    for e in entries:
+        a, flux, res = e.outputs()
+        single = e.plan.solve_batch(e.currents, opts,
+                                    reference_config(partitions=e.partitions,
+                                                     inexact_forcing=forcing),
+                                    safeguarded=True)
+        say(f"  {e.model.name}: items {e.items} x {e.partitions} wg, newton "
+            f"{res['newton_iterations'].tolist()} pcg {res['linear_iterations'].tolist()}")
+        assert np.all(res["status"] == 0)
+        assert np.all(res["relative_step"] < 1e-6) and np.all(res["equation_residual"] <= 1e-8)
+        assert np.array_equal(a, single["a"]), e.model.name
+        assert np.array_equal(flux, single["flux"]), e.model.name
+        assert res.tobytes() == single["results"].tobytes(), e.model.name
+    group.close()
+    for e in entries:
+        e.plan.close()
