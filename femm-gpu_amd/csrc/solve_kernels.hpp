@@ -265,7 +265,10 @@ struct GridTeam {
     return Uniform(m);
   }
 
-  __device__ __attribute__((always_inline)) int AnyTrue(bool local) { return Max(local ? 1.0 : 0.0) > 0.0; }
+  __device__ __attribute__((always_inline)) int AnyTrue(bool local)
+  {
+    return Max(local ? 1.0 : 0.0) > 0.0;
+  }
 };
 
 __device__ inline int UniformInt(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -514,7 +517,8 @@ __device__ inline T WideSlice(const DevicePlan &P, const T *__restrict__ vals,
 // Two slices per wave per step (slices s and s + kWaves): twice the gathers in flight per
 // round trip for the FP32 iterations, whose loads take half the registers of FP64 ones.
 template <typename T, typename V>
-__device__ __attribute__((always_inline)) double Spmv2(const DevicePlan &P, const Span &S, const T *__restrict__ vals,
+__device__ __attribute__((always_inline)) double Spmv2(const DevicePlan &P, const Span &S,
+                                                        const T *__restrict__ vals,
                         const V *__restrict__ v, V *__restrict__ y, const V *__restrict__ w)
 {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -575,7 +579,8 @@ __device__ __attribute__((always_inline)) double Spmv2(const DevicePlan &P, cons
 // kPipeline = false: the plain two-round-trip loop, for rarely used products (FP64
 // residuals inside the mixed form, equation residuals) where registers matter more.
 template <typename T, typename V, bool kPipeline = true>
-__device__ __attribute__((always_inline)) double Spmv(const DevicePlan &P, const Span &S, const T *__restrict__ vals,
+__device__ __attribute__((always_inline)) double Spmv(const DevicePlan &P, const Span &S,
+                                                       const T *__restrict__ vals,
                        const V *__restrict__ v, V *__restrict__ y, const V *__restrict__ w)
 {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1718,7 +1723,8 @@ __device__ inline const double *WarmItem(const DevicePlan &P, const double *warm
 }
 
 // True when a launch with prior results leaves this item as the earlier attempt left it.
-__device__ __attribute__((always_inline)) bool KeepPrior(const femm_item_result_t &earlier, int32_t mode)
+__device__ __attribute__((always_inline)) bool KeepPrior(const femm_item_result_t &earlier,
+                                                         int32_t mode)
 {
   if (mode == kPriorRedoFailed)
     return earlier.status == FEMM_OK;
@@ -1947,9 +1953,9 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
           break;
       }
     }
-    const double tol =
-        safeguarded ? (exact_next ? o.linear_relative_tolerance : ForcingTolerance(o, cfg, norm / scale))
-                    : LinearTolerance(o, adaptive, prev);
+    const double tol = !safeguarded ? LinearTolerance(o, adaptive, prev)
+                       : exact_next ? o.linear_relative_tolerance
+                                    : ForcingTolerance(o, cfg, norm / scale);
     exact_next = false;
     if (cfg.pcg_warm_start) {
       for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
@@ -1970,7 +1976,8 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
       break;
     }
     if (safeguarded) {
-      const bool gates = step < 100.0 * o.relative_tolerance && norm / scale <= o.relative_tolerance;
+      const bool gates =
+          step < 100.0 * o.relative_tolerance && norm / scale <= o.relative_tolerance;
       if (gates && tol > o.linear_relative_tolerance) {
         // Both gates met on an inexact solve: solve this linearization again to the exact
         // tolerance (the assembly is still the current state's) before accepting.
@@ -2073,11 +2080,10 @@ enum SolveMode : int32_t {
 // written out.  With prior results, an item the earlier launch kept (KeepPrior) is left
 // untouched and the others add their counters to the earlier ones.
 template <int kForm, int kMode, class Team>
-__device__ __attribute__((always_inline)) void SolveItem(const DevicePlan &P, const DeviceWork &W, int item, bool lead,
-                          bool inputs_ok, const double *currents, const NewtonArgs &args,
-                          const double *warm_item, double *A, double *flux_out,
-                          femm_item_result_t *results, const femm_item_result_t *prior,
-                          Team &R)
+__device__ __attribute__((always_inline)) void SolveItem(
+    const DevicePlan &P, const DeviceWork &W, int item, bool lead, bool inputs_ok,
+    const double *currents, const NewtonArgs &args, const double *warm_item, double *A,
+    double *flux_out, femm_item_result_t *results, const femm_item_result_t *prior, Team &R)
 {
   femm_item_result_t earlier{};
   if (prior != nullptr)
