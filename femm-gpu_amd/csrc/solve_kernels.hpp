@@ -2211,8 +2211,10 @@ __global__ __launch_bounds__(kBlockThreads) void TeamListKernel(
   const int p = UniformInt(ListTeams(static_cast<int>(gridDim.x), count, base.nslices));
   const int teams = static_cast<int>(gridDim.x) / p;
   const int g = UniformInt(static_cast<int>(blockIdx.x) % p);
-  for (int t = UniformInt(static_cast<int>(blockIdx.x) / p); t < teams && t < count;
-       t += teams) {
+  const int team = UniformInt(static_cast<int>(blockIdx.x) / p);
+  if (team >= teams)
+    return;   // workgroups past the last whole team
+  for (int t = team; t < count; t += teams) {
     const int item = UniformInt(W.cont[1 + t]);
     DevicePlan P = base;
     const bool inputs_ok = ApplyVariant(P, VS, variant_of_item, item);

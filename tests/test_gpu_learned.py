@@ -416,3 +416,30 @@ def test_inexact_forcing_matches_oracle_and_the_plain_solve():
     assert np.array_equal(again["a"], inexact["a"])
     ws.close()
     plan.close()
+
+
+def test_redo_of_more_items_than_workgroups_matches_plain_solve():
+    """A zero warm budget sends every item to the plain-start redo (ListItemsKernel +
+    TeamListKernel).  With 300 items and at most 256 co-resident workgroups, each team is one
+    workgroup and some teams take a second item; a one-workgroup team sums exactly as the
+    one-workgroup kernel, so every item equals the plain solve bit for bit."""
+    say("case redo of 300 items: saturating ring 8 cells, warm budget 0")
+    model, base = meshes.square_coil_ring(cells=8, seed=3, ring_bh=True)
+    plan = Plan(model)
+    scale = np.linspace(0.2, 1.5, 300)[:, None]
+    cur = np.ascontiguousarray(base[None, :] * scale)
+    train = plan.solve_batch(cur[::10], accepted_options(), reference_config())
+    ws = WarmStart(plan, inputs=cur.shape[1], modes=2, hidden=16, seed=3)
+    ws.fit(cur[::10], train["a"], train_settings(steps=50, batch=16))
+    plain = plan.solve_batch(cur, accepted_options(), reference_config())
+    out = ws.solve_batch(cur, cur, accepted_options(), reference_config(), warm_iterations=0)
+    res, pres = out["results"], plain["results"]
+    say(f"  statuses {np.unique(res['status'])}, newton {res['newton_iterations'].min()}.."
+        f"{res['newton_iterations'].max()}")
+    assert np.all(res["status"] == 0)
+    assert np.array_equal(res["newton_iterations"], pres["newton_iterations"])
+    assert np.array_equal(res["linear_iterations"], pres["linear_iterations"])
+    assert np.array_equal(out["a"], plain["a"])
+    assert np.array_equal(out["flux"], plain["flux"])
+    ws.close()
+    plan.close()
