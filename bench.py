@@ -145,6 +145,9 @@ def main():
     ap.add_argument("--forcing", type=float, default=0.01,
                     help="learned form: inexact-Newton forcing of the safeguarded warm attempt "
                          "(linear tolerance = forcing x relative equation residual; 0 = exact)")
+    ap.add_argument("--inner-tolerance", type=float, default=0.1,
+                    help="mixed forms: FP64 residual replacement once the FP32 residual has "
+                         "dropped by this factor since the last one")
     ap.add_argument("--split-newton", type=int, default=0,
                     help="learned form: warm-attempt iterations per item in the first launch "
                          "before the items still iterating resume as partitioned teams "
@@ -280,7 +283,8 @@ def main():
               f" -> {fit['final_loss']:.3e}", file=sys.stderr, flush=True)
         del ta, tf, tr, train_cur
         cfg = mixed_config(block_size=args.block_size, split_newton_iterations=args.split_newton,
-                           inexact_forcing=args.forcing)
+                           inexact_forcing=args.forcing,
+                           inner_relative_tolerance=args.inner_tolerance)
         ms, wall = timer.run(lambda: ws.solve_batch_device(
             B, currents_dev.data_ptr(), currents_dev.data_ptr(), a.data_ptr(), flux.data_ptr(),
             res.data_ptr(), stream, options, cfg, WARM_BUDGET), args.steps, args.warmup)
