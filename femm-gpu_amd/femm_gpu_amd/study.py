@@ -38,7 +38,9 @@ def main(argv=None):
     ap.add_argument("--test-seed", type=int, default=9001)
     ap.add_argument("--modes", type=int, default=64)
     ap.add_argument("--steps", type=int, default=24000)
-    ap.add_argument("--warm-budget", type=int, default=16)
+    ap.add_argument("--warm-budget", type=int, default=24,
+                    help="safeguarded iterations of the warm attempt before the plain-start "
+                         "fallback (the inexact attempt needs more, cheaper iterations)")
     ap.add_argument("--forcing", type=float, default=0.01,
                     help="inexact-Newton forcing of the warm attempt (0 = exact linear solves)")
     ap.add_argument("--block-size", type=int, default=4,
