@@ -290,6 +290,22 @@ def main():
             res.data_ptr(), stream, options, cfg, WARM_BUDGET), args.steps, args.warmup)
         forms["learned_mixed"] = collect(ms, wall, True, args.block_size)
         report("learned_mixed", forms["learned_mixed"])
+        # The same solve through the host-pointer entry point: inputs copied in, A / flux /
+        # results copied out (PCIe-inclusive; not the headline value).
+        ws.solve_batch(currents, currents, options, cfg, WARM_BUDGET)   # untimed warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        host = ws.solve_batch(currents, currents, options, cfg, WARM_BUDGET)
+        host_s = time.perf_counter() - t0
+        host_ok = int(((host["results"]["status"] == 0)
+                       & (host["results"]["relative_step"] < 1e-6)
+                       & (host["results"]["equation_residual"] <= 1e-8)).sum())
+        forms["learned_mixed"]["host_buffers"] = dict(
+            ms=host_s * 1e3, solves_per_s=host_ok / host_s, items_passing=host_ok,
+            bytes_out=int(host["a"].nbytes + host["flux"].nbytes + host["results"].nbytes))
+        print(f"# learned_mixed through host buffers: {host_s * 1e3:.1f} ms, pass {host_ok}/{B}",
+              file=sys.stderr, flush=True)
+        del host
         forms["learned_mixed"]["spmv"] = spmv_rate(True)
         print(f"# spmv fp32: {forms['learned_mixed']['spmv']}", file=sys.stderr, flush=True)
 
@@ -349,6 +365,7 @@ def main():
                    "speedup_vs_plain_fp64": value / max(plain["solves_per_s"], 1e-30),
                    "max_A_rel_l2_vs_plain_fp64": rel_l2,
                    "max_flux_rel_vs_plain_fp64": fdiff,
+                   "host_buffers": chosen["host_buffers"],
                    "training": training},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

@@ -182,6 +182,10 @@ int SolveDevice(femm_plan *plan, int32_t items, const int32_t *variant_of_item,
                 const femm_solve_config_t *config, const double *warm_starts, double *a,
                 double *flux, femm_item_result_t *results, void *stream, bool safeguarded = false)
 {
+  // An empty batch is an empty result, as SolveBatch returns for empty currents
+  // (solvers/nonlinear_solver.cu:189-192); nothing is launched.
+  if (plan != nullptr && items == 0)
+    return FEMM_OK;
   if (plan == nullptr || items <= 0 || currents == nullptr || a == nullptr || flux == nullptr ||
       results == nullptr)
     return FEMM_INVALID_ARGUMENT;
@@ -458,6 +462,8 @@ int femm_newton_batch_variants_device(femm_plan *plan, int32_t items,
                                       const double *warm_starts, double *a, double *flux,
                                       femm_item_result_t *results, void *stream)
 {
+  if (plan != nullptr && items == 0)
+    return FEMM_OK;
   if (plan == nullptr || variant_of_item == nullptr || plan->dev.variants.count == 0)
     return FEMM_INVALID_ARGUMENT;
   return SolveDevice(plan, items, variant_of_item, currents, options, config, warm_starts, a,
@@ -559,6 +565,8 @@ int SolveHost(femm_plan *plan, int32_t items, const int32_t *variant_of_item,
               double *bx, double *by, double *flux, femm_item_result_t *results,
               bool safeguarded = false)
 {
+  if (plan != nullptr && items == 0)   // empty batch: empty result, nothing touched
+    return FEMM_OK;
   if (plan == nullptr || items <= 0 || currents == nullptr || a == nullptr || flux == nullptr ||
       results == nullptr)
     return FEMM_INVALID_ARGUMENT;
@@ -635,6 +643,8 @@ int femm_newton_batch_variants(femm_plan *plan, int32_t items, const int32_t *va
                                const femm_solve_config_t *config, const double *warm_starts,
                                double *a, double *flux, femm_item_result_t *results)
 {
+  if (plan != nullptr && items == 0)
+    return FEMM_OK;
   if (plan == nullptr || variant_of_item == nullptr || plan->dev.variants.count == 0)
     return FEMM_INVALID_ARGUMENT;
   return SolveHost(plan, items, variant_of_item, currents, options, config, warm_starts, a,

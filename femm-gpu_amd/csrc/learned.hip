@@ -668,6 +668,8 @@ int femm_newton_batch_learned_device(femm_plan *plan, femm_warm_start *ws, int32
                                      int32_t warm_newton_iterations, double *a, double *flux,
                                      femm_item_result_t *results, void *stream)
 {
+  if (plan != nullptr && ws != nullptr && ws->plan == plan && items == 0)
+    return FEMM_OK;   // empty batch: empty result, nothing launched
   if (plan == nullptr || ws == nullptr || ws->plan != plan || items <= 0 || features == nullptr ||
       currents == nullptr || options == nullptr || config == nullptr || a == nullptr ||
       flux == nullptr || results == nullptr || warm_newton_iterations < 0)
@@ -716,6 +718,8 @@ int femm_newton_batch_learned(femm_plan *plan, femm_warm_start *ws, int32_t item
                               const femm_solve_config_t *config, int32_t warm_newton_iterations,
                               double *a, double *flux, femm_item_result_t *results)
 {
+  if (plan != nullptr && ws != nullptr && ws->plan == plan && items == 0)
+    return FEMM_OK;   // empty batch: empty result, nothing touched
   if (plan == nullptr || ws == nullptr || items <= 0 || features == nullptr ||
       currents == nullptr || a == nullptr || flux == nullptr || results == nullptr)
     return FEMM_INVALID_ARGUMENT;

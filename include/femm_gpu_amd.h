@@ -118,7 +118,8 @@ int femm_pcg_batch_block(femm_plan *plan, int32_t items, const double *values, c
  * NonlinearP1Solver::SolveBatch (solvers/nonlinear_solver.cu:183-530) and
  * FinalizeResult (:16-83).  currents[items][C]; warm_starts[items][N] or NULL;
  * config NULL = FP64 reference algorithm.  Outputs: a[items][N],
- * bx/by[items][T] (may be NULL), flux[items][C], results[items]. */
+ * bx/by[items][T] (may be NULL), flux[items][C], results[items].  items = 0 returns
+ * FEMM_OK without touching any pointer or the device (an empty batch, :189-192). */
 int femm_newton_batch(femm_plan *plan, int32_t items, const double *currents,
                       const femm_newton_options_t *options, const femm_solve_config_t *config,
                       const double *warm_starts, double *a, double *bx, double *by, double *flux,

@@ -152,3 +152,24 @@ def test_group_entry_points_validate_without_gpu():
         assert err.value.code == 6           # GPU_UNAVAILABLE
     group.close()
     plan.close()
+
+
+def test_empty_batch_is_an_empty_result_without_gpu():
+    """SolveBatch returns an empty result for empty currents (nonlinear_solver.cu:189-192):
+    items = 0 succeeds on every Newton batch entry point without touching a pointer or the
+    device (so it holds on hosts without a GPU), while negative counts stay invalid."""
+    plan = Plan(ff.three_region())
+    for safeguarded in (False, True):
+        out = plan.solve_batch(np.zeros((0, plan.C)), safeguarded=safeguarded, fields=True)
+        assert out["a"].shape == (0, plan.N) and out["flux"].shape == (0, plan.C)
+        assert out["results"].shape == (0,) and out["bx"].shape == (0, plan.T)
+        plan.solve_batch_device(0, 0, 0, 0, 0, 0, safeguarded=safeguarded)
+    lib = library()
+    null = ctypes.c_void_p(0)
+    for name in ("femm_newton_batch", "femm_newton_batch_safeguarded"):
+        fn = getattr(lib, name)
+        assert fn(plan.handle, 0, null, null, null, null, null, null, null, null, null) == 0
+        assert fn(plan.handle, -1, null, null, null, null, null, null, null, null, null) == 2
+    assert lib.femm_newton_batch(None, 0, null, null, null, null, null, null, null, null,
+                                 null) == 2
+    plan.close()
