@@ -303,7 +303,7 @@ def _run_direct(request, parse_error, path, log) -> int:
         try:
             schema = json.loads(data.decode("utf-8"))["schema_version"]
             model, _ = load_artifact(request["mesh_artifact_path"])
-        except (ValueError, KeyError, TypeError, IndexError) as err:
+        except Exception as err:   # any malformed artifact is an invalid argument
             status, error, model = INVALID_ARGUMENT, f"invalid mesh artifact: {err}", None
         if status == OK and not request_matches_artifact(request, schema, model):
             status = INVALID_ARGUMENT
@@ -312,7 +312,12 @@ def _run_direct(request, parse_error, path, log) -> int:
     post_requested = request["compute_force_torque"] or bool(request["airgap_angles_deg"])
     solve_status = status
     if status == OK:
-        solution, solve_status, post_requested, post, post_status = _solve(request, model)
+        from .solver import FemmError
+
+        try:
+            solution, solve_status, post_requested, post, post_status = _solve(request, model)
+        except FemmError as err:    # the plan or the solve call failed as a whole
+            solve_status = err.code
         status = solve_status
         if solve_status == OK and post_requested:
             status = post_status

@@ -183,3 +183,24 @@ def test_success_layout_and_number_format():
     doc = json.loads(text)
     assert (doc["status"], doc["postprocess_status"]) == ("FAIL", "NOT_RUN")
     assert doc["mesh_node_count"] == 0 and doc["convergence"]["iterations"] == 4
+
+
+def test_solver_failure_is_reported_in_the_response(tmp_path):
+    """A valid request on a host without a GPU: the library's GPU_UNAVAILABLE becomes the
+    response's solve status (no solve attempted, as when the reference's Initialize fails)."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    art = tmp_path / "mesh.gpu.json"
+    shutil.copyfile(ARTIFACT, art)
+    req, resp = tmp_path / "request.json", tmp_path / "response.json"
+    req.write_text(json.dumps(_request(art, _sha(art))), encoding="utf-8")
+    assert sample.run_sample(str(req), str(resp)) == 1
+    doc = json.loads(resp.read_text(encoding="utf-8"))
+    assert (doc["status"], doc["solve_status"], doc["error_identifier"], doc["error_message"]) == \
+        ("FAIL", "GPU_UNAVAILABLE", "GPU_FEMM_GPU_UNAVAILABLE", "GPU_UNAVAILABLE")
+    assert doc["problem_type"] == "planar" and doc["convergence"]["iterations"] == 0
+    assert list(tmp_path.glob("*.tmp.*")) == []
