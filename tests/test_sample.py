@@ -204,3 +204,20 @@ def test_solver_failure_is_reported_in_the_response(tmp_path):
         ("FAIL", "GPU_UNAVAILABLE", "GPU_FEMM_GPU_UNAVAILABLE", "GPU_UNAVAILABLE")
     assert doc["problem_type"] == "planar" and doc["convergence"]["iterations"] == 0
     assert list(tmp_path.glob("*.tmp.*")) == []
+
+
+def test_command_line_exit_codes(tmp_path):
+    """`python -m femm_gpu_amd.sample`: 2 for a wrong command line, 1 with a published FAIL
+    response for an invalid request (the reference CLI's exit contract, app/cli.cpp:20-181)."""
+    import os
+    import subprocess
+    import sys
+
+    env = dict(os.environ, PYTHONPATH=str(Path(sample.__file__).resolve().parents[1]))
+    cmd = [sys.executable, "-m", "femm_gpu_amd.sample"]
+    assert subprocess.run(cmd, env=env, capture_output=True).returncode == 2
+    req, resp = tmp_path / "request.json", tmp_path / "response.json"
+    req.write_text(json.dumps(_request("m.json", "a" * 64, extra=1)), encoding="utf-8")
+    done = subprocess.run(cmd + [str(req), str(resp)], env=env, capture_output=True, text=True)
+    assert done.returncode == 1 and "FAIL planar-DC request" in done.stderr
+    assert json.loads(resp.read_text(encoding="utf-8"))["status"] == "FAIL"
