@@ -266,7 +266,10 @@ def run_sample(request_path, response_path, log=sys.stderr) -> int:
         print("FAIL magnetostatic solve: response must not overwrite mesh artifact", file=log)
         return 1
     temporary = f"{response_path}.tmp.{time.time_ns()}.{next(_sequence)}"
-    code = _run_direct(request, parse_error, temporary, log)
+    try:
+        code = _run_direct(request, parse_error, temporary, log)
+    except OSError:            # the response could not be written at all
+        code = 1
     try:
         os.replace(temporary, response_path)
     except OSError:

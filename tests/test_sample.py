@@ -221,3 +221,9 @@ def test_command_line_exit_codes(tmp_path):
     done = subprocess.run(cmd + [str(req), str(resp)], env=env, capture_output=True, text=True)
     assert done.returncode == 1 and "FAIL planar-DC request" in done.stderr
     assert json.loads(resp.read_text(encoding="utf-8"))["status"] == "FAIL"
+
+
+def test_unwritable_response_location_fails_cleanly(tmp_path):
+    req = tmp_path / "request.json"
+    req.write_text("{}", encoding="utf-8")
+    assert sample.run_sample(str(req), str(tmp_path / "missing_dir" / "response.json")) == 1
