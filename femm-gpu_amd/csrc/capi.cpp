@@ -9,9 +9,11 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <memory>
 #include <string>
 #include <vector>
 
+#include "api_guard.hpp"
 #include "device.hpp"
 #include "femm_gpu_amd.h"
 #include "handle.hpp"
@@ -222,18 +224,16 @@ extern "C" {
 
 int femm_plan_create(const femm_model_t *model, femm_plan **out)
 {
+  FEMM_API_BEGIN
   if (out == nullptr)
     return FEMM_INVALID_ARGUMENT;
   *out = nullptr;
-  femm_plan *plan = new (std::nothrow) femm_plan;
-  if (plan == nullptr)
-    return FEMM_INTERNAL_ERROR;
+  // Owned until it is handed out, so a failed host allocation below releases it.
+  std::unique_ptr<femm_plan> plan(new femm_plan);
   std::string error;
   int st = femm::BuildHostPlan(model, &plan->host, &error);
-  if (st != FEMM_OK) {
-    delete plan;
+  if (st != FEMM_OK)
     return st;
-  }
   ModelCopy &mc = plan->model;
   const size_t N = static_cast<size_t>(model->node_count), T = static_cast<size_t>(model->triangle_count);
   mc.node_xy.assign(model->node_xy, model->node_xy + 2 * N);
@@ -261,12 +261,14 @@ int femm_plan_create(const femm_model_t *model, femm_plan **out)
   mc.view.curve_h = nullptr;
   // The device copy is uploaded by the first call that computes, so the
   // symbolic plan is available on hosts without a GPU.
-  *out = plan;
+  *out = plan.release();
   return FEMM_OK;
+  FEMM_API_END
 }
 
 void femm_plan_destroy(femm_plan *plan)
 {
+  FEMM_API_BEGIN
   if (plan == nullptr)
     return;
   if (plan->dev.base != nullptr || plan->io != nullptr)
@@ -277,10 +279,12 @@ void femm_plan_destroy(femm_plan *plan)
   if (plan->post_base)
     (void)hipFree(plan->post_base);
   delete plan;
+  FEMM_API_END_VOID
 }
 
 int femm_plan_sizes(const femm_plan *plan, int32_t sizes[6])
 {
+  FEMM_API_BEGIN
   if (plan == nullptr || sizes == nullptr)
     return FEMM_INVALID_ARGUMENT;
   const femm::HostPlan &h = plan->host;
@@ -291,12 +295,14 @@ int femm_plan_sizes(const femm_plan *plan, int32_t sizes[6])
   sizes[4] = h.C;
   sizes[5] = h.Q;
   return FEMM_OK;
+  FEMM_API_END
 }
 
 int femm_plan_copy_symbolic(const femm_plan *plan, int32_t *row_offsets, int32_t *column_indices,
                             int32_t *diagonal_slots, int32_t *free_nodes, int32_t *node_to_free,
                             int8_t *node_sign, double *boundary_values)
 {
+  FEMM_API_BEGIN
   if (plan == nullptr)
     return FEMM_INVALID_ARGUMENT;
   const femm::HostPlan &h = plan->host;
@@ -315,12 +321,14 @@ int femm_plan_copy_symbolic(const femm_plan *plan, int32_t *row_offsets, int32_t
   if (boundary_values)
     std::copy(h.boundary.begin(), h.boundary.end(), boundary_values);
   return FEMM_OK;
+  FEMM_API_END
 }
 
 int femm_assemble_batch(femm_plan *plan, int32_t items, const double *states,
                         const double *currents, int32_t use_newton, double *values, double *rhs,
                         double *diagonal)
 {
+  FEMM_API_BEGIN
   if (plan == nullptr || items <= 0 || states == nullptr || currents == nullptr)
     return FEMM_INVALID_ARGUMENT;
   const femm::HostPlan &h = plan->host;
@@ -375,19 +383,23 @@ int femm_assemble_batch(femm_plan *plan, int32_t items, const double *states,
     if (!v)
       return FEMM_ASSEMBLY_FAILED;
   return FEMM_OK;
+  FEMM_API_END
 }
 
 int femm_pcg_batch(femm_plan *plan, int32_t items, const double *values, const double *rhs,
                    double relative_tolerance, int32_t max_iterations, double *x, double *info)
 {
+  FEMM_API_BEGIN
   return femm_pcg_batch_block(plan, items, values, rhs, relative_tolerance, max_iterations, 1, x,
                               info);
+  FEMM_API_END
 }
 
 int femm_pcg_batch_block(femm_plan *plan, int32_t items, const double *values, const double *rhs,
                          double relative_tolerance, int32_t max_iterations, int32_t block_size,
                          double *x, double *info)
 {
+  FEMM_API_BEGIN
   if (plan == nullptr || items <= 0 || values == nullptr || rhs == nullptr ||
       (block_size != 1 && block_size != 2 && block_size != 4 && block_size != 8))
     return FEMM_INVALID_ARGUMENT;
@@ -422,11 +434,13 @@ int femm_pcg_batch_block(femm_plan *plan, int32_t items, const double *values, c
                    hipSuccess))
     return FEMM_INTERNAL_ERROR;
   return FEMM_OK;
+  FEMM_API_END
 }
 
 int femm_spmv_batch_device(femm_plan *plan, int32_t items, int32_t precision, int32_t reps,
                            const void *x, void *y, void *stream)
 {
+  FEMM_API_BEGIN
   if (plan == nullptr || items <= 0 || reps < 0 || x == nullptr || y == nullptr ||
       (precision != FEMM_PRECISION_FP64 && precision != FEMM_PRECISION_MIXED))
     return FEMM_INVALID_ARGUMENT;
@@ -434,6 +448,7 @@ int femm_spmv_batch_device(femm_plan *plan, int32_t items, int32_t precision, in
     return FEMM_INVALID_ARGUMENT;   // no operator assembled on the device yet
   return femm::LaunchSpmv(plan->dev, items, precision == FEMM_PRECISION_MIXED, reps, x, y,
                           static_cast<hipStream_t>(stream));
+  FEMM_API_END
 }
 
 int femm_newton_batch_device(femm_plan *plan, int32_t items, const double *currents,
@@ -441,8 +456,10 @@ int femm_newton_batch_device(femm_plan *plan, int32_t items, const double *curre
                              const femm_solve_config_t *config, const double *warm_starts,
                              double *a, double *flux, femm_item_result_t *results, void *stream)
 {
+  FEMM_API_BEGIN
   return SolveDevice(plan, items, nullptr, currents, options, config, warm_starts, a, flux,
                      results, stream);
+  FEMM_API_END
 }
 
 int femm_newton_batch_safeguarded_device(femm_plan *plan, int32_t items, const double *currents,
@@ -451,8 +468,10 @@ int femm_newton_batch_safeguarded_device(femm_plan *plan, int32_t items, const d
                                          const double *warm_starts, double *a, double *flux,
                                          femm_item_result_t *results, void *stream)
 {
+  FEMM_API_BEGIN
   return SolveDevice(plan, items, nullptr, currents, options, config, warm_starts, a, flux,
                      results, stream, true);
+  FEMM_API_END
 }
 
 int femm_newton_batch_variants_device(femm_plan *plan, int32_t items,
@@ -462,17 +481,20 @@ int femm_newton_batch_variants_device(femm_plan *plan, int32_t items,
                                       const double *warm_starts, double *a, double *flux,
                                       femm_item_result_t *results, void *stream)
 {
+  FEMM_API_BEGIN
   if (plan != nullptr && items == 0)
     return FEMM_OK;
   if (plan == nullptr || variant_of_item == nullptr || plan->dev.variants.count == 0)
     return FEMM_INVALID_ARGUMENT;
   return SolveDevice(plan, items, variant_of_item, currents, options, config, warm_starts, a,
                      flux, results, stream);
+  FEMM_API_END
 }
 
 int femm_plan_autotune(femm_plan *plan, int32_t items, const femm_solve_config_t *config,
                        int32_t *partitions)
 {
+  FEMM_API_BEGIN
   if (plan == nullptr || items <= 0 || partitions == nullptr)
     return FEMM_INVALID_ARGUMENT;
   const femm_solve_config_t c = config ? *config : DefaultConfig();
@@ -502,10 +524,12 @@ int femm_plan_autotune(femm_plan *plan, int32_t items, const femm_solve_config_t
   tuned.emplace_back(items, best);
   *partitions = best;
   return FEMM_OK;
+  FEMM_API_END
 }
 
 int femm_plan_set_variants(femm_plan *plan, int32_t count, const femm_model_t *variants)
 {
+  FEMM_API_BEGIN
   if (plan == nullptr || count < 0 || (count > 0 && variants == nullptr))
     return FEMM_INVALID_ARGUMENT;
   int st = EnsurePlanDevice(plan);
@@ -551,6 +575,7 @@ int femm_plan_set_variants(femm_plan *plan, int32_t count, const femm_model_t *v
     }
   }
   return hipDeviceSynchronize() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
+  FEMM_API_END
 }
 
 }  // extern "C"
@@ -624,8 +649,10 @@ int femm_newton_batch(femm_plan *plan, int32_t items, const double *currents,
                       const double *warm_starts, double *a, double *bx, double *by, double *flux,
                       femm_item_result_t *results)
 {
+  FEMM_API_BEGIN
   return SolveHost(plan, items, nullptr, currents, options, config, warm_starts, a, bx, by, flux,
                    results);
+  FEMM_API_END
 }
 
 int femm_newton_batch_safeguarded(femm_plan *plan, int32_t items, const double *currents,
@@ -634,8 +661,10 @@ int femm_newton_batch_safeguarded(femm_plan *plan, int32_t items, const double *
                                   double *a, double *bx, double *by, double *flux,
                                   femm_item_result_t *results)
 {
+  FEMM_API_BEGIN
   return SolveHost(plan, items, nullptr, currents, options, config, warm_starts, a, bx, by, flux,
                    results, true);
+  FEMM_API_END
 }
 
 int femm_newton_batch_variants(femm_plan *plan, int32_t items, const int32_t *variant_of_item,
@@ -643,12 +672,14 @@ int femm_newton_batch_variants(femm_plan *plan, int32_t items, const int32_t *va
                                const femm_solve_config_t *config, const double *warm_starts,
                                double *a, double *flux, femm_item_result_t *results)
 {
+  FEMM_API_BEGIN
   if (plan != nullptr && items == 0)
     return FEMM_OK;
   if (plan == nullptr || variant_of_item == nullptr || plan->dev.variants.count == 0)
     return FEMM_INVALID_ARGUMENT;
   return SolveHost(plan, items, variant_of_item, currents, options, config, warm_starts, a,
                    nullptr, nullptr, flux, results);
+  FEMM_API_END
 }
 
 int femm_tangent_multi_rhs(femm_plan *plan, const double *base_a, const double *base_currents,
@@ -656,6 +687,7 @@ int femm_tangent_multi_rhs(femm_plan *plan, const double *base_a, const double *
                            const femm_newton_options_t *options, double *da, double *dbx,
                            double *dby, double *dflux, femm_item_result_t *results)
 {
+  FEMM_API_BEGIN
   if (plan == nullptr || base_a == nullptr || base_currents == nullptr || directions <= 0 ||
       direction_currents == nullptr || options == nullptr || da == nullptr || dflux == nullptr ||
       results == nullptr)
@@ -732,11 +764,13 @@ int femm_tangent_multi_rhs(femm_plan *plan, const double *base_a, const double *
                       hipSuccess)))
     return FEMM_INTERNAL_ERROR;
   return FEMM_OK;
+  FEMM_API_END
 }
 
 int femm_finalize_batch(femm_plan *plan, int32_t items, const double *states, double *bx,
                         double *by, double *flux)
 {
+  FEMM_API_BEGIN
   if (plan == nullptr || items <= 0 || states == nullptr)
     return FEMM_INVALID_ARGUMENT;
   const femm::HostPlan &h = plan->host;
@@ -764,10 +798,12 @@ int femm_finalize_batch(femm_plan *plan, int32_t items, const double *states, do
                    hipSuccess))
     return FEMM_INTERNAL_ERROR;
   return FEMM_OK;
+  FEMM_API_END
 }
 
 int femm_plan_set_postprocess(femm_plan *plan, const femm_postprocess_options_t *options)
 {
+  FEMM_API_BEGIN
   if (plan == nullptr || options == nullptr)
     return FEMM_INVALID_ARGUMENT;
   int st = EnsurePlanDevice(plan);
@@ -787,29 +823,35 @@ int femm_plan_set_postprocess(femm_plan *plan, const femm_postprocess_options_t 
     return st;
   plan->post_ready = true;
   return FEMM_OK;
+  FEMM_API_END
 }
 
 int femm_postprocess_copy_mask(const femm_plan *plan, double *mask)
 {
+  FEMM_API_BEGIN
   if (plan == nullptr || mask == nullptr || !plan->post_ready)
     return FEMM_INVALID_ARGUMENT;
   std::copy(plan->post.mask.begin(), plan->post.mask.end(), mask);
   return FEMM_OK;
+  FEMM_API_END
 }
 
 int femm_postprocess_batch_device(femm_plan *plan, int32_t items, const double *states,
                                   double *loads, double *radial_b, void *stream)
 {
+  FEMM_API_BEGIN
   if (plan == nullptr || items <= 0 || states == nullptr || loads == nullptr ||
       (radial_b == nullptr && plan->post_dev.nprobe > 0) || !plan->post_ready)
     return FEMM_INVALID_ARGUMENT;
   return femm::LaunchPostprocess(plan->post_dev, items, states, loads, radial_b,
                                  static_cast<hipStream_t>(stream));
+  FEMM_API_END
 }
 
 int femm_postprocess_batch(femm_plan *plan, int32_t items, const double *states, double *loads,
                            double *radial_b)
 {
+  FEMM_API_BEGIN
   if (plan == nullptr || items <= 0 || states == nullptr || loads == nullptr || !plan->post_ready)
     return FEMM_INVALID_ARGUMENT;
   const femm::HostPlan &h = plan->host;
@@ -833,6 +875,7 @@ int femm_postprocess_batch(femm_plan *plan, int32_t items, const double *states,
        hipMemcpy(radial_b, d_p, sizeof(double) * B * P, hipMemcpyDeviceToHost) != hipSuccess))
     return FEMM_INTERNAL_ERROR;
   return FEMM_OK;
+  FEMM_API_END
 }
 
 const char *femm_status_name(int32_t status)
@@ -844,6 +887,7 @@ const char *femm_status_name(int32_t status)
 
 int femm_device_info(char *text, size_t capacity)
 {
+  FEMM_API_BEGIN
   if (text == nullptr || capacity == 0)
     return FEMM_INVALID_ARGUMENT;
   hipDeviceProp_t prop;
@@ -857,6 +901,7 @@ int femm_device_info(char *text, size_t capacity)
                 prop.multiProcessorCount, prop.totalGlobalMem / 1073741824.0,
                 prop.l2CacheSize / 1024);
   return FEMM_OK;
+  FEMM_API_END
 }
 
 }  // extern "C"

@@ -10,6 +10,7 @@
 #include <new>
 #include <vector>
 
+#include "api_guard.hpp"
 #include "device.hpp"
 #include "femm_gpu_amd.h"
 #include "handle.hpp"
@@ -79,6 +80,7 @@ extern "C" {
 
 int femm_group_create(int32_t count, const femm_group_entry_t *entries, femm_group **out)
 {
+  FEMM_API_BEGIN
   if (out == nullptr)
     return FEMM_INVALID_ARGUMENT;
   *out = nullptr;
@@ -91,10 +93,12 @@ int femm_group_create(int32_t count, const femm_group_entry_t *entries, femm_gro
   group->entries.assign(entries, entries + count);
   *out = group;
   return FEMM_OK;
+  FEMM_API_END
 }
 
 void femm_group_destroy(femm_group *group)
 {
+  FEMM_API_BEGIN
   if (group == nullptr)
     return;
   if (group->table != nullptr) {
@@ -102,10 +106,12 @@ void femm_group_destroy(femm_group *group)
     (void)hipFree(group->table);
   }
   delete group;
+  FEMM_API_END_VOID
 }
 
 int femm_group_set_partitions(femm_group *group, const int32_t *partitions)
 {
+  FEMM_API_BEGIN
   if (group == nullptr || partitions == nullptr)
     return FEMM_INVALID_ARGUMENT;
   for (size_t e = 0; e < group->entries.size(); ++e)
@@ -114,11 +120,13 @@ int femm_group_set_partitions(femm_group *group, const int32_t *partitions)
   for (size_t e = 0; e < group->entries.size(); ++e)
     group->entries[e].partitions = partitions[e];
   return FEMM_OK;
+  FEMM_API_END
 }
 
 int femm_group_autotune(femm_group *group, const femm_solve_config_t *config,
                         const double *work, int32_t *partitions, int32_t *blocks)
 {
+  FEMM_API_BEGIN
   if (group == nullptr)
     return FEMM_INVALID_ARGUMENT;
   if (!femm::DevicePresent())
@@ -191,6 +199,7 @@ int femm_group_autotune(femm_group *group, const femm_solve_config_t *config,
   if (blocks != nullptr)
     *blocks = static_cast<int32_t>(used);
   return FEMM_OK;
+  FEMM_API_END
 }
 
 // One group solve: the reference driver followed by the bounded recovery, or the safeguarded
@@ -235,13 +244,17 @@ static int GroupSolve(femm_group *group, const femm_newton_options_t *options,
 int femm_newton_group(femm_group *group, const femm_newton_options_t *options,
                       const femm_solve_config_t *config, void *stream)
 {
+  FEMM_API_BEGIN
   return GroupSolve(group, options, config, stream, false);
+  FEMM_API_END
 }
 
 int femm_newton_group_safeguarded(femm_group *group, const femm_newton_options_t *options,
                                   const femm_solve_config_t *config, void *stream)
 {
+  FEMM_API_BEGIN
   return GroupSolve(group, options, config, stream, true);
+  FEMM_API_END
 }
 
 }  // extern "C"

@@ -24,6 +24,7 @@
 #include <random>
 #include <vector>
 
+#include "api_guard.hpp"
 #include "device.hpp"
 #include "femm_gpu_amd.h"
 #include "handle.hpp"
@@ -567,6 +568,7 @@ extern "C" {
 int femm_warm_start_create(femm_plan *plan, int32_t inputs, int32_t modes, int32_t hidden,
                            uint64_t seed, femm_warm_start **out)
 {
+  FEMM_API_BEGIN
   if (out == nullptr || plan == nullptr || inputs <= 0 || modes <= 0 || modes > 1024)
     return FEMM_INVALID_ARGUMENT;
   *out = nullptr;
@@ -587,20 +589,24 @@ int femm_warm_start_create(femm_plan *plan, int32_t inputs, int32_t modes, int32
   }
   *out = ws;
   return FEMM_OK;
+  FEMM_API_END
 }
 
 void femm_warm_start_destroy(femm_warm_start *ws)
 {
+  FEMM_API_BEGIN
   if (ws == nullptr)
     return;
   (void)hipDeviceSynchronize();
   delete ws;
+  FEMM_API_END_VOID
 }
 
 int femm_warm_start_fit_device(femm_warm_start *ws, int32_t samples, const double *features,
                                const double *states, const femm_warm_start_train_t *train,
                                void *stream)
 {
+  FEMM_API_BEGIN
   if (ws == nullptr || samples <= 0 || features == nullptr || states == nullptr ||
       train == nullptr || train->steps < 0 || train->batch <= 0 ||
       !(train->learning_rate > 0.0f) || !std::isfinite(train->learning_rate))
@@ -608,11 +614,13 @@ int femm_warm_start_fit_device(femm_warm_start *ws, int32_t samples, const doubl
   ws->fitted = false;
   ws->report = femm_warm_start_report_t{};
   return FitDevice(ws, samples, features, states, *train, static_cast<hipStream_t>(stream));
+  FEMM_API_END
 }
 
 int femm_warm_start_fit(femm_warm_start *ws, int32_t samples, const double *features,
                         const double *states, const femm_warm_start_train_t *train)
 {
+  FEMM_API_BEGIN
   if (ws == nullptr || samples <= 0 || features == nullptr || states == nullptr)
     return FEMM_INVALID_ARGUMENT;
   femm::DeviceArray<double> x, a;
@@ -628,19 +636,23 @@ int femm_warm_start_fit(femm_warm_start *ws, int32_t samples, const double *feat
   if (st == FEMM_OK && hipDeviceSynchronize() != hipSuccess)
     return FEMM_INTERNAL_ERROR;
   return st;
+  FEMM_API_END
 }
 
 int femm_warm_start_report(const femm_warm_start *ws, femm_warm_start_report_t *report)
 {
+  FEMM_API_BEGIN
   if (ws == nullptr || report == nullptr)
     return FEMM_INVALID_ARGUMENT;
   *report = ws->report;
   return ws->fitted ? FEMM_OK : FEMM_INVALID_ARGUMENT;
+  FEMM_API_END
 }
 
 int femm_warm_start_predict(femm_warm_start *ws, int32_t items, const double *features,
                             double *a0)
 {
+  FEMM_API_BEGIN
   if (ws == nullptr || items <= 0 || features == nullptr || a0 == nullptr)
     return FEMM_INVALID_ARGUMENT;
   if (!ws->fitted)
@@ -659,6 +671,7 @@ int femm_warm_start_predict(femm_warm_start *ws, int32_t items, const double *fe
                 hipMemcpyDeviceToHost) != hipSuccess)
     return FEMM_INTERNAL_ERROR;
   return FEMM_OK;
+  FEMM_API_END
 }
 
 int femm_newton_batch_learned_device(femm_plan *plan, femm_warm_start *ws, int32_t items,
@@ -668,6 +681,7 @@ int femm_newton_batch_learned_device(femm_plan *plan, femm_warm_start *ws, int32
                                      int32_t warm_newton_iterations, double *a, double *flux,
                                      femm_item_result_t *results, void *stream)
 {
+  FEMM_API_BEGIN
   if (plan != nullptr && ws != nullptr && ws->plan == plan && items == 0)
     return FEMM_OK;   // empty batch: empty result, nothing launched
   if (plan == nullptr || ws == nullptr || ws->plan != plan || items <= 0 || features == nullptr ||
@@ -710,6 +724,7 @@ int femm_newton_batch_learned_device(femm_plan *plan, femm_warm_start *ws, int32
   x.prior_mode = femm::kPriorRedoUnconverged;
   return femm::LaunchSafeguarded(plan->dev, items, currents, plain_opts, *config, nullptr, a, flux,
                                  results, s, x);
+  FEMM_API_END
 }
 
 int femm_newton_batch_learned(femm_plan *plan, femm_warm_start *ws, int32_t items,
@@ -718,6 +733,7 @@ int femm_newton_batch_learned(femm_plan *plan, femm_warm_start *ws, int32_t item
                               const femm_solve_config_t *config, int32_t warm_newton_iterations,
                               double *a, double *flux, femm_item_result_t *results)
 {
+  FEMM_API_BEGIN
   if (plan != nullptr && ws != nullptr && ws->plan == plan && items == 0)
     return FEMM_OK;   // empty batch: empty result, nothing touched
   if (plan == nullptr || ws == nullptr || items <= 0 || features == nullptr ||
@@ -753,6 +769,7 @@ int femm_newton_batch_learned(femm_plan *plan, femm_warm_start *ws, int32_t item
           hipSuccess)
     return FEMM_INTERNAL_ERROR;
   return FEMM_OK;
+  FEMM_API_END
 }
 
 }  // extern "C"

@@ -7,6 +7,7 @@
 // block; every output element is accumulated by one lane in a fixed K order, so
 // results are bitwise repeatable.
 
+#include "api_guard.hpp"
 #include "mlp.hpp"
 
 #include <algorithm>
@@ -446,6 +447,7 @@ extern "C" {
 
 int femm_mlp_create(int32_t inputs, int32_t hidden, int32_t outputs, uint64_t seed, femm_mlp **out)
 {
+  FEMM_API_BEGIN
   if (out == nullptr)
     return FEMM_INVALID_ARGUMENT;
   *out = nullptr;
@@ -461,18 +463,22 @@ int femm_mlp_create(int32_t inputs, int32_t hidden, int32_t outputs, uint64_t se
   }
   *out = p;
   return FEMM_OK;
+  FEMM_API_END
 }
 
 void femm_mlp_destroy(femm_mlp *p)
 {
+  FEMM_API_BEGIN
   if (p == nullptr)
     return;
   (void)hipDeviceSynchronize();
   delete p;
+  FEMM_API_END_VOID
 }
 
 int femm_mlp_layout(const femm_mlp *p, int64_t layout[10])
 {
+  FEMM_API_BEGIN
   if (p == nullptr || layout == nullptr)
     return FEMM_INVALID_ARGUMENT;
   const femm::MlpLayout &L = p->net.L;
@@ -481,34 +487,42 @@ int femm_mlp_layout(const femm_mlp *p, int64_t layout[10])
                          static_cast<int64_t>(L.w3), static_cast<int64_t>(L.b3)};
   std::copy(v, v + 10, layout);
   return FEMM_OK;
+  FEMM_API_END
 }
 
 int femm_mlp_get_parameters(const femm_mlp *p, float *params)
 {
+  FEMM_API_BEGIN
   if (p == nullptr || params == nullptr)
     return FEMM_INVALID_ARGUMENT;
   return p->net.GetParameters(params);
+  FEMM_API_END
 }
 
 int femm_mlp_set_parameters(femm_mlp *p, const float *params)
 {
+  FEMM_API_BEGIN
   if (p == nullptr || params == nullptr)
     return FEMM_INVALID_ARGUMENT;
   for (size_t i = 0; i < p->net.L.count; ++i)
     if (!std::isfinite(params[i]))
       return FEMM_INVALID_ARGUMENT;
   return p->net.SetParameters(params);
+  FEMM_API_END
 }
 
 int femm_mlp_set_standardization(femm_mlp *p, const double *mean, const double *stddev)
 {
+  FEMM_API_BEGIN
   if (p == nullptr || mean == nullptr || stddev == nullptr)
     return FEMM_INVALID_ARGUMENT;
   return p->net.SetStandardisation(mean, stddev);
+  FEMM_API_END
 }
 
 int femm_mlp_forward(femm_mlp *p, int32_t rows, const double *x, float *y)
 {
+  FEMM_API_BEGIN
   if (p == nullptr || rows <= 0 || x == nullptr || y == nullptr)
     return FEMM_INVALID_ARGUMENT;
   femm::Mlp &net = p->net;
@@ -529,11 +543,13 @@ int femm_mlp_forward(femm_mlp *p, int32_t rows, const double *x, float *y)
     for (int c = 0; c < net.L.out; ++c)
       y[static_cast<size_t>(r) * net.L.out + c] = full[static_cast<size_t>(r) * net.L.out_p + c];
   return FEMM_OK;
+  FEMM_API_END
 }
 
 int femm_mlp_gradients(femm_mlp *p, int32_t rows, const double *x, const float *target,
                        float *loss, float *gradients)
 {
+  FEMM_API_BEGIN
   if (p == nullptr || rows <= 0 || x == nullptr || target == nullptr)
     return FEMM_INVALID_ARGUMENT;
   femm::Mlp &net = p->net;
@@ -560,12 +576,14 @@ int femm_mlp_gradients(femm_mlp *p, int32_t rows, const double *x, const float *
                               hipMemcpyDeviceToHost) != hipSuccess))
     return FEMM_INTERNAL_ERROR;
   return FEMM_OK;
+  FEMM_API_END
 }
 
 int femm_mlp_train(femm_mlp *p, int32_t samples, const double *x, const float *target,
                    int32_t steps, int32_t batch, float learning_rate, uint64_t seed,
                    float *loss_history)
 {
+  FEMM_API_BEGIN
   if (p == nullptr || samples <= 0 || x == nullptr || target == nullptr || steps < 0 ||
       batch <= 0 || !(learning_rate > 0.0f) || !std::isfinite(learning_rate))
     return FEMM_INVALID_ARGUMENT;
@@ -588,6 +606,7 @@ int femm_mlp_train(femm_mlp *p, int32_t samples, const double *x, const float *t
            hipSuccess))
     return FEMM_INTERNAL_ERROR;
   return FEMM_OK;
+  FEMM_API_END
 }
 
 }  // extern "C"

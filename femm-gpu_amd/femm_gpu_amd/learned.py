@@ -10,6 +10,7 @@ MLP, trained on the GPU.  ``Mlp`` exposes the network alone (gradient checks, tr
 from __future__ import annotations
 
 import ctypes
+import sys
 
 import numpy as np
 
@@ -70,6 +71,10 @@ class Mlp:
             self._handle = ctypes.c_void_p()
 
     def __del__(self):
+        # At interpreter shutdown the HIP runtime may already be torn down: leave the
+        # handle to process exit instead of calling into the device from a finalizer.
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -144,6 +149,10 @@ class WarmStart:
             self._handle = ctypes.c_void_p()
 
     def __del__(self):
+        # At interpreter shutdown the HIP runtime may already be torn down: leave the
+        # handle to process exit instead of calling into the device from a finalizer.
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

@@ -101,6 +101,9 @@ def square_coil_ring(cells=64, seed=12044, side=0.2, coil_side=0.04, ring_in=0.0
 
 
 # Motor geometry (mm) of the 12-slot / 4-pole surface-PM sector.
+SHAFT_SLEEVE_MM = 11.0   # free space between the shaft (10 mm, A = 0) and the rotor iron
+AIRGAP_PROBE_MM = 34.5   # air-gap B probe circle (gap 34..35 mm)
+ROTOR_GROUP, AIR_GROUP = 1, 0
 _MOTOR_BANDS = [  # (r_start, r_end, intervals)
     (10.0, 30.0, 20),    # rotor iron
     (30.0, 34.0, 12),    # magnets / inter-pole air
@@ -125,11 +128,15 @@ class MotorFamily:
     `slot_phases` (circuit, sign) pairs.  The default (15, 45, 75) gives three full slots
     with phases A, -C, B; (0, 30, 60, 90) cuts the sector through slot centres, giving two
     half slots (A, and its anti-periodic image -A) and two full slots (-C, B), i.e. four
-    coil regions.  Rotor: iron below r = 30 mm and two parallel-magnetised NdFeB blocks
-    (mu_r 1.05) between 30 and 34 mm spanning rotor angles 9..45 and 45..81 deg with
-    magnetisation angles 27 and 63 deg (both turning with the rotor).  A = 0 on the shaft
-    (r = 10 mm) and outer (r = 75 mm) arcs; nodes on theta = 0 and 90 deg are anti-periodic
-    pairs."""
+    coil regions.  Rotor: a non-magnetic shaft sleeve (free space) between 10 and 11 mm,
+    iron from 11 to 30 mm and two parallel-magnetised NdFeB blocks (mu_r 1.05) between 30
+    and 34 mm spanning rotor angles 9..45 and 45..81 deg with magnetisation angles 27 and
+    63 deg (both turning with the rotor).  A = 0 on the shaft (r = 10 mm) and outer
+    (r = 75 mm) arcs; nodes on theta = 0 and 90 deg are anti-periodic pairs.
+
+    FEMM block groups: 0 free space (air gap, slot openings, inter-pole gaps, shaft sleeve),
+    1 the rotor body (iron and both magnets: the body the weighted-stress torque acts on;
+    it touches no Dirichlet node), 2 stator iron, 5 + k coil region k."""
 
     def __init__(self, angular_cells=480, seed=20251203, jitter=0.15, radial_scale=1,
                  slot_centres_deg=(15.0, 45.0, 75.0), slot_phases=((0, 1.0), (1, -1.0), (2, 1.0)),
@@ -154,7 +161,8 @@ class MotorFamily:
         interior = np.zeros((nr, nt), bool)
         interior[1:-1, 1:-1] = True
         # Keep every material-interface grid line straight so labels follow the geometry.
-        band_edges_mm = {b[0] for b in _MOTOR_BANDS} | {b[1] for b in _MOTOR_BANDS} | {38.0, 57.0}
+        band_edges_mm = ({b[0] for b in _MOTOR_BANDS} | {b[1] for b in _MOTOR_BANDS}
+                         | {SHAFT_SLEEVE_MM, 38.0, 57.0})
         r_mm = r_nodes * 1e3
         on_radial_edge = np.array([any(abs(r - e) < 1e-9 for e in band_edges_mm) for r in r_mm])
         interior[on_radial_edge, :] = False
@@ -206,7 +214,7 @@ class MotorFamily:
         rc, tc = self.rc, self.tc
         phi = tc - rotor_steps * self.cell_deg          # rotor-frame angle
         label = np.full(rc.shape, self.AIR, np.int32)
-        label[rc < 30] = self.ROTOR
+        label[(rc > SHAFT_SLEEVE_MM) & (rc < 30)] = self.ROTOR
         band = (rc > 30) & (rc < 34)
         label[band & (phi > 9) & (phi < 45)] = self.MAG1
         label[band & (phi > 45) & (phi < 81)] = self.MAG2
@@ -236,8 +244,8 @@ class MotorFamily:
         mats = [Material(1.0 / MU0, group=0),
                 Material(1.0, bh_curve=0, group=1),
                 Material(1.0, bh_curve=0, group=2),
-                Material(1.0 / (MU0 * 1.05), hc=hc[0], magnetization_deg=27.0 + turn, group=3),
-                Material(1.0 / (MU0 * 1.05), hc=hc[1], magnetization_deg=63.0 + turn, group=4)]
+                Material(1.0 / (MU0 * 1.05), hc=hc[0], magnetization_deg=27.0 + turn, group=1),
+                Material(1.0 / (MU0 * 1.05), hc=hc[1], magnetization_deg=63.0 + turn, group=1)]
         for k, (circuit, sign) in enumerate(self.slot_phases):
             coil_area = float(self.area[tlabel == self.COIL0 + k].sum())
             turns = sign * fill[k] * coil_area
@@ -423,3 +431,12 @@ def solenoid_rz(radial_cells=64, axial_cells=128, seed=4, plunger_mu=None):
     boundary = np.nonzero(~interior)[0].astype(np.int32)
     return (Model(xy, tri, label, mats, [(b, h)], boundary, np.zeros(len(boundary)), None, 1, 1.0,
                   AXISYMMETRIC, name=f"solenoid_rz_{nr}x{nz}_seed{seed}"), np.array([2e6]))
+
+
+def motor_postprocess(probes=32):
+    """Weighted-stress torque on the rotor body (group 1) against free space (group 0) and
+    radial air-gap B at `probes` angles (k + 1/2) * 90 / probes deg on the 34.5 mm circle:
+    keyword arguments of ``Plan.set_postprocess`` for every :class:`MotorFamily` model."""
+    angles = (np.arange(probes) + 0.5) * (0.5 * math.pi / probes)
+    return dict(selected_group=ROTOR_GROUP, air_group=AIR_GROUP,
+                radius_m=AIRGAP_PROBE_MM * 1e-3, angles_rad=angles)

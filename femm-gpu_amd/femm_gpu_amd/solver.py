@@ -13,6 +13,7 @@ target, loading fails loudly.  There is no CPU fallback.
 from __future__ import annotations
 
 import ctypes
+import sys
 from pathlib import Path
 
 import numpy as np
@@ -133,6 +134,10 @@ class Plan:
             self._handle = ctypes.c_void_p()
 
     def __del__(self):
+        # At interpreter shutdown the HIP runtime may already be torn down: leave the
+        # handle to process exit instead of calling into the device from a finalizer.
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -371,6 +376,10 @@ class Group:
             self._handle = ctypes.c_void_p()
 
     def __del__(self):
+        # At interpreter shutdown the HIP runtime may already be torn down: leave the
+        # handle to process exit instead of calling into the device from a finalizer.
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

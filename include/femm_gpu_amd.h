@@ -2,7 +2,9 @@
  * femm_gpu_amd: MI355X-native nonlinear magnetostatic solve path (C ABI).
  *
  * Plain pointers and sizes only.  Every function returns a status code of
- * femm_model.h (the reference's Status integers) and never throws.  A plan
+ * femm_model.h (the reference's Status integers) and never throws: a failed host
+ * or device allocation returns FEMM_GPU_ALLOCATION_FAILED, any other internal
+ * exception FEMM_INTERNAL_ERROR.  A plan
  * owns all device memory it allocates; host arrays stay owned by the caller.
  * Calls on one plan are single-thread-affine; a plan may be created on any
  * thread.  Host-pointer entry points return after results are on the host.

@@ -24,7 +24,7 @@ sys.path.insert(0, str(ROOT / "oracle"))
 import mlp_oracle as mo  # noqa: E402
 
 # sha256 prefix of the default motor_sector mesh (nodes, triangles, labels).
-DEFAULT_MOTOR_DIGEST = "dbf174dd20387f49"
+DEFAULT_MOTOR_DIGEST = "872cb79793dbea9a"
 
 
 def test_bf16_round_to_nearest_even_known_answers():

@@ -169,3 +169,37 @@ def test_empty_batch_is_an_empty_result_without_gpu():
     assert lib.femm_newton_batch(None, 0, null, null, null, null, null, null, null, null,
                                  null) == 2
     plan.close()
+
+
+_ALLOCATION_PROBE = r"""
+import ctypes, re, resource, sys
+sys.path[:0] = [sys.argv[1]]
+from femm_gpu_amd import meshes
+from femm_gpu_amd.solver import library
+lib = library()
+model, _ = meshes.square_coil_ring(cells=256)
+cm = model.c_struct()
+vm = int(re.search(r"VmSize:\s+(\d+) kB", open("/proc/self/status").read()).group(1)) * 1024
+resource.setrlimit(resource.RLIMIT_AS, (vm + (16 << 20), resource.RLIM_INFINITY))
+handle = ctypes.c_void_p()
+limited = lib.femm_plan_create(ctypes.byref(cm), ctypes.byref(handle))
+resource.setrlimit(resource.RLIMIT_AS, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+print("limited", limited, handle.value is None)
+again = lib.femm_plan_create(ctypes.byref(cm), ctypes.byref(handle))
+print("unlimited", again, handle.value is not None)
+lib.femm_plan_destroy(handle)
+"""
+
+
+def test_host_allocation_failure_is_a_status_not_an_abort():
+    """A std::bad_alloc inside femm_plan_create (a 66k-node model under an address-space
+    limit that leaves 16 MiB of headroom) must come back as FEMM_GPU_ALLOCATION_FAILED with
+    no handle, not cross the C ABI; the same call then succeeds without the limit."""
+    import subprocess
+    import sys
+    out = subprocess.run([sys.executable, "-c", _ALLOCATION_PROBE, str(ROOT / "femm-gpu_amd")],
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr[-2000:]
+    lines = dict(line.split(" ", 1) for line in out.stdout.strip().splitlines())
+    assert lines["limited"] == "7 True"
+    assert lines["unlimited"] == "0 True"
