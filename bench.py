@@ -274,12 +274,12 @@ def main():
         hist = {int(q): int(v) for q, v in zip(*np.unique(newton, return_counts=True))}
         return dict(ms=ms, solve_ms=solve_ms, postprocess_ms=post_ms, wall=wall,
                     steps=steps, warmup=warmup, passing=int(passing.sum()), newton_hist=hist,
-                    solves_per_s=float(passing.sum()) / (ms * 1e-3),
+                    solves_per_s=float(passing.sum()) / (ms * 1e-3) if ms > 0 else None,
                     newton_mean=float(r["newton_iterations"].mean()),
                     pcg_mean=float(r["linear_iterations"].mean()),
                     pcg_max=int(r["linear_iterations"].max()),
                     max_equation_residual=float(r["equation_residual"].max()),
-                    achieved_gbs=bytes_total / (solve_ms * 1e-3) / 1e9,
+                    achieved_gbs=bytes_total / (solve_ms * 1e-3) / 1e9 if solve_ms > 0 else None,
                     pcg_iter_bytes=pcg_iter_bytes,
                     a=a.clone(), flux=flux.clone(), loads=loads.clone(), radial=radial.clone())
 

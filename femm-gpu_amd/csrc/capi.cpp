@@ -79,6 +79,7 @@ femm_solve_config_t DefaultConfig()
   c.block_size = 1;
   c.split_newton_iterations = 0;
   c.inexact_forcing = 0.0;
+  c.acceptance_linear_tolerance = 0.0;
   return c;
 }
 
@@ -163,7 +164,8 @@ bool ConfigValid(const femm_solve_config_t &c)
                         (c.precision == FEMM_PRECISION_FP64 &&
                          (c.block_size == 2 || c.block_size == 8));
   return (c.precision == FEMM_PRECISION_FP64 || c.precision == FEMM_PRECISION_MIXED) && block_ok &&
-         c.split_newton_iterations >= 0 && c.inexact_forcing >= 0.0 && c.inexact_forcing < 1.0;
+         c.split_newton_iterations >= 0 && c.inexact_forcing >= 0.0 && c.inexact_forcing < 1.0 &&
+         c.acceptance_linear_tolerance >= 0.0 && std::isfinite(c.acceptance_linear_tolerance);
 }
 
 femm_newton_options_t DefaultNewtonOptions() { return DefaultOptions(); }

@@ -108,6 +108,8 @@ struct NewtonArgs {
   femm_solve_config_t config;
   int32_t prior_mode;
   femm_newton_options_t warm_options;   // learned mode: the warm attempt's options
+  int32_t resume;                       // safeguarded list launch: 1 = continue each item
+                                        // from its state in A, 0 = from the plain start
 };
 
 // Which items a launch with prior results solves again (`prior` may alias `results`;

@@ -705,8 +705,10 @@ int femm_newton_batch_learned_device(femm_plan *plan, femm_warm_start *ws, int32
     return st;
   // Per item, the safeguarded Newton (full steps, Armijo on the equation residual, both
   // acceptance gates) from the predicted start within the warm budget; then, in a second
-  // launch on the same stream, the reference driver from the plain start for the items
-  // that attempt did not accept.
+  // launch on the same stream, the same safeguarded policy from the plain start with the
+  // full budget for the items that attempt did not accept.  That policy is the reference's
+  // own recovery (SolveSafeguarded, nonlinear_solver.cu:534-659), so no further recovery
+  // launch follows.
   femm_newton_options_t warm_opts = *options;
   warm_opts.require_equation_residual = 1;
   warm_opts.max_newton_iterations = std::min(warm_newton_iterations, options->max_newton_iterations);
@@ -714,16 +716,8 @@ int femm_newton_batch_learned_device(femm_plan *plan, femm_warm_start *ws, int32
   plain_opts.require_equation_residual = 1;
   femm::LaunchExtras x;
   x.variant_of_item = variant_of_item;
-  if ((st = femm::LaunchLearned(plan->dev, items, currents, warm_opts, plain_opts, *config,
-                                ws->a0.get(), a, flux, results, s, x)) != FEMM_OK)
-    return st;
-  if (!options->recover_nonconvergence || options->max_newton_iterations <= 0)
-    return FEMM_OK;
-  // Bounded recovery of items the plain driver left unconverged.
-  x.prior = results;
-  x.prior_mode = femm::kPriorRedoUnconverged;
-  return femm::LaunchSafeguarded(plan->dev, items, currents, plain_opts, *config, nullptr, a, flux,
-                                 results, s, x);
+  return femm::LaunchLearned(plan->dev, items, currents, warm_opts, plain_opts, *config,
+                             ws->a0.get(), a, flux, results, s, x);
   FEMM_API_END
 }
 

@@ -1746,6 +1746,22 @@ __device__ __attribute__((always_inline)) double ForcingTolerance(const femm_new
   return fmax(exact, fmin(1e-3, cfg.inexact_forcing * rel_residual));
 }
 
+// Safeguarded acceptance: the linearization that meets both gates is solved to
+// cfg.acceptance_linear_tolerance when that is tighter than the options' exact tolerance,
+// and the item is then returned at that candidate.
+__device__ __attribute__((always_inline)) bool TightAcceptance(const femm_newton_options_t &o,
+                                                                const femm_solve_config_t &cfg)
+{
+  return cfg.acceptance_linear_tolerance > 0.0 &&
+         cfg.acceptance_linear_tolerance < o.linear_relative_tolerance;
+}
+
+__device__ __attribute__((always_inline)) double AcceptTolerance(const femm_newton_options_t &o,
+                                                                  const femm_solve_config_t &cfg)
+{
+  return TightAcceptance(o, cfg) ? cfg.acceptance_linear_tolerance : o.linear_relative_tolerance;
+}
+
 // Inexact-Newton inner tolerance (physics/nonlinear_model.cpp:166-174).
 __device__ double LinearTolerance(const femm_newton_options_t &o, bool adaptive, double prev)
 {
@@ -1924,8 +1940,9 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
   bool eq_current = false;
   for (int it = 0; status == FEMM_NONLINEAR_SOLVE_NOT_CONVERGED; ++it) {
     if (safeguarded) {
-      // The assembly at the current state is current (start or accepted trial).
-      if (it >= o.max_newton_iterations)
+      // The assembly at the current state is current (start or accepted trial).  A pending
+      // re-solve finishes the step that met the gates, whatever the budget.
+      if (it >= o.max_newton_iterations && !exact_next)
         break;
       if (!isfinite(norm)) {
         status = FEMM_NUMERICAL_NONFINITE;
@@ -1954,10 +1971,12 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
       }
     }
     const double tol = !safeguarded ? LinearTolerance(o, adaptive, prev)
-                       : exact_next ? o.linear_relative_tolerance
+                       : exact_next ? AcceptTolerance(o, cfg)
                                     : ForcingTolerance(o, cfg, norm / scale);
+    // A re-solve of the same linearization starts from its last solution.
+    const bool resolve = exact_next;
     exact_next = false;
-    if (cfg.pcg_warm_start) {
+    if (cfg.pcg_warm_start && !resolve) {
       for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
         w.x[i] = a[P.free_nodes[i]];
       R.Sync();
@@ -1978,19 +1997,19 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
     if (safeguarded) {
       const bool gates =
           step < 100.0 * o.relative_tolerance && norm / scale <= o.relative_tolerance;
-      if (gates && tol > o.linear_relative_tolerance) {
-        // Both gates met on an inexact solve: solve this linearization again to the exact
-        // tolerance (the assembly is still the current state's) before accepting.
+      if (gates && tol > AcceptTolerance(o, cfg)) {
+        // Both gates met on an inexact solve: solve this linearization again to the
+        // acceptance tolerance (the assembly is still the current state's) before accepting.
         exact_next = true;
         continue;
       }
-      if (gates && !(cfg.inexact_forcing > 0.0)) {
+      if (gates && !(cfg.inexact_forcing > 0.0) && !TightAcceptance(o, cfg)) {
         status = FEMM_OK;
         break;
       }
-      // Armijo backtracking.  With inexact forcing, a step that met both gates is tried at
-      // alpha = 1 only: the item is accepted at that candidate if its own equation residual
-      // passes, else at the current state.
+      // Armijo backtracking.  With inexact forcing or a tight acceptance tolerance, a step
+      // that met both gates is tried at alpha = 1 only: the item is accepted at that
+      // candidate if its own equation residual passes, else at the current state.
       bool accepted = false;
       double alpha = 1.0;
       for (int bt = 0; bt < 20; ++bt, alpha *= 0.5) {
@@ -2195,11 +2214,12 @@ __host__ __device__ inline int ListTeams(int grid, int count, int nslices)
 
 // Solves the items ListItemsKernel listed, the launch's workgroups shared out evenly
 // (ListTeams per item; with more items than teams, each team takes items t, t + teams, ...),
-// counters added to the earlier launch's.  kModeSafeguarded resumes each item from the state
-// the capped launch left in A with the options' budget (the rest of the capped attempt's):
-// the state is copied to the item's trial buffer first, because the solve rewrites A with
-// the plain start (for the residual scale) before it takes the warm start, and it uses the
-// trial buffer only after that.  kModeNewton solves each item again from the plain start.
+// counters added to the earlier launch's.  kModeSafeguarded with args.resume continues each
+// item from the state the capped launch left in A with the options' budget (the rest of the
+// capped attempt's): the state is copied to the item's trial buffer first, because the solve
+// rewrites A with the plain start (for the residual scale) before it takes the warm start,
+// and it uses the trial buffer only after that.  Without args.resume, and in kModeNewton,
+// each item is solved again from the plain start.
 template <int kForm, int kMode>
 __global__ __launch_bounds__(kBlockThreads) void TeamListKernel(
     DevicePlan base, DeviceWork W, const double *currents, NewtonArgs args, double *A,
@@ -2227,7 +2247,7 @@ __global__ __launch_bounds__(kBlockThreads) void TeamListKernel(
                g,
                0};
     const double *start = nullptr;
-    if constexpr (kMode == kModeSafeguarded) {
+    if (kMode == kModeSafeguarded && args.resume) {
       const double *a = A + static_cast<size_t>(item) * P.N;
       double *keep = W.trial + static_cast<size_t>(item) * P.N;
       for (int v = R.S.node0 + threadIdx.x; v < R.S.node1; v += kBlockThreads)
@@ -2409,9 +2429,9 @@ int WithForm(int form, F &&f)
 
 // One solve form's launch: `partitions` > 1 runs the team kernel cooperatively, otherwise
 // one workgroup per item.  The learned mode is launches of the other modes: the
-// safeguarded attempt from `warm` with the warm options over every item, then the driver
-// from the plain start over the items that attempt did not accept, their counters added to
-// the attempt's.  (One kernel holding both attempts per item needs more registers than a
+// safeguarded attempt from `warm` with the warm options over every item, then the same
+// safeguarded policy from the plain start over the items that attempt did not accept,
+// their counters added to the attempt's.  (One kernel holding both attempts per item needs more registers than a
 // 512-thread workgroup has per lane and spills inside the PCG loop.)  With
 // one workgroup per item, the plain-start redo runs the rejected items only, in teams that
 // share the whole GPU (TeamListKernel), so a few fallbacks do not run on a few compute units
@@ -2442,20 +2462,24 @@ int LaunchFormSolve(const DeviceBuffers &dev, int32_t items, int32_t partitions,
     if (resume) {
       NewtonArgs rest = warm_args;
       rest.options.max_newton_iterations = budget - split;
+      rest.resume = 1;
       st = LaunchList<kForm, kModeSafeguarded>(dev, items, split, currents, rest, a, flux,
                                                results, stream, x);
       if (st != FEMM_OK)
         return st;
     }
+    // The items the attempt did not accept: the same safeguarded policy (the reference's
+    // SolveSafeguarded, its own recovery) from the plain start with the full budget.
+    NewtonArgs plain_args = args;
+    plain_args.resume = 0;
     if (partitions <= 1)
-      return LaunchList<kForm, kModeNewton>(dev, items, 0, currents, args, a, flux, results,
-                                            stream, x);
+      return LaunchList<kForm, kModeSafeguarded>(dev, items, 0, currents, plain_args, a, flux,
+                                                 results, stream, x);
     LaunchExtras redo = x;
     redo.prior = results;
-    NewtonArgs plain_args = args;
     plain_args.prior_mode = kPriorRedoFailed;
-    return LaunchFormSolve<kForm, kModeNewton>(dev, items, partitions, currents, plain_args,
-                                               nullptr, a, flux, results, stream, redo);
+    return LaunchFormSolve<kForm, kModeSafeguarded>(dev, items, partitions, currents, plain_args,
+                                                    nullptr, a, flux, results, stream, redo);
   } else {
     if (partitions > 1)
       return LaunchTeams<kForm, kMode>(dev, items, partitions, currents, args, warm, a, flux,

@@ -34,7 +34,8 @@ class SolveConfig(ctypes.Structure):
                 ("verify_equation_residual", ctypes.c_int32), ("max_refinements", ctypes.c_int32),
                 ("inner_relative_tolerance", ctypes.c_double), ("partitions", ctypes.c_int32),
                 ("block_size", ctypes.c_int32), ("split_newton_iterations", ctypes.c_int32),
-                ("inexact_forcing", ctypes.c_double)]
+                ("inexact_forcing", ctypes.c_double),
+                ("acceptance_linear_tolerance", ctypes.c_double)]
 
 
 class ItemResult(ctypes.Structure):
@@ -68,9 +69,13 @@ def reference_config(**kw) -> SolveConfig:
 
 def mixed_config(**kw) -> SolveConfig:
     """FP32 Jacobi PCG with FP64 reliable updates (true residual replaced whenever the FP32
-    residual has dropped 10x), each Newton correction starting from the current state."""
+    residual has dropped 10x), each Newton correction starting from the current state.  In
+    the safeguarded policy (learned solves, femm_newton_batch_safeguarded) the step that meets
+    both gates is solved again to 1e-11 and returned at that candidate, so the state is fixed
+    to about 1e-11 times the operator's conditioning; the reference driver ignores it."""
     values = dict(precision=MIXED, pcg_warm_start=1, verify_equation_residual=1,
-                  max_refinements=64, inner_relative_tolerance=0.1, partitions=1, block_size=1)
+                  max_refinements=64, inner_relative_tolerance=0.1, partitions=1, block_size=1,
+                  acceptance_linear_tolerance=1e-11)
     values.update(kw)
     return SolveConfig(**values)
 

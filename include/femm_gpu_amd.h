@@ -62,6 +62,16 @@ typedef struct femm_solve_config {
                                   both gates is solved again to the exact tolerance before the
                                   item is accepted (at the exact candidate when its own
                                   equation residual passes). */
+  double acceptance_linear_tolerance; /* safeguarded Newton: 0 = the step that meets both
+                                  gates is accepted as the reference's SolveSafeguarded does
+                                  (at the current state).  t > 0 (at most the options'
+                                  linear_relative_tolerance) = the linearization that met the
+                                  gates is solved again to t, starting from its last solution
+                                  when pcg_warm_start is set, and the item is returned at that
+                                  candidate when its own equation residual passes; the state
+                                  is then fixed to about t times the operator's conditioning
+                                  rather than to the gates' resolution.  No counterpart in the
+                                  reference. */
 } femm_solve_config_t;
 
 /* Per-item outcome (reference SolveInfo plus the Newton counters of
@@ -380,10 +390,10 @@ int femm_warm_start_predict(femm_warm_start *ws, int32_t items, const double *fe
 /* Batched solve from the learned start: the reference's safeguarded Newton (full steps,
  * Armijo backtracking on the equation residual, SolveSafeguarded,
  * nonlinear_solver.cu:534-659) with at most warm_newton_iterations iterations and both
- * acceptance gates; every item not accepted there is solved again from the plain start by
- * the reference driver (NonlinearP1Solver::SolveBatch policy) in the same call, followed by
- * the bounded recovery when options->recover_nonconvergence is set.  Counters of the
- * attempts add up per item. */
+ * acceptance gates; every item not accepted there is solved again in the same call from
+ * the plain start by the same safeguarded policy with options->max_newton_iterations (the
+ * reference's own recovery path), so every item is accepted under one policy.  Counters of
+ * the attempts add up per item. */
 int femm_newton_batch_learned(femm_plan *plan, femm_warm_start *ws, int32_t items,
                               const int32_t *variant_of_item, const double *features,
                               const double *currents, const femm_newton_options_t *options,

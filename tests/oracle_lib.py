@@ -223,6 +223,32 @@ def safeguarded_solve(model, currents, options, warm_start=None, forcing=0.0):
                 step=res.relative_step, eq=res.equation_residual)
 
 
+# Past the acceptance gates: the reference's SolveSafeguarded policy in FP64 with the
+# equation-residual gate at 1e-11 (step gate 1e-9) and every linear solve to 1e-12.
+CONVERGED_OPTIONS = dict(relative_tolerance=1e-11, linear_relative_tolerance=1e-12,
+                         recover_nonconvergence=0)
+
+
+def converged_solve(model, currents, start=None):
+    """The oracle's solution continued past the gates: the reference driver's accepted state
+    (or `start`), then safeguarded Newton with CONVERGED_OPTIONS; adds bx / by / flux."""
+    from femm_gpu_amd.model import accepted_options
+    if start is None:
+        start = newton_solve(model, currents, accepted_options())["a"]
+    r = safeguarded_solve(model, currents, accepted_options(**CONVERGED_OPTIONS), warm_start=start)
+    st, bx, by, flux = finalize(model, r["a"])
+    r.update(bx=bx, by=by, flux=flux, finalize_status=st)
+    return r
+
+
+def converged_batch(model, currents, threads=4):
+    """converged_solve for every row of currents, `threads` items at a time."""
+    from concurrent.futures import ThreadPoolExecutor
+    currents = np.ascontiguousarray(currents, np.float64).reshape(-1, model.circuit_count)
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as pool:
+        return list(pool.map(lambda c: converged_solve(model, c), list(currents)))
+
+
 def tangent(model, base_a, base_currents, directions, options):
     """SolveTangentMultiRhs restated: da[D][N], dflux[D][C], per-direction status / iterations."""
     N, C = model.node_count, model.circuit_count
