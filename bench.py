@@ -373,6 +373,7 @@ def main():
                                   a.data_ptr(), flux.data_ptr(), res.data_ptr(), stream, options,
                                   cfg, WARM_BUDGET)
 
+        cfg_accept = cfg.acceptance_linear_tolerance
         forms["learned_mixed"] = collect(timer.run([learned_solve, postprocess], args.steps,
                                                    args.warmup), True, args.block_size,
                                          args.steps, args.warmup)
@@ -440,7 +441,6 @@ def main():
         cpu = cpu_baseline(model, currents, options)
         print(f"# cpu baseline {cpu}", file=sys.stderr, flush=True)
 
-    workload = f"motor_sector N={plan.N} T={plan.T} batch {B}"
 
     def summary(f):
         return {"solves_per_s": f["solves_per_s"], "ms_per_step": f["ms"],
@@ -486,8 +486,9 @@ def main():
                    "training": training},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(f"learned_mixed block {args.block_size}",
-                                                 workload),
+                     "traffic": measured_traffic(
+                         f"learned_mixed block {args.block_size} acceptance "
+                         f"{cfg_accept:g}", f"{model.name} batch {B}"),
                      "kernel": f"SolveKernel<{3 if args.block_size > 1 else 1}, 1> (mixed "
                                "form, safeguarded attempt from the learned start; the plain-"
                                "start redo launch of rejected items is timed with it; the "
