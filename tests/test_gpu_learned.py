@@ -199,11 +199,12 @@ def test_learned_warm_start_gains_and_matches_plain():
                            warm_iterations=12)
     assert np.array_equal(again["a"], learned["a"])
     # A zero warm budget: every item is solved from the plain start by the fallback (the
-    # safeguarded policy), which runs the rejected items in teams of workgroups (summation
-    # order differs from the one-workgroup safeguarded solve only).
-    fb = ws.solve_batch(test_cur[:4], test_cur[:4], accepted_options(), reference_config(),
-                        warm_iterations=0)
-    sg = plan.solve_batch(test_cur[:4], accepted_options(), reference_config(), safeguarded=True)
+    # safeguarded policy, here in FP64 with the 1e-11 acceptance solve), which runs the
+    # rejected items in teams of workgroups (summation order differs from the one-workgroup
+    # safeguarded solve only).
+    tight = reference_config(acceptance_linear_tolerance=1e-11)
+    fb = ws.solve_batch(test_cur[:4], test_cur[:4], accepted_options(), tight, warm_iterations=0)
+    sg = plan.solve_batch(test_cur[:4], accepted_options(), tight, safeguarded=True)
     for b in range(4):
         assert fb["results"][b]["status"] == 0
         assert abs(int(fb["results"][b]["newton_iterations"]) -
