@@ -205,65 +205,12 @@ class Model:
 
 
 def load_artifact(path):
-    """Load a ``gpu_femm_magnetostatic_mesh_v1`` / ``gpu_femm_planar_dc_mesh_v1`` artifact.
-
-    Follows the unit and label mapping of gpu_solver/src/io/mesh_artifact.cpp:198-418:
-    mm -> m, nu0 = 1/(mu0 mu_r), base materials first and then one expanded material per
-    region (group, circuit, PM angle, J_per_A = turns / region area), zero outer
-    Dirichlet, signed node constraints.  Returns (Model, circuit currents).
-    """
-    import json
-
-    with open(path, "r", encoding="utf-8") as handle:
-        doc = json.load(handle)
-    schema = doc["schema_version"]
-    if schema not in ("gpu_femm_magnetostatic_mesh_v1", "gpu_femm_planar_dc_mesh_v1"):
-        raise ValueError(f"unsupported artifact schema {schema}")
-    res = doc["resolved"]
-    model_info = res["model"]
-    if model_info["frequency_hz"] != 0:
-        raise ValueError("unsupported model physics")
-    problem = AXISYMMETRIC if model_info["problem_type"] == "axisymmetric" else PLANAR
-    nodes = np.asarray(res["nodes_mm"], np.float64) * 1e-3
-    materials, curves, lookup = [], [], {}
-    for entry in res["materials"]:
-        if entry["mu_x"] != entry["mu_y"] or entry["lam_type"] != 0 or entry["lam_fill"] != 1:
-            raise ValueError("invalid material definition")
-        mat = Material(1.0 / (MU0 * entry["mu_x"]), hc=float(entry["H_c_A_per_m"]))
-        if entry["B_T"]:
-            mat.bh_curve = len(curves)
-            curves.append((np.asarray(entry["B_T"], np.float64),
-                           np.asarray(entry["H_A_per_m"], np.float64)))
-        lookup[entry["id"]] = len(materials)
-        materials.append(mat)
-    currents = [float(c["current_A"]) for c in res["circuits"]]
-    regions = res["regions"]
-    region_index = {r["id"]: i for i, r in enumerate(regions)}
-    tris = np.asarray(res["triangles"]["node_indices"], np.int32)
-    tri_region = np.asarray([region_index[r] for r in res["triangles"]["region_ids"]], np.int32)
-    p = nodes[tris]
-    area = 0.5 * ((p[:, 1, 0] - p[:, 0, 0]) * (p[:, 2, 1] - p[:, 0, 1])
-                  - (p[:, 2, 0] - p[:, 0, 0]) * (p[:, 1, 1] - p[:, 0, 1]))
-    if np.any(area <= 0):
-        raise ValueError("triangle is not CCW")
-    region_area = np.zeros(len(regions))
-    for e in range(len(tris)):   # element order, as the reference sums areas
-        region_area[tri_region[e]] += area[e]
-    base = len(materials)
-    for i, r in enumerate(regions):
-        src = materials[lookup[r["material_id"]]]
-        circuit = int(r["circuit_index"])
-        materials.append(Material(src.reluctivity_zero, src.alpha,
-                                  0.0 if circuit < 0 else r["turns"] / region_area[i], src.hc,
-                                  float(r["pm_magnetization_deg"]), src.bh_curve, circuit,
-                                  int(r["group_number"])))
-    od = res["outer_dirichlet"]
-    cons = []
-    for c in res.get("node_constraints", []):
-        cons.append((c["node_a"], c["node_b"], 1 if c["relation"] == "periodic" else -1))
-    model = Model(nodes, tris, base + tri_region, materials, curves,
-                  np.asarray(od["node_indices"], np.int32),
-                  np.asarray(od["A_Wb_per_m"], np.float64),
-                  np.asarray(cons, np.int32).reshape(-1, 3), len(currents),
-                  float(model_info["depth_mm"]) * 1e-3, problem, name=str(path))
-    return model, np.asarray(currents, np.float64)
+    """Load a mesh artifact (``gpu_femm_magnetostatic_mesh_v1``, ``gpu_femm_planar_dc_mesh_v1``
+    or ``gpu_femm_mesh_v1``) with the library's strict parser (include/femm_wire.h,
+    femm-gpu_amd/csrc/host/artifact.cpp; reference gpu_solver/src/io/mesh_artifact.cpp:14-440):
+    mm -> m, nu0 = 1/(mu0 mu_r), base materials first and then one material label per region
+    (group, circuit, PM angle, J_per_A = turns / region area), zero outer Dirichlet, signed node
+    constraints; duplicate keys, unknown fields, a wrong canonical identity and every other
+    rule violation raise ``wire.ArtifactError``.  Returns (Model, circuit currents)."""
+    from .wire import load_artifact as strict_load
+    return strict_load(path)

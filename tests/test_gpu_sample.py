@@ -59,33 +59,6 @@ def test_generic_sample_request_reproduces_frozen_femm(tmp_path, name, pot_l2, f
     assert doc["convergence"]["residual_l2"] < 1e-6
 
 
-def _planar_dc_artifact(path, model, current, turns):
-    """Write the frozen PM / coil model as a gpu_femm_planar_dc_mesh_v1 artifact: one region
-    per label with group number = label, the coil's turns chosen so J_per_A = turns / area."""
-    from femm_gpu_amd.model import MU0
-
-    mats, regions = [], []
-    for i, m in enumerate(model.materials):
-        b, h = (model.curves[m.bh_curve] if m.bh_curve >= 0 else ([], []))
-        mats.append(dict(id=f"m{i}", mu_x=1.0 / (MU0 * m.reluctivity_zero),
-                         mu_y=1.0 / (MU0 * m.reluctivity_zero), lam_type=0, lam_fill=1,
-                         H_c_A_per_m=m.hc, B_T=[float(x) for x in b],
-                         H_A_per_m=[float(x) for x in h]))
-        regions.append(dict(id=f"r{i}", material_id=f"m{i}", circuit_index=m.circuit,
-                            turns=turns if m.circuit >= 0 else 0.0,
-                            pm_magnetization_deg=m.magnetization_deg, group_number=i))
-    doc = dict(schema_version="gpu_femm_planar_dc_mesh_v1", resolved=dict(
-        model=dict(frequency_hz=0, problem_type="planar", depth_mm=model.depth_m * 1e3),
-        nodes_mm=(model.node_xy * 1e3).tolist(), materials=mats,
-        circuits=[dict(current_A=current)], regions=regions,
-        triangles=dict(node_indices=model.triangles.tolist(),
-                       region_ids=[f"r{k}" for k in model.triangle_material]),
-        outer_dirichlet=dict(node_indices=model.dirichlet_nodes.tolist(),
-                             A_Wb_per_m=model.dirichlet_values.tolist()),
-        node_constraints=[]))
-    path.write_text(json.dumps(doc), encoding="utf-8")
-
-
 def test_planar_dc_request_with_force_torque_and_airgap_probes(tmp_path):
     """The planar-DC path with postprocessing: weighted-stress loads on the PM group against
     air and the four smoothed air-gap samples at r = 1 mm, against the frozen FEMM values
@@ -96,9 +69,9 @@ def test_planar_dc_request_with_force_torque_and_airgap_probes(tmp_path):
     model, _, current, flux = ff.nonlinear_pm_coil()
     post = ff.nonlinear_pm_coil_postprocess()
     art = tmp_path / "pm_coil.gpu.json"
-    _planar_dc_artifact(art, model, current, 50.0)
+    sha = ff.write_artifact(art, model, [current], turns={2: 50.0})
     request = dict(protocol=sample.PLANAR_DC, mesh_artifact_path=str(art),
-                   mesh_artifact_sha256=hashlib.sha256(art.read_bytes()).hexdigest(),
+                   mesh_artifact_sha256=sha,
                    circuit_currents_A=[current], force_group_number=post["selected_label"],
                    stress_air_group_number=3, airgap_radius_mm=post["radius_m"] * 1e3,
                    airgap_angles_deg=[a * 180.0 / math.pi for a in post["angles"]],
@@ -121,3 +94,59 @@ def test_planar_dc_request_with_force_torque_and_airgap_probes(tmp_path):
     assert len(doc["airgap_radial_flux_density_T"]) == len(post["radial_b"])
     for got, want in zip(doc["airgap_radial_flux_density_T"], post["radial_b"]):
         assert near(got, want, 1e-9)
+
+
+def _near(got, want, atol):
+    return abs(got - want) <= atol + 1e-5 * abs(want)
+
+
+def test_motor_batch_reproduces_frozen_femm(tmp_path):
+    """gpu_femm_motor_batch_v1 end to end: the frozen PM / coil model as a posed motor artifact
+    (gpu_femm_mesh_v1), five items in chunks of three: the frozen current (twice, and once
+    through the single-sample adapter), its negative and half, and one item whose artifact SHA
+    does not match.  The frozen-current items carry the frozen FEMM flux, weighted-stress loads
+    and air-gap samples at the reference's 1e-5 bar; the mismatched item fails alone (batch
+    PARTIAL_FAILURE); solved items equal the single-sample adapter's numbers bit for bit."""
+    say("case motor batch pm_coil")
+    model, _, current, flux = ff.nonlinear_pm_coil()
+    post = ff.nonlinear_pm_coil_postprocess()
+    art = tmp_path / "pm_coil.motor.gpu.json"
+    sha = ff.write_artifact(art, model, [current], schema="gpu_femm_mesh_v1", turns={2: 50.0})
+    base = dict(protocol=sample.MOTOR, mesh_artifact_path=str(art), mesh_artifact_sha256=sha,
+                base_motor_fem_sha256="b" * 64, source_fem_sha256="a" * 64,
+                circuit_currents_A=[current], selected_group_number=post["selected_label"],
+                air_group_number=3, airgap_radius_mm=post["radius_m"] * 1e3,
+                airgap_angles_deg=[a * 180.0 / math.pi for a in post["angles"]],
+                rotor_angle_deg=0.0, displacement_mm=[0.0, 0.0])
+    items = [("frozen", base), ("negative", dict(base, circuit_currents_A=[-current])),
+             ("bad-sha", dict(base, mesh_artifact_sha256="0" * 64)),
+             ("half", dict(base, circuit_currents_A=[0.5 * current])),
+             ("frozen.again", base)]
+    batch = dict(protocol=sample.MOTOR_BATCH, max_items_per_chunk=3,
+                 items=[dict(task_id=t, request=r) for t, r in items])
+    req, resp = tmp_path / "batch.json", tmp_path / "batch.response.json"
+    req.write_text(json.dumps(batch), encoding="utf-8")
+    assert sample.run_motor_batch(str(req), str(resp)) == 1
+    doc = json.loads(resp.read_text(encoding="utf-8"))
+    assert (doc["status"], doc["solve_status"]) == ("FAIL", "PARTIAL_FAILURE")
+    got = {i["task_id"]: i["response"] for i in doc["items"]}
+    assert [i["task_id"] for i in doc["items"]] == [t for t, _ in items]
+    assert got["bad-sha"]["solve_status"] == "INVALID_ARGUMENT"
+    for t in ("frozen", "negative", "half", "frozen.again"):
+        assert (got[t]["status"], got[t]["solve_status"]) == ("PASS", "OK"), t
+        assert got[t]["mesh_element_count"] == model.triangle_count
+    r = got["frozen"]
+    say(f"  flux {r['circuit_flux_linkage_Wb'][0]:.12e} (frozen {flux:.12e}), torque "
+        f"{r['torque_Nm']:.12e} (frozen {post['torque']:.12e})")
+    assert _near(r["circuit_flux_linkage_Wb"][0], flux, 1e-12)
+    assert _near(r["Fx_N"], post["fx"], 1e-8) and _near(r["Fy_N"], post["fy"], 1e-8)
+    assert _near(r["torque_Nm"], post["torque"], 1e-10)
+    for g, w in zip(r["airgap_radial_flux_density_T"], post["radial_b"]):
+        assert _near(g, w, 1e-9)
+    # the same item in another chunk, and through the single-sample adapter: same numbers
+    assert got["frozen.again"] == r
+    single = tmp_path / "single.json"
+    req.write_text(json.dumps(base), encoding="utf-8")
+    assert sample.run_motor_sample(str(req), str(single)) == 0
+    assert json.loads(single.read_text(encoding="utf-8")) == r
+    assert doc["cache"]["chunk_count"] == 2 and doc["cache"]["mesh_cache_hits"] == 3
