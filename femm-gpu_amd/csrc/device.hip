@@ -504,6 +504,57 @@ int CopyVariant(const HostPlan &topo, int32_t index, const HostPlan &part, Devic
   return FEMM_OK;
 }
 
+size_t WorkBytes(const HostPlan &h, int32_t capacity, bool mixed, bool blocks)
+{
+  const size_t ci = static_cast<size_t>(capacity);
+  const size_t nv = static_cast<size_t>(VectorStride(h.n));   // per-item vector stride
+  size_t bytes = Bytes<double>(ci * h.T * kElemStride) + Bytes<double>(ci * h.sell_size) +
+                 6 * Bytes<double>(ci * nv) + Bytes<double>(ci * h.N);
+  if (mixed)
+    bytes += Bytes<float>(ci * h.sell_size) + 5 * Bytes<float>(ci * nv);
+  if (blocks)
+    bytes += Bytes<double>(ci * kMaxBlock * nv) + (mixed ? Bytes<float>(ci * kMaxBlock * nv) : 0);
+  bytes += Bytes<double>(ci * 2 * kMaxPartitions * 4) + Bytes<uint32_t>(ci * kBarrierWords) +
+           Bytes<int32_t>(ci) + Bytes<int32_t>(ci + 1);
+  return bytes;
+}
+
+DeviceWork CarveWork(const HostPlan &h, int32_t capacity, bool mixed, bool blocks, void *base)
+{
+  const size_t ci = static_cast<size_t>(capacity);
+  const size_t nv = static_cast<size_t>(VectorStride(h.n));
+  char *cur = static_cast<char *>(base);
+  DeviceWork w{};
+  w.capacity = capacity;
+  w.elem = Carve<double>(cur, ci * h.T * kElemStride);
+  w.vals = Carve<double>(cur, ci * h.sell_size);
+  w.rhs = Carve<double>(cur, ci * nv);
+  w.inv = Carve<double>(cur, ci * nv);
+  w.x = Carve<double>(cur, ci * nv);
+  w.r = Carve<double>(cur, ci * nv);
+  w.p = Carve<double>(cur, ci * nv);
+  w.ap = Carve<double>(cur, ci * nv);
+  w.trial = Carve<double>(cur, ci * h.N);
+  if (mixed) {
+    w.vals32 = Carve<float>(cur, ci * h.sell_size);
+    w.r32 = Carve<float>(cur, ci * nv);
+    w.p32 = Carve<float>(cur, ci * nv);
+    w.ap32 = Carve<float>(cur, ci * nv);
+    w.d32 = Carve<float>(cur, ci * nv);
+    w.inv32 = Carve<float>(cur, ci * nv);
+  }
+  if (blocks) {
+    w.bjac = Carve<double>(cur, ci * kMaxBlock * nv);
+    if (mixed)
+      w.bjac32 = Carve<float>(cur, ci * kMaxBlock * nv);
+  }
+  w.team_partials = Carve<double>(cur, ci * 2 * kMaxPartitions * 4);
+  w.team_barrier = Carve<uint32_t>(cur, ci * kBarrierWords);
+  w.team_abort = Carve<int32_t>(cur, ci);
+  w.cont = Carve<int32_t>(cur, ci + 1);
+  return w;
+}
+
 int EnsureWork(const HostPlan &h, int32_t items, bool mixed, DeviceBuffers *dev, bool blocks)
 {
   const bool have_mixed = dev->work.vals32 != nullptr;
@@ -520,17 +571,7 @@ int EnsureWork(const HostPlan &h, int32_t items, bool mixed, DeviceBuffers *dev,
     (void)hipFree(dev->work_base);
     dev->work_base = nullptr;
   }
-  const size_t ci = static_cast<size_t>(cap);
-  const size_t nv = static_cast<size_t>(VectorStride(h.n));   // per-item vector stride
-  size_t bytes = Bytes<double>(ci * h.T * kElemStride) + Bytes<double>(ci * h.sell_size) +
-                 6 * Bytes<double>(ci * nv) + Bytes<double>(ci * h.N);
-  if (want_mixed)
-    bytes += Bytes<float>(ci * h.sell_size) + 5 * Bytes<float>(ci * nv);
-  if (want_blocks)
-    bytes += Bytes<double>(ci * kMaxBlock * nv) +
-             (want_mixed ? Bytes<float>(ci * kMaxBlock * nv) : 0);
-  bytes += Bytes<double>(ci * 2 * kMaxPartitions * 4) + Bytes<uint32_t>(ci * kBarrierWords) +
-           Bytes<int32_t>(ci) + Bytes<int32_t>(ci + 1);
+  const size_t bytes = WorkBytes(h, cap, want_mixed, want_blocks);
   void *base = nullptr;
   if (hipMalloc(&base, bytes) != hipSuccess)
     return FEMM_GPU_ALLOCATION_FAILED;
@@ -539,36 +580,7 @@ int EnsureWork(const HostPlan &h, int32_t items, bool mixed, DeviceBuffers *dev,
     (void)hipFree(base);
     return FEMM_INTERNAL_ERROR;
   }
-  char *cur = static_cast<char *>(base);
-  DeviceWork &w = dev->work;
-  w = DeviceWork{};
-  w.capacity = cap;
-  w.elem = Carve<double>(cur, ci * h.T * kElemStride);
-  w.vals = Carve<double>(cur, ci * h.sell_size);
-  w.rhs = Carve<double>(cur, ci * nv);
-  w.inv = Carve<double>(cur, ci * nv);
-  w.x = Carve<double>(cur, ci * nv);
-  w.r = Carve<double>(cur, ci * nv);
-  w.p = Carve<double>(cur, ci * nv);
-  w.ap = Carve<double>(cur, ci * nv);
-  w.trial = Carve<double>(cur, ci * h.N);
-  if (want_mixed) {
-    w.vals32 = Carve<float>(cur, ci * h.sell_size);
-    w.r32 = Carve<float>(cur, ci * nv);
-    w.p32 = Carve<float>(cur, ci * nv);
-    w.ap32 = Carve<float>(cur, ci * nv);
-    w.d32 = Carve<float>(cur, ci * nv);
-    w.inv32 = Carve<float>(cur, ci * nv);
-  }
-  if (want_blocks) {
-    w.bjac = Carve<double>(cur, ci * kMaxBlock * nv);
-    if (want_mixed)
-      w.bjac32 = Carve<float>(cur, ci * kMaxBlock * nv);
-  }
-  w.team_partials = Carve<double>(cur, ci * 2 * kMaxPartitions * 4);
-  w.team_barrier = Carve<uint32_t>(cur, ci * kBarrierWords);
-  w.team_abort = Carve<int32_t>(cur, ci);
-  w.cont = Carve<int32_t>(cur, ci + 1);
+  dev->work = CarveWork(h, cap, want_mixed, want_blocks, base);
   dev->work_base = base;
   dev->work_bytes = bytes;
   return hipDeviceSynchronize() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;

@@ -98,6 +98,10 @@ int UploadPlan(const HostPlan &host, DeviceBuffers *dev);
 int AllocateVariants(const HostPlan &topology, int32_t count, DeviceBuffers *dev);
 int CopyVariant(const HostPlan &topology, int32_t index, const HostPlan &part, DeviceBuffers *dev);
 // Workspace for `items` items; `mixed` adds the FP32 arrays, `blocks` the block-Jacobi planes.
+// Bytes of a solve workspace for `capacity` items and its layout from `base` (every
+// array 256-byte aligned); EnsureWork allocates, zeroes and carves one.
+size_t WorkBytes(const HostPlan &host, int32_t capacity, bool mixed, bool blocks);
+DeviceWork CarveWork(const HostPlan &host, int32_t capacity, bool mixed, bool blocks, void *base);
 int EnsureWork(const HostPlan &host, int32_t items, bool mixed, DeviceBuffers *dev,
                bool blocks = false);
 void ReleaseDevice(DeviceBuffers *dev);

@@ -226,6 +226,21 @@ class Plan:
             ctypes.c_void_p(a_ptr), ctypes.c_void_p(flux_ptr), ctypes.c_void_p(results_ptr),
             ctypes.c_void_p(stream_ptr)), name)
 
+    def solve_batch_device_noalloc(self, currents_ptr, items, a_ptr, flux_ptr, results_ptr,
+                                   stream_ptr, options: CNewtonOptions = None,
+                                   config: SolveConfig = None, warm_ptr=None):
+        """femm_newton_batch_device_noalloc: the device-pointer solve with its workspace in
+        the library's fixed device array (no device allocation; the plan must already be
+        resident).  Returns the status code instead of raising, so callers can see the
+        array's size limit (FEMM_GPU_ALLOCATION_FAILED)."""
+        options = options or production_options()
+        config = config or reference_config()
+        return int(self._lib.femm_newton_batch_device_noalloc(
+            self._handle, int(items), ctypes.c_void_p(currents_ptr), ctypes.byref(options),
+            ctypes.byref(config), None if warm_ptr is None else ctypes.c_void_p(warm_ptr),
+            ctypes.c_void_p(a_ptr), ctypes.c_void_p(flux_ptr), ctypes.c_void_p(results_ptr),
+            ctypes.c_void_p(stream_ptr)))
+
     def spmv_batch_device(self, items, precision, reps, x_ptr, y_ptr, stream_ptr):
         """reps products per item on the workspace operator (femm_spmv_batch_device)."""
         _check(self._lib.femm_spmv_batch_device(self._handle, int(items), int(precision),

@@ -146,6 +146,27 @@ int femm_newton_batch_device(femm_plan *plan, int32_t items, const double *curre
                              const femm_solve_config_t *config, const double *warm_starts,
                              double *a, double *flux, femm_item_result_t *results, void *stream);
 
+/* femm_newton_batch_device without any device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid it).  Same arguments, kernels and results as
+ * femm_newton_batch_device, but the solve workspace lives in a fixed array of
+ * FEMM_NOALLOC_SCRATCH_BYTES (128 MiB) that the library holds on every GPU it runs on until
+ * the process ends, instead of the plan's workspace.  Limits:
+ *  - the plan must already be resident on the device (any earlier computing call on it
+ *    uploads it; a fresh plan gives FEMM_INVALID_ARGUMENT);
+ *  - a batch whose workspace exceeds the array gives FEMM_GPU_ALLOCATION_FAILED and
+ *    launches nothing (per item about 96 T + 8 S + 48 n + 8 N + 17 KiB bytes in FP64, S the
+ *    SELL-64 value slots, about 1.2 nnz; the mixed form adds 4 S + 20 n, block-Jacobi
+ *    64 n (+ 32 n mixed): about 21 MB per item of the 58,682-node motor, so 6 items);
+ *  - every call on a GPU shares that GPU's array and first clears the part it uses, on
+ *    `stream`: calls that overlap on one GPU (other streams or threads) race. */
+#define FEMM_NOALLOC_SCRATCH_BYTES (128ull << 20)
+int femm_newton_batch_device_noalloc(femm_plan *plan, int32_t items, const double *currents,
+                                     const femm_newton_options_t *options,
+                                     const femm_solve_config_t *config,
+                                     const double *warm_starts, double *a, double *flux,
+                                     femm_item_result_t *results, void *stream);
+
 /* Batched safeguarded Newton.  Replaces NonlinearP1Solver::SolveSafeguarded
  * (solvers/nonlinear_solver.cu:534-659): full Newton steps from the plain start (or
  * warm_starts, the hook of Solve, :667-698) with Armijo backtracking on the equation residual
