@@ -19,6 +19,12 @@ struct femm_group {
   std::vector<femm_group_entry_t> entries;
   std::vector<femm::GroupEntry> uploaded;   // the table as last copied to the device
   femm::GroupEntry *table = nullptr;        // device copy, entries.size() records
+  // Captured launch sequence (femm_group_capture) with its own device copy of the table, so
+  // later eager solves with another layout do not change what the graph runs, and the table
+  // it was captured with (a replay refuses once a plan's device arrays or workspace moved).
+  hipGraphExec_t graph = nullptr;
+  femm::GroupEntry *graph_table = nullptr;
+  std::vector<femm::GroupEntry> graph_entries;
 };
 
 namespace
@@ -74,6 +80,55 @@ int PrepareTable(femm_group *group, bool mixed, bool blocks, std::vector<femm::G
   return FEMM_OK;
 }
 
+void ReleaseGraph(femm_group *group)
+{
+  if (group->graph != nullptr)
+    (void)hipGraphExecDestroy(group->graph);
+  if (group->graph_table != nullptr)
+    (void)hipFree(group->graph_table);
+  group->graph = nullptr;
+  group->graph_table = nullptr;
+  group->graph_entries.clear();
+}
+
+// The device state and launch table of a group solve with config `c`, uploaded to
+// group->table when it changed (after the stream's earlier launches finished with it).
+int PrepareGroup(femm_group *group, const femm_solve_config_t &c, hipStream_t s,
+                 std::vector<femm::GroupEntry> *table)
+{
+  int st = PrepareTable(group, c.precision == FEMM_PRECISION_MIXED, c.block_size > 1, table);
+  if (st != FEMM_OK)
+    return st;
+  if (SameTable(*table, group->uploaded))
+    return FEMM_OK;
+  if (hipStreamSynchronize(s) != hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  if (group->table == nullptr &&
+      hipMalloc(&group->table, sizeof(femm::GroupEntry) * group->entries.size()) != hipSuccess) {
+    group->table = nullptr;
+    return FEMM_GPU_ALLOCATION_FAILED;
+  }
+  if (hipMemcpy(group->table, table->data(), sizeof(femm::GroupEntry) * table->size(),
+                hipMemcpyHostToDevice) != hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  group->uploaded = *table;
+  return FEMM_OK;
+}
+
+// The launches of one group solve on `s`: the reference driver followed by the bounded
+// recovery, or the safeguarded Newton alone.  Only kernels are enqueued (no copies or
+// memsets), so the sequence can be captured into a graph.
+int EnqueueGroup(const femm::GroupEntry *device_table, const std::vector<femm::GroupEntry> &table,
+                 const femm_newton_options_t &o, const femm_solve_config_t &c, bool safeguarded,
+                 hipStream_t s)
+{
+  int st = femm::LaunchGroup(device_table, table, o, c, safeguarded, false, s);
+  if (safeguarded || st != FEMM_OK || !o.recover_nonconvergence || o.max_newton_iterations <= 0)
+    return st;
+  // Bounded recovery of exhausted items from the plain start, as femm_newton_batch.
+  return femm::LaunchGroup(device_table, table, o, c, true, true, s);
+}
+
 }  // namespace
 
 extern "C" {
@@ -101,10 +156,11 @@ void femm_group_destroy(femm_group *group)
   FEMM_API_BEGIN
   if (group == nullptr)
     return;
-  if (group->table != nullptr) {
+  if (group->table != nullptr || group->graph != nullptr)
     (void)hipDeviceSynchronize();
+  if (group->table != nullptr)
     (void)hipFree(group->table);
-  }
+  ReleaseGraph(group);
   delete group;
   FEMM_API_END_VOID
 }
@@ -205,7 +261,7 @@ int femm_group_autotune(femm_group *group, const femm_solve_config_t *config,
 // One group solve: the reference driver followed by the bounded recovery, or the safeguarded
 // Newton alone.
 static int GroupSolve(femm_group *group, const femm_newton_options_t *options,
-               const femm_solve_config_t *config, void *stream, bool safeguarded)
+                      const femm_solve_config_t *config, void *stream, bool safeguarded)
 {
   if (group == nullptr)
     return FEMM_INVALID_ARGUMENT;
@@ -217,28 +273,89 @@ static int GroupSolve(femm_group *group, const femm_newton_options_t *options,
     return FEMM_INVALID_ARGUMENT;
   const hipStream_t s = static_cast<hipStream_t>(stream);
   std::vector<femm::GroupEntry> table;
-  int st = PrepareTable(group, c.precision == FEMM_PRECISION_MIXED, c.block_size > 1, &table);
+  const int st = PrepareGroup(group, c, s, &table);
   if (st != FEMM_OK)
     return st;
-  if (!SameTable(table, group->uploaded)) {
-    // Launches already enqueued may still read the old table.
-    if (hipStreamSynchronize(s) != hipSuccess)
-      return FEMM_INTERNAL_ERROR;
-    if (group->table == nullptr &&
-        hipMalloc(&group->table, sizeof(femm::GroupEntry) * group->entries.size()) != hipSuccess) {
-      group->table = nullptr;
-      return FEMM_GPU_ALLOCATION_FAILED;
-    }
-    if (hipMemcpy(group->table, table.data(), sizeof(femm::GroupEntry) * table.size(),
-                  hipMemcpyHostToDevice) != hipSuccess)
-      return FEMM_INTERNAL_ERROR;
-    group->uploaded = table;
-  }
-  st = femm::LaunchGroup(group->table, table, o, c, safeguarded, false, s);
-  if (safeguarded || st != FEMM_OK || !o.recover_nonconvergence || o.max_newton_iterations <= 0)
+  return EnqueueGroup(group->table, table, o, c, safeguarded, s);
+}
+
+int femm_group_capture(femm_group *group, const femm_newton_options_t *options,
+                       const femm_solve_config_t *config, int32_t safeguarded)
+{
+  FEMM_API_BEGIN
+  if (group == nullptr)
+    return FEMM_INVALID_ARGUMENT;
+  if (!femm::DevicePresent())
+    return FEMM_GPU_UNAVAILABLE;
+  const femm_newton_options_t o = options ? *options : femm::DefaultNewtonOptions();
+  const femm_solve_config_t c = config ? *config : femm::DefaultSolveConfig();
+  if (!femm::ConfigValid(c))
+    return FEMM_INVALID_ARGUMENT;
+  if (hipDeviceSynchronize() != hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  ReleaseGraph(group);
+  // Everything that allocates or copies happens before the capture.
+  std::vector<femm::GroupEntry> table;
+  int st = PrepareGroup(group, c, nullptr, &table);
+  if (st != FEMM_OK)
     return st;
-  // Bounded recovery of exhausted items from the plain start, as femm_newton_batch.
-  return femm::LaunchGroup(group->table, table, o, c, true, true, s);
+  if (hipMalloc(&group->graph_table, sizeof(femm::GroupEntry) * table.size()) != hipSuccess) {
+    group->graph_table = nullptr;
+    return FEMM_GPU_ALLOCATION_FAILED;
+  }
+  if (hipMemcpy(group->graph_table, table.data(), sizeof(femm::GroupEntry) * table.size(),
+                hipMemcpyHostToDevice) != hipSuccess) {
+    ReleaseGraph(group);
+    return FEMM_INTERNAL_ERROR;
+  }
+  hipStream_t s = nullptr;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+    ReleaseGraph(group);
+    return FEMM_INTERNAL_ERROR;
+  }
+  hipGraph_t graph = nullptr;
+  st = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess
+           ? FEMM_OK
+           : FEMM_INTERNAL_ERROR;
+  if (st == FEMM_OK) {
+    const int launched = EnqueueGroup(group->graph_table, table, o, c, safeguarded != 0, s);
+    const bool ended = hipStreamEndCapture(s, &graph) == hipSuccess;
+    st = launched != FEMM_OK ? launched : (ended ? FEMM_OK : FEMM_INTERNAL_ERROR);
+  }
+  if (st == FEMM_OK && hipGraphInstantiate(&group->graph, graph, nullptr, nullptr, 0) != hipSuccess) {
+    group->graph = nullptr;
+    st = FEMM_INTERNAL_ERROR;
+  }
+  if (graph != nullptr)
+    (void)hipGraphDestroy(graph);
+  (void)hipStreamDestroy(s);
+  if (st != FEMM_OK) {
+    ReleaseGraph(group);
+    return st;
+  }
+  group->graph_entries = table;
+  return FEMM_OK;
+  FEMM_API_END
+}
+
+int femm_group_replay(femm_group *group, void *stream)
+{
+  FEMM_API_BEGIN
+  if (group == nullptr || group->graph == nullptr)
+    return FEMM_INVALID_ARGUMENT;
+  // The graph holds the device arrays and workspaces it was captured on: refuse once a
+  // plan's were released or reallocated (a larger batch, another precision or block size).
+  for (size_t e = 0; e < group->entries.size(); ++e) {
+    const femm::DeviceBuffers &dev = group->entries[e].plan->dev;
+    const femm::GroupEntry &g = group->graph_entries[e];
+    if (std::memcmp(&dev.plan, &g.plan, sizeof(g.plan)) != 0 ||
+        std::memcmp(&dev.work, &g.work, sizeof(g.work)) != 0)
+      return FEMM_INVALID_ARGUMENT;
+  }
+  return hipGraphLaunch(group->graph, static_cast<hipStream_t>(stream)) == hipSuccess
+             ? FEMM_OK
+             : FEMM_INTERNAL_ERROR;
+  FEMM_API_END
 }
 
 int femm_newton_group(femm_group *group, const femm_newton_options_t *options,

@@ -15,8 +15,8 @@ mesh, FP64 reference form, 1e-6 step / 1e-8 equation-residual gates.  Forms time
 * group_inexact / group_inexact_mixed: the tuned launch running the safeguarded Newton with
   inexact linear solves (forcing 0.01, femm_newton_group_safeguarded) in the FP64 and the
   mixed form;
-* group_graph: group_tuned captured once in a HIP graph and replayed (when the runtime
-  captures cooperative launches; otherwise reported as unavailable).
+* group_graph: group_tuned captured once in the group's HIP graph (femm_group_capture) and
+  replayed (femm_group_replay).
 
 Reported per form: GPU time, solves/s, items passing, and the largest A / flux difference
 to the per-mesh form over the meshes that pass the gates in both.
@@ -172,22 +172,11 @@ def main(argv=None):
         ms = timed(lambda: group.solve(stream.cuda_stream, opts, icfg, safeguarded=True))
         record(name, ms, dict(partitions=parts, blocks=blocks))
 
-    # HIP graph of the tuned group launch (table already uploaded by the runs above).
-    try:
-        graph = torch.cuda.CUDAGraph()
-        side = torch.cuda.Stream()
-        side.wait_stream(stream)
-        with torch.cuda.stream(side):
-            group.solve(side.cuda_stream, opts, cfg)   # warm on the capture stream
-        torch.cuda.synchronize()
-        with torch.cuda.graph(graph, stream=side):
-            group.solve(side.cuda_stream, opts, cfg)
-        torch.cuda.synchronize()
-        ms = timed(graph.replay)
-        record("group_graph", ms, dict(partitions=parts, blocks=blocks))
-    except Exception as exc:  # capture refused by the runtime: report, keep the rest
-        report["forms"]["group_graph"] = dict(unavailable=f"{type(exc).__name__}: {exc}"[:300])
-        _log(f"# group_graph unavailable: {exc}")
+    # The tuned group launch captured in the library's HIP graph (femm_group_capture) and
+    # replayed (femm_group_replay).
+    group.capture(opts, cfg)
+    ms = timed(lambda: group.replay(stream.cuda_stream))
+    record("group_graph", ms, dict(partitions=parts, blocks=blocks))
     per = report["forms"]["per_mesh"]["ms"]
     for k in ("group", "group_tuned", "group_inexact", "group_inexact_mixed", "group_graph"):
         if "ms" in report["forms"].get(k, {}):

@@ -432,3 +432,19 @@ class Group:
         name = "femm_newton_group_safeguarded" if safeguarded else "femm_newton_group"
         _check(getattr(self._lib, name)(self._handle, ctypes.byref(options),
                                         ctypes.byref(config), ctypes.c_void_p(stream_ptr)), name)
+
+    def capture(self, options: CNewtonOptions = None, config: SolveConfig = None,
+                safeguarded=False):
+        """Capture the launches of one solve(...) call into the group's HIP graph
+        (femm_group_capture); synchronises the device."""
+        options = options or production_options()
+        config = config or reference_config()
+        _check(self._lib.femm_group_capture(self._handle, ctypes.byref(options),
+                                            ctypes.byref(config), int(bool(safeguarded))),
+               "femm_group_capture")
+
+    def replay(self, stream_ptr):
+        """One replay of the captured graph on the stream (femm_group_replay); nothing is
+        synchronised."""
+        _check(self._lib.femm_group_replay(self._handle, ctypes.c_void_p(stream_ptr)),
+               "femm_group_replay")

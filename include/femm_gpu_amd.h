@@ -245,6 +245,21 @@ int femm_newton_group(femm_group *group, const femm_newton_options_t *options,
 int femm_newton_group_safeguarded(femm_group *group, const femm_newton_options_t *options,
                                   const femm_solve_config_t *config, void *stream);
 
+/* The launches of one femm_newton_group call (safeguarded = 0) or femm_newton_group_safeguarded
+ * call (safeguarded != 0) captured once into a HIP graph that the group keeps; replacing an
+ * earlier capture.  No counterpart in the reference, which launches per Newton step from the
+ * host (solvers/nonlinear_solver.cu:183-530).  Capture synchronises the device, prepares every
+ * entry's device state (this may allocate) and records only kernels, so a replay neither
+ * allocates nor copies.  The graph keeps the options, config, partitions and the entries'
+ * pointers of the capture; each replay re-reads the currents and rewrites a / flux / results
+ * as the eager call would, with the same results bit for bit. */
+int femm_group_capture(femm_group *group, const femm_newton_options_t *options,
+                       const femm_solve_config_t *config, int32_t safeguarded);
+/* One replay of the captured graph on `stream`, not synchronised.  FEMM_INVALID_ARGUMENT
+ * without a capture, or once an entry's plan has released or reallocated the device arrays or
+ * workspace the graph was captured on (capture again then). */
+int femm_group_replay(femm_group *group, void *stream);
+
 /* Per-item material variants on the plan's mesh.  Each variant is a full model with the
  * plan's nodes, triangles, Dirichlet nodes and values, node constraints, problem type,
  * depth, circuit count, material count and B-H knot layout; its material values, the

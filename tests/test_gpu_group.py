@@ -167,3 +167,61 @@ def test_group_safeguarded_matches_per_plan_solves(forcing):
     group.close()
     for e in entries:
         e.plan.close()
+
+
+@pytest.mark.parametrize("form", ["driver_fp64", "safeguarded_mixed_block4"])
+def test_group_graph_replays_equal_eager_solves(form):
+    """femm_group_capture / femm_group_replay: every replay re-reads the currents and rewrites
+    the outputs exactly as the eager group solve does (bitwise), with eager work between the
+    replays; a replay after a plan's workspace grew is refused until the group is captured
+    again."""
+    say(f"case group graph ({form}): rings 8/12/20 cells, replays vs eager solves")
+    safeguarded = form.startswith("safeguarded")
+    cfg = (lambda **k: mixed_config(block_size=4, inexact_forcing=0.01, **k)) if safeguarded \
+        else reference_config
+    opts = accepted_options()
+    entries = [_Entry(8, 2, 1, 31), _Entry(12, 1, 2, 32), _Entry(20, 2, 2, 33)]
+    group = Group([e.spec() for e in entries])
+    stream = torch.cuda.current_stream()
+
+    def run(scale, replay):
+        for e in entries:
+            e.cur.copy_(torch.from_numpy(e.currents * scale).cuda())
+            e.a.fill_(float("nan"))
+            e.flux.fill_(float("nan"))
+            e.res.fill_(255)
+        if replay:
+            group.replay(stream.cuda_stream)
+        else:
+            group.solve(stream.cuda_stream, opts, cfg(), safeguarded=safeguarded)
+        torch.cuda.synchronize()
+        return [e.outputs() for e in entries]
+
+    eager = {s: run(s, False) for s in (1.0, 0.7)}
+    for e, (a, flux, res) in zip(entries, eager[1.0]):
+        say(f"  {e.model.name}: items {e.items} x {e.partitions} wg, status "
+            f"{res['status'].tolist()} newton {res['newton_iterations'].tolist()}")
+        assert (res["status"] == 0).all()
+    group.capture(opts, cfg(), safeguarded=safeguarded)
+    for k, scale in enumerate((1.0, 0.7, 1.0)):
+        got = run(scale, True)
+        for e, (a, flux, res), (ra, rflux, rres) in zip(entries, got, eager[scale]):
+            assert np.array_equal(a, ra) and np.array_equal(flux, rflux), (k, e.model.name)
+            assert res.tobytes() == rres.tobytes(), (k, e.model.name)
+        # eager work between replays: a host solve on one of the group's plans
+        entries[0].plan.solve_batch(entries[0].currents, opts, cfg(partitions=1))
+        say(f"  replay {k} (scale {scale}): bitwise equal to the eager solve")
+    # A larger batch reallocates the plan's workspace: the graph no longer matches it.
+    big = np.repeat(entries[2].currents, 3, axis=0)
+    entries[2].plan.solve_batch(big, opts, cfg(partitions=1))
+    with pytest.raises(RuntimeError):
+        group.replay(stream.cuda_stream)
+    group.capture(opts, cfg(), safeguarded=safeguarded)
+    got = run(0.7, True)
+    for (a, flux, res), (ra, rflux, rres) in zip(got, eager[0.7]):
+        assert np.array_equal(a, ra) and np.array_equal(flux, rflux)
+        assert res.tobytes() == rres.tobytes()
+    say("  refused after the workspace grew; replay after a new capture equal")
+    group.close()
+    for e in entries:
+        e.plan.close()
