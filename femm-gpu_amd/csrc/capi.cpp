@@ -261,6 +261,8 @@ int femm_plan_create(const femm_model_t *model, femm_plan **out)
   mc.view.curve_offsets = mc.curve_offsets.data();
   mc.view.curve_b = nullptr;
   mc.view.curve_h = nullptr;
+  mc.view.air_gap_count = static_cast<int32_t>(plan->host.ages.size());
+  mc.view.air_gaps = plan->host.ages.empty() ? nullptr : plan->host.ages.data();
   // The device copy is uploaded by the first call that computes, so the
   // symbolic plan is available on hosts without a GPU.
   *out = plan.release();
@@ -296,6 +298,28 @@ int femm_plan_sizes(const femm_plan *plan, int32_t sizes[6])
   sizes[3] = h.nnz;
   sizes[4] = h.C;
   sizes[5] = h.Q;
+  return FEMM_OK;
+  FEMM_API_END
+}
+
+int femm_plan_copy_air_gap_terms(const femm_plan *plan, double *age_values, double *age_rhs)
+{
+  FEMM_API_BEGIN
+  if (plan == nullptr)
+    return FEMM_INVALID_ARGUMENT;
+  const femm::HostPlan &h = plan->host;
+  if (age_values) {
+    if (h.age_vals.empty())
+      std::fill(age_values, age_values + h.nnz, 0.0);
+    else
+      std::copy(h.age_vals.begin(), h.age_vals.end(), age_values);
+  }
+  if (age_rhs) {
+    if (h.age_rhs.empty())
+      std::fill(age_rhs, age_rhs + h.n, 0.0);
+    else
+      std::copy(h.age_rhs.begin(), h.age_rhs.end(), age_rhs);
+  }
   return FEMM_OK;
   FEMM_API_END
 }
@@ -817,7 +841,11 @@ int femm_plan_set_postprocess(femm_plan *plan, const femm_postprocess_options_t 
     plan->post_base = nullptr;
   }
   plan->post_ready = false;
-  st = femm::BuildPostPlan(&plan->model.view, plan->host, *options, &plan->post);
+  // A model with air-gap elements gets the element's own postprocess, as the reference's
+  // adapters choose for sliding-band artifacts (app/sample_adapter.cpp:59-61, 434-436).
+  st = plan->host.ages.empty()
+           ? femm::BuildPostPlan(&plan->model.view, plan->host, *options, &plan->post)
+           : femm::BuildAgePostPlan(plan->host, plan->model.view.depth_m, *options, &plan->post);
   if (st != FEMM_OK)
     return st;
   st = femm::UploadPostPlan(plan->post, plan->dev, &plan->post_dev, &plan->post_base);

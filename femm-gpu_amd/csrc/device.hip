@@ -291,6 +291,80 @@ __global__ __launch_bounds__(kBlockThreads) void PostprocessKernel(DevicePostPla
   }
 }
 
+// Per item: the air-gap element postprocess (ComputeAirGapElementPostprocess,
+// postprocess/airgap_postprocess.cpp:41-160).  Each cell's centre field comes from its
+// four virtual corners (Catmull-Rom interpolants of the rings at the element's shifts):
+// Br = d(A)/ds across the arc and Bt = -dA/dr across the gap; the Maxwell stress on the
+// mean circle gives the force and torque; the first element's Br per cell (kept in LDS)
+// gives the discrete Fourier series evaluated at the probe angles.
+__global__ __launch_bounds__(kBlockThreads) void AgePostprocessKernel(DevicePostPlan P,
+                                                                      const double *states,
+                                                                      double *loads,
+                                                                      double *radial)
+{
+  extern __shared__ double age_lds[];   // [cells0] Br, [H] cos, [H] sin coefficients
+  __shared__ double lds[2 * kWaves * 4];
+  BlockReducer R{lds, 0};
+  const int item = blockIdx.x;
+  const double *a = states + static_cast<size_t>(item) * P.N;
+  constexpr double kMu0 = 4.0e-7 * 3.141592653589793238462643383279502884;
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int c = threadIdx.x; c < P.age_cells; c += kBlockThreads) {
+    const int g = P.age_cell_element[c];
+    const int k = c - P.age_first[g];
+    const double *par = P.age_par + static_cast<size_t>(g) * kAgeParams;
+    double v[10];
+    for (int q = 0; q < 10; ++q)
+      v[q] = a[P.age_nodes[10 * c + q]] * P.age_w[10 * c + q];
+    double in0 = 0.0, in1 = 0.0, out0 = 0.0, out1 = 0.0;
+    for (int q = 0; q < 4; ++q) {
+      in0 += par[q] * v[q];
+      in1 += par[q] * v[q + 1];
+      out0 += par[4 + q] * v[q + 5];
+      out1 += par[4 + q] * v[q + 6];
+    }
+    const double dt = par[8], radius = par[9], dr = par[10];
+    const double br = ((in1 - in0) + (out1 - out0)) / (2.0 * dt * radius);
+    const double bt = ((in0 + in1) - (out0 + out1)) / (2.0 * dr);
+    const double theta = (static_cast<double>(k) + 0.5) * dt;
+    const double normal = br * br - bt * bt, shear = 2.0 * br * bt;
+    s[0] += par[11] * (normal * cos(theta) - shear * sin(theta));
+    s[1] += par[11] * (normal * sin(theta) + shear * cos(theta));
+    s[2] += par[12] * br * bt / kMu0;
+    if (g == 0)
+      age_lds[k] = br;
+  }
+  R.Sum(s);   // ends with a barrier: the first element's Br is complete
+  if (threadIdx.x == 0) {
+    loads[3 * item] = s[0];
+    loads[3 * item + 1] = s[1];
+    loads[3 * item + 2] = s[2];
+  }
+  const int cells0 = P.age_cells0, H = P.age_harmonics;
+  const double dt0 = P.age_par[8];
+  for (int h = threadIdx.x; h < H; h += kBlockThreads) {
+    const double order = static_cast<double>(P.age_order[h]);
+    double cs = 0.0, sn = 0.0;
+    for (int k = 0; k < cells0; ++k) {
+      const double phase = (static_cast<double>(k) + 0.5) * dt0 * order;
+      cs += age_lds[k] * cos(phase);
+      sn += age_lds[k] * sin(phase);
+    }
+    age_lds[cells0 + h] = cs / P.age_norm[h];
+    age_lds[cells0 + H + h] = sn / P.age_norm[h];
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < P.nprobe; p += kBlockThreads) {
+    const double angle = P.probe_angle[p];
+    double value = 0.0;
+    for (int h = 0; h < H; ++h) {
+      const double phase = static_cast<double>(P.age_order[h]) * angle;
+      value += age_lds[cells0 + h] * cos(phase) + age_lds[cells0 + H + h] * sin(phase);
+    }
+    radial[static_cast<size_t>(item) * P.nprobe + p] = value;
+  }
+}
+
 template <typename T>
 T *Carve(char *&cursor, size_t count)
 {
@@ -356,6 +430,7 @@ int UploadPlan(const HostPlan &h, DeviceBuffers *dev)
     P.slot_to_sell = up(h.slot_to_sell);
     P.diag_sell = up(h.diag_sell);
     P.row_offsets = up(h.row_offsets);
+    P.age_vals = h.age_vals.empty() ? nullptr : up(h.age_vals);
     if (pass == 0) {
       void *mem = nullptr;
       if (hipMalloc(&mem, offset) != hipSuccess)
@@ -807,8 +882,17 @@ int LaunchPostprocess(const DevicePostPlan &post, int32_t items, const double *s
 {
   if (items <= 0)
     return FEMM_INVALID_ARGUMENT;
-  hipLaunchKernelGGL(PostprocessKernel, dim3(items), dim3(kBlockThreads), 0, stream, post, states,
-                     loads, radial_b);
+  if (post.age_mode) {
+    const size_t lds = sizeof(double) * (static_cast<size_t>(post.age_cells0) +
+                                         2 * static_cast<size_t>(post.age_harmonics));
+    if (lds > kAgeLdsLimit)
+      return FEMM_UNSUPPORTED_FEATURE;
+    hipLaunchKernelGGL(AgePostprocessKernel, dim3(items), dim3(kBlockThreads), lds, stream, post,
+                       states, loads, radial_b);
+  } else {
+    hipLaunchKernelGGL(PostprocessKernel, dim3(items), dim3(kBlockThreads), 0, stream, post,
+                       states, loads, radial_b);
+  }
   return hipGetLastError() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
 }
 

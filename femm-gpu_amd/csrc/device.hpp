@@ -33,6 +33,9 @@ struct DevicePlan {
   const double *pm_rhs, *circuit_rhs;
   const int32_t *slice_ptr, *slice_width, *sell_cols, *slot_to_sell, *diag_sell;
   const int32_t *row_offsets;   // CSR [n + 1]: slot range of a row partition
+  // Native air-gap elements: fixed stiffness per CSR slot [nnz], null without them (their
+  // fixed-node load is part of pm_rhs).
+  const double *age_vals;
 };
 
 // Per-item device workspaces, item-major.  Strides in elements.

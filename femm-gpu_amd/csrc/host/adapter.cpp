@@ -24,6 +24,7 @@
 #include "artifact.hpp"
 #include "femm_gpu_amd.h"
 #include "protocol.hpp"
+#include "sliding_band.hpp"
 
 namespace femm
 {
@@ -132,6 +133,24 @@ struct PlanHandle {
       femm_plan_destroy(plan);
   }
 };
+
+// The plan of an artifact posed for a request: a sliding-band artifact's air-gap elements
+// turned to the requested rotor angle (ApplySlidingBandRotorAngle,
+// io/sample_request.cpp:448-496), any other artifact as parsed.
+int CreatePlan(const Artifact &art, double rotor_angle_deg, PlanHandle *h)
+{
+  if (!art.sliding_band) {
+    const femm_model_t m = art.View();
+    return femm_plan_create(&m, &h->plan);
+  }
+  std::vector<femm_air_gap_t> gaps;
+  std::vector<femm_age_point_t> points;
+  if (!RotateAirGaps(art.air_gaps.data(), static_cast<int32_t>(art.air_gaps.size()),
+                     art.rotor_angle_deg, rotor_angle_deg, &gaps, &points))
+    return FEMM_INVALID_ARGUMENT;
+  const femm_model_t m = art.ViewWith(gaps);
+  return femm_plan_create(&m, &h->plan);
+}
 
 // Solves `currents.size()` items on `plan` (one batched call); fills solutions and statuses.
 int SolveBatch(femm_plan *plan, const Artifact &art, const std::vector<std::vector<double>> &currents,
@@ -338,8 +357,7 @@ int MotorSampleAdapter(const std::string &request_path, const std::string &respo
   bool solved = false;
   if (status == FEMM_OK) {
     PlanHandle h;
-    const femm_model_t m = load.artifact.View();
-    status = femm_plan_create(&m, &h.plan);
+    status = CreatePlan(load.artifact, request.rotor_angle_deg, &h);
     std::vector<int> st;
     if (status == FEMM_OK) {
       status = SolveBatch(h.plan, load.artifact, {request.currents}, &sol, &st);
@@ -413,14 +431,17 @@ int BatchAdapter(const std::string &request_path, const std::string &response_pa
   counters.effective_chunk = first.status == FEMM_OK
                                  ? EffectiveChunk(request.max_items_per_chunk, first.artifact)
                                  : request.max_items_per_chunk;
-  // Plans stay cached per artifact across chunks (the symbolic CSR is built once).
-  std::map<size_t, std::unique_ptr<PlanHandle>> plans;
+  // Plans stay cached per artifact (and rotor angle, for a sliding band) across chunks (the
+  // symbolic CSR is built once per operator, as the reference's operator groups,
+  // app/motor_batch.cpp:190-262).
+  using PlanKey = std::pair<size_t, double>;
+  std::map<PlanKey, std::unique_ptr<PlanHandle>> plans;
   for (size_t lo = 0; lo < items; lo += counters.effective_chunk) {
     const size_t hi = std::min(items, lo + static_cast<size_t>(counters.effective_chunk));
     ++counters.chunk_count;
-    // Items that pass their artifact checks, grouped by artifact in first-seen order.
-    std::vector<size_t> group_order;
-    std::map<size_t, std::vector<size_t>> groups;
+    // Items that pass their artifact checks, grouped by operator in first-seen order.
+    std::vector<PlanKey> group_order;
+    std::map<PlanKey, std::vector<size_t>> groups;
     for (size_t k = lo; k < hi; ++k) {
       const MotorRequest &r = request.items[k].request;
       const Load &load = *loads[slot[k]];
@@ -430,21 +451,21 @@ int BatchAdapter(const std::string &request_path, const std::string &response_pa
       out[k].status = st;
       if (st != FEMM_OK)
         continue;
-      if (!groups.count(slot[k]))
-        group_order.push_back(slot[k]);
-      groups[slot[k]].push_back(k);
+      const PlanKey key{slot[k], load.artifact.sliding_band ? r.rotor_angle_deg : 0.0};
+      if (!groups.count(key))
+        group_order.push_back(key);
+      groups[key].push_back(k);
     }
-    for (const size_t s : group_order) {
-      const std::vector<size_t> &group = groups[s];
-      const Artifact &art = loads[s]->artifact;
-      auto cached = plans.find(s);
+    for (const PlanKey &key : group_order) {
+      const std::vector<size_t> &group = groups[key];
+      const Artifact &art = loads[key.first]->artifact;
+      auto cached = plans.find(key);
       int st = FEMM_OK;
       if (cached == plans.end()) {
         auto h = std::make_unique<PlanHandle>();
-        const femm_model_t m = art.View();
-        st = femm_plan_create(&m, &h->plan);
+        st = CreatePlan(art, key.second, h.get());
         if (st == FEMM_OK)
-          cached = plans.emplace(s, std::move(h)).first;
+          cached = plans.emplace(key, std::move(h)).first;
         counters.mesh_cache_hits += static_cast<int32_t>(group.size()) - 1;
       } else {
         counters.mesh_cache_hits += static_cast<int32_t>(group.size());

@@ -43,7 +43,7 @@ bool Sha(const Json *v, bool lower)
 }
 
 struct Schema {
-  bool neutral = false, generic = false, motor = false;
+  bool neutral = false, generic = false, motor = false, sliding = false;
 };
 
 bool Header(const Json &root, Artifact *a, Schema *sc, const Json **resolved, std::string *error)
@@ -54,9 +54,10 @@ bool Header(const Json &root, Artifact *a, Schema *sc, const Json **resolved, st
   const std::string &s = schema->text;
   sc->generic = s == "gpu_femm_magnetostatic_mesh_v1";
   sc->neutral = sc->generic || s == "gpu_femm_planar_dc_mesh_v1";
-  sc->motor = s == "gpu_femm_mesh_v1";
-  if (s == "gpu_femm_mesh_v2" || s == "gpu_femm_mesh_v3" || s == "gpu_femm_mesh_v4") {
-    *error = "unsupported artifact schema: sliding-band and sector motor meshes";
+  sc->sliding = s == "gpu_femm_mesh_v2";
+  sc->motor = s == "gpu_femm_mesh_v1" || sc->sliding;
+  if (s == "gpu_femm_mesh_v3" || s == "gpu_femm_mesh_v4") {
+    *error = "unsupported artifact schema: sector motor meshes";
     return false;
   }
   if (!sc->neutral && !sc->motor)
@@ -85,6 +86,11 @@ bool Header(const Json &root, Artifact *a, Schema *sc, const Json **resolved, st
   else if (sc->neutral)
     body_ok = ExactMembers(**resolved, {"source_fem_sha256", "model", "nodes_mm", "triangles",
                                         "regions", "materials", "circuits", "outer_dirichlet"},
+                           error);
+  else if (sc->sliding)
+    body_ok = ExactMembers(**resolved, {"source_fem_sha256", "base_motor_fem_sha256", "model",
+                                        "pose", "nodes_mm", "triangles", "regions", "materials",
+                                        "circuits", "outer_dirichlet", "air_gap_elements"},
                            error);
   else
     body_ok = ExactMembers(**resolved, {"source_fem_sha256", "base_motor_fem_sha256", "model",
@@ -348,6 +354,96 @@ bool Constraints(const Json &list, Artifact *a, std::string *error)
 
 }  // namespace
 
+// The air-gap elements of a sliding-band artifact (mesh_artifact.cpp:594-705): one element
+// object, or a non-empty array of them, each with exactly its ten members; the mesh must be
+// the centred reference pose at 0 degrees.
+bool AirGaps(const Json &resolved, Artifact *a, std::string *error)
+{
+  auto it = resolved.members.find("air_gap_elements");
+  const Json *ages = it == resolved.members.end() ? nullptr : &it->second;
+  if (ages == nullptr ||
+      (ages->type != Json::Type::kArray && ages->type != Json::Type::kObject) ||
+      a->displacement_mm[0] != 0.0 || a->displacement_mm[1] != 0.0 || a->rotor_angle_deg != 0.0)
+    return Fail(error, "sliding-band artifacts require a centered zero-degree reference mesh");
+  std::vector<const Json *> entries;
+  if (ages->type == Json::Type::kArray) {
+    if (ages->items.empty())
+      return Fail(error, "sliding-band requires an air-gap element");
+    for (const Json &e : ages->items)
+      entries.push_back(&e);
+  } else {
+    entries.push_back(ages);
+  }
+  for (const Json *entry : entries) {
+    if (!ExactMembers(*entry, {"name", "periodicity", "center_mm", "ri_mm", "ro_mm",
+                               "arc_length_deg", "sector_count", "inner_shift", "outer_shift",
+                               "quad_points"}, error))
+      return false;
+    const Json *name = Member(*entry, "name", Json::Type::kString, error);
+    const Json *periodicity = Member(*entry, "periodicity", Json::Type::kString, error);
+    const Json *center = Member(*entry, "center_mm", Json::Type::kArray, error);
+    const Json *ri = Member(*entry, "ri_mm", Json::Type::kNumber, error);
+    const Json *ro = Member(*entry, "ro_mm", Json::Type::kNumber, error);
+    const Json *arc = Member(*entry, "arc_length_deg", Json::Type::kNumber, error);
+    const Json *sectors = Member(*entry, "sector_count", Json::Type::kNumber, error);
+    const Json *inner = Member(*entry, "inner_shift", Json::Type::kNumber, error);
+    const Json *outer = Member(*entry, "outer_shift", Json::Type::kNumber, error);
+    const Json *points = Member(*entry, "quad_points", Json::Type::kArray, error);
+    int32_t cells = 0;
+    std::vector<double> c;
+    if (name == nullptr || name->text.empty() || periodicity == nullptr || center == nullptr ||
+        ri == nullptr || ro == nullptr || arc == nullptr || sectors == nullptr ||
+        inner == nullptr || outer == nullptr || points == nullptr ||
+        !NumberArray(*center, &c) || c.size() != 2 || !Int32Of(*sectors, &cells) || cells < 2 ||
+        points->items.size() != static_cast<size_t>(cells) + 1 ||
+        (periodicity->text != "periodic" && periodicity->text != "antiperiodic"))
+      return Fail(error, "invalid sliding-band air-gap element");
+    femm_air_gap_t age{};
+    age.antiperiodic = periodicity->text == "antiperiodic" ? 1 : 0;
+    age.cells = cells;
+    age.center_xy[0] = c[0] * 1e-3;
+    age.center_xy[1] = c[1] * 1e-3;
+    age.inner_radius_m = ri->number * 1e-3;
+    age.outer_radius_m = ro->number * 1e-3;
+    age.arc_length_deg = arc->number;
+    age.inner_shift = inner->number;
+    age.outer_shift = outer->number;
+    a->age_offsets.push_back(a->age_points.size());
+    for (const Json &point : points->items) {
+      if (!ExactMembers(point, {"n0", "w0", "n1", "w1", "n2", "w2", "n3", "w3"}, error))
+        return false;
+      femm_age_point_t rec{};
+      static const char *const kNodes[4] = {"n0", "n1", "n2", "n3"};
+      static const char *const kWeights[4] = {"w0", "w1", "w2", "w3"};
+      for (int q = 0; q < 4; ++q) {
+        const Json *n = Member(point, kNodes[q], Json::Type::kNumber, error);
+        const Json *w = Member(point, kWeights[q], Json::Type::kNumber, error);
+        if (n == nullptr || w == nullptr || !Int32Of(*n, &rec.node[q]))
+          return Fail(error, "invalid sliding-band quadrature point");
+        rec.weight[q] = w->number;
+      }
+      a->age_points.push_back(rec);
+    }
+    a->air_gaps.push_back(age);
+  }
+  a->sliding_band = true;
+  return true;
+}
+
+void Artifact::Repoint()
+{
+  for (size_t g = 0; g < air_gaps.size(); ++g)
+    air_gaps[g].points = age_points.data() + age_offsets[g];
+}
+
+femm_model_t Artifact::ViewWith(const std::vector<femm_air_gap_t> &gaps) const
+{
+  femm_model_t m = View();
+  m.air_gap_count = static_cast<int32_t>(gaps.size());
+  m.air_gaps = gaps.empty() ? nullptr : gaps.data();
+  return m;
+}
+
 femm_model_t Artifact::View() const
 {
   femm_model_t m{};
@@ -370,6 +466,8 @@ femm_model_t Artifact::View() const
   m.constraints = constraints.data();
   m.circuit_count = static_cast<int32_t>(currents.size());
   m.depth_m = depth_m;
+  m.air_gap_count = static_cast<int32_t>(air_gaps.size());
+  m.air_gaps = air_gaps.empty() ? nullptr : air_gaps.data();
   return m;
 }
 
@@ -436,7 +534,19 @@ bool ParseArtifact(const std::string &text, Artifact *out, std::string *error)
   }
   if (!Boundary(*od, &a, error) || (sc.generic && !Constraints(*cons, &a, error)))
     return false;
+  if (sc.sliding && !AirGaps(*res, &a, error))
+    return false;
   *out = std::move(a);
+  out->Repoint();
+  // ValidateNonlinearModel of the mapped model (mesh_artifact.cpp:706-709).
+  const femm_model_t view = out->View();
+  HostPlan check;
+  std::string why;
+  if (BuildHostPlan(&view, &check, &why) != FEMM_OK) {
+    *out = Artifact{};
+    *error = "mapped mesh model validation failed";
+    return false;
+  }
   return true;
 }
 

@@ -2,9 +2,10 @@
 // gpu_solver/src/io/mesh_artifact.cpp:14-440).
 //
 // Schemas: gpu_femm_planar_dc_mesh_v1 and gpu_femm_magnetostatic_mesh_v1 (neutral: lower-case
-// identities, canonical identity of `resolved` checked) and gpu_femm_mesh_v1 (a posed,
-// conforming motor mesh with base / pose identities).  Sliding-band (v2 / v4) and sector (v3)
-// motor schemas are rejected as unsupported.
+// identities, canonical identity of `resolved` checked), gpu_femm_mesh_v1 (a posed,
+// conforming motor mesh with base / pose identities) and gpu_femm_mesh_v2 (a sliding-band
+// motor mesh at rotor angle 0 whose air gap is one or more native air-gap elements,
+// mesh_artifact.cpp:594-705).  Sector motor schemas (v3 / v4) are rejected as unsupported.
 #pragma once
 
 #include <cstdint>
@@ -39,9 +40,17 @@ struct Artifact {
   std::vector<femm_node_constraint_t> constraints;
   std::vector<double> currents;                 // circuits' current_A
   std::vector<int32_t> triangle_groups;         // FEMM group number per triangle
+  bool sliding_band = false;                    // gpu_femm_mesh_v2
+  std::vector<femm_air_gap_t> air_gaps;         // points: see age_offsets
+  std::vector<femm_age_point_t> age_points;
+  std::vector<size_t> age_offsets;              // first record of each element
 
   // A view of the arrays above (valid while this object lives and is not modified).
   femm_model_t View() const;
+  // View() with the air-gap elements replaced by `gaps` (e.g. turned to a rotor angle).
+  femm_model_t ViewWith(const std::vector<femm_air_gap_t> &gaps) const;
+  // Points every air-gap element at its records in age_points (after a copy or move).
+  void Repoint();
   bool HasGroup(int32_t group) const;
 };
 

@@ -9,6 +9,7 @@
 #include "artifact.hpp"
 #include "femm_wire.h"
 #include "json.hpp"
+#include "sliding_band.hpp"
 
 struct femm_artifact {
   femm::host::Artifact artifact;
@@ -29,6 +30,30 @@ void CopyText(const std::string &text, char *out, size_t capacity)
 }  // namespace
 
 extern "C" {
+
+int femm_air_gaps_rotate(int32_t count, const femm_air_gap_t *gaps, double posed_deg,
+                         double angle_deg, femm_air_gap_t *out, femm_age_point_t *out_points,
+                         size_t point_capacity)
+{
+  FEMM_API_BEGIN
+  if (count < 0 || (count > 0 && (gaps == nullptr || out == nullptr)))
+    return FEMM_INVALID_ARGUMENT;
+  std::vector<femm_air_gap_t> rotated;
+  std::vector<femm_age_point_t> points;
+  if (!femm::host::RotateAirGaps(gaps, count, posed_deg, angle_deg, &rotated, &points))
+    return FEMM_INVALID_ARGUMENT;
+  if (points.size() > point_capacity || (!points.empty() && out_points == nullptr))
+    return FEMM_INVALID_ARGUMENT;
+  std::copy(points.begin(), points.end(), out_points);
+  size_t at = 0;
+  for (int32_t g = 0; g < count; ++g) {
+    out[g] = rotated[g];
+    out[g].points = out_points + at;
+    at += static_cast<size_t>(rotated[g].cells) + 1;
+  }
+  return FEMM_OK;
+  FEMM_API_END
+}
 
 int femm_artifact_parse(const char *text, size_t length, femm_artifact **out, char *error,
                         size_t error_capacity)

@@ -2,6 +2,7 @@
 
 #include "protocol.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <set>
@@ -230,10 +231,29 @@ bool SampleMatchesArtifact(const SampleRequest &r, const Artifact &a)
 
 bool MotorMatchesArtifact(const MotorRequest &r, const Artifact &a)
 {
-  return a.schema == "gpu_femm_mesh_v1" && r.base_sha256 == a.base_sha256 &&
-         r.source_sha256 == a.source_sha256 && r.rotor_angle_deg == a.rotor_angle_deg &&
-         r.displacement_mm[0] == a.displacement_mm[0] &&
-         r.displacement_mm[1] == a.displacement_mm[1] && r.currents.size() == a.currents.size();
+  if (a.schema != "gpu_femm_mesh_v1" && a.schema != "gpu_femm_mesh_v2")
+    return false;
+  // A sliding-band artifact is the reference pose: any rotor angle, no displacement; its
+  // postprocess needs the selected / air groups and probes inside every element's gap.
+  const bool pose = a.sliding_band ? r.displacement_mm[0] == 0.0 && r.displacement_mm[1] == 0.0
+                                   : r.rotor_angle_deg == a.rotor_angle_deg &&
+                                         r.displacement_mm[0] == a.displacement_mm[0] &&
+                                         r.displacement_mm[1] == a.displacement_mm[1];
+  bool post = true;
+  if (a.sliding_band) {
+    post = a.HasGroup(r.selected_group) && a.HasGroup(r.air_group);
+    if (post && !r.airgap_angles_deg.empty()) {
+      post = !a.air_gaps.empty();
+      for (const femm_air_gap_t &age : a.air_gaps) {
+        const double inner_mm = 1000.0 * age.inner_radius_m, outer_mm = 1000.0 * age.outer_radius_m;
+        const double tolerance_mm = 1e-9 * std::max(1.0, std::abs(outer_mm));
+        post = post && r.airgap_radius_mm >= inner_mm - tolerance_mm &&
+               r.airgap_radius_mm <= outer_mm + tolerance_mm;
+      }
+    }
+  }
+  return r.base_sha256 == a.base_sha256 && r.source_sha256 == a.source_sha256 && pose && post &&
+         r.currents.size() == a.currents.size();
 }
 
 void WriteSampleResponse(std::ostream &out, int status, int solve_status, bool post_requested,

@@ -64,7 +64,8 @@ bool ParseBatchRequest(const std::string &text, BatchRequest *out, std::string *
 
 // PlanarDcRequestMatchesArtifact (app/sample_adapter.cpp:183-224) for a conforming artifact.
 bool SampleMatchesArtifact(const SampleRequest &r, const Artifact &a);
-// RequestMatchesArtifact (io/sample_request.cpp:396-446) for a posed gpu_femm_mesh_v1 artifact.
+// RequestMatchesArtifact (io/sample_request.cpp:396-446) for a posed gpu_femm_mesh_v1 or a
+// sliding-band gpu_femm_mesh_v2 artifact.
 bool MotorMatchesArtifact(const MotorRequest &r, const Artifact &a);
 
 // The solve's outputs for one item.

@@ -2,6 +2,8 @@
 
 #include "plan.hpp"
 
+#include "age.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -223,6 +225,18 @@ bool SameTopology(const femm_model_t *a, const femm_model_t *b)
     return bytes == 0 || (x != nullptr && y != nullptr && std::memcmp(x, y, bytes) == 0);
   };
   const size_t N = static_cast<size_t>(a->node_count), T = static_cast<size_t>(a->triangle_count);
+  if (a->air_gap_count != b->air_gap_count || a->air_gap_count < 0)
+    return false;
+  for (int32_t g = 0; g < a->air_gap_count; ++g) {
+    const femm_air_gap_t &x = a->air_gaps[g], &y = b->air_gaps[g];
+    if (x.antiperiodic != y.antiperiodic || x.cells != y.cells ||
+        !(x.center_xy[0] == y.center_xy[0]) || !(x.center_xy[1] == y.center_xy[1]) ||
+        !(x.inner_radius_m == y.inner_radius_m) || !(x.outer_radius_m == y.outer_radius_m) ||
+        !(x.arc_length_deg == y.arc_length_deg) || !(x.inner_shift == y.inner_shift) ||
+        !(x.outer_shift == y.outer_shift) || x.cells < 0 ||
+        !same(x.points, y.points, sizeof(femm_age_point_t) * (static_cast<size_t>(x.cells) + 1)))
+      return false;
+  }
   return same(a->node_xy, b->node_xy, sizeof(double) * 2 * N) &&
          same(a->triangle_nodes, b->triangle_nodes, sizeof(int32_t) * 3 * T) &&
          same(a->curve_offsets, b->curve_offsets,
@@ -231,6 +245,60 @@ bool SameTopology(const femm_model_t *a, const femm_model_t *b)
          same(a->dirichlet_values, b->dirichlet_values, sizeof(double) * a->dirichlet_count) &&
          same(a->constraints, b->constraints,
               sizeof(femm_node_constraint_t) * a->constraint_count);
+}
+
+// Fixed AGE stiffness per CSR slot and fixed-node load per row, accumulated element by
+// element, cell by cell, i then j (nonlinear_topology.cpp:180-232); also keeps a copy of the
+// elements for the postprocess.
+static int BuildAirGapTerms(const femm_model_t *m, HostPlan *p)
+{
+  p->age_vals.assign(p->nnz, 0.0);
+  p->age_rhs.assign(p->n, 0.0);
+  auto slot_of = [p](int32_t row, int32_t col) {
+    const int32_t *lo = p->cols.data() + p->row_offsets[row];
+    const int32_t *hi = p->cols.data() + p->row_offsets[row + 1];
+    return static_cast<int32_t>(std::lower_bound(lo, hi, col) - p->cols.data());
+  };
+  size_t total_points = 0;
+  for (int32_t g = 0; g < m->air_gap_count; ++g)
+    total_points += static_cast<size_t>(m->air_gaps[g].cells) + 1;
+  p->ages.assign(m->air_gaps, m->air_gaps + m->air_gap_count);
+  p->age_points.clear();
+  p->age_points.reserve(total_points);
+  for (int32_t g = 0; g < m->air_gap_count; ++g) {
+    const femm_air_gap_t &age = m->air_gaps[g];
+    p->age_points.insert(p->age_points.end(), age.points, age.points + age.cells + 1);
+    double K, Ki, ci, co, matrix[10][10];
+    AgeAssemblyParameters(age, &K, &Ki, &ci, &co);
+    if (!AgeCellMatrix(K, Ki, ci, co, matrix))
+      return FEMM_MESH_INVALID;
+    for (int32_t k = 0; k < age.cells; ++k) {
+      int32_t node[10];
+      double weight[10];
+      AgeStencil(age, k, node, weight);
+      for (int i = 0; i < 10; ++i) {
+        const int32_t row = p->node_to_free[node[i]];
+        if (row < 0)
+          continue;
+        const double si = static_cast<double>(p->node_sign[node[i]]);
+        for (int j = 0; j < 10; ++j) {
+          const double value = matrix[i][j] * weight[i] * weight[j] * kAgeReluctivity;
+          const int32_t col = p->node_to_free[node[j]];
+          if (col >= 0)
+            p->age_vals[slot_of(row, col)] +=
+                static_cast<double>(p->node_sign[node[i]] * p->node_sign[node[j]]) * value;
+          else
+            p->age_rhs[row] -= si * value * p->boundary[node[j]];
+        }
+      }
+    }
+  }
+  size_t at = 0;
+  for (femm_air_gap_t &age : p->ages) {
+    age.points = p->age_points.data() + at;
+    at += static_cast<size_t>(age.cells) + 1;
+  }
+  return FEMM_OK;
 }
 
 int BuildHostPlan(const femm_model_t *m, HostPlan *p, std::string *error, const HostPlan *topology)
@@ -305,6 +373,9 @@ int BuildHostPlan(const femm_model_t *m, HostPlan *p, std::string *error, const 
     if (!Basis(x, y, true, &basis[e]))
       return fail(FEMM_MESH_INVALID, "triangle basis");
   }
+  const int age_status = ValidateAirGaps(m);
+  if (age_status != FEMM_OK)
+    return fail(age_status, "invalid air-gap element");
   if (topology != nullptr) {
     // Variant of an existing plan: same constraints and Dirichlet data, same DOF map.
     p->node_sign = topology->node_sign;
@@ -459,8 +530,12 @@ int BuildHostPlan(const femm_model_t *m, HostPlan *p, std::string *error, const 
     }
   }
 
-  if (topology != nullptr)
+  if (topology != nullptr) {
+    // A variant shares the base plan's air-gap elements and their fixed-node load.
+    for (size_t r = 0; r < topology->age_rhs.size(); ++r)
+      p->pm_rhs[r] = p->pm_rhs[r] + topology->age_rhs[r];
     return FEMM_OK;
+  }
 
   // Symbolic CSR: sorted unique (row, col) among free roots of every triangle
   // coupling (the rule of nonlinear_topology.cpp:37-85), built by a stable sort of
@@ -491,25 +566,47 @@ int BuildHostPlan(const femm_model_t *m, HostPlan *p, std::string *error, const 
       }
     }
   }
+  // Air-gap element couplings join the pattern (nonlinear_topology.cpp:46-59) but carry no
+  // triangle contributor (marker -1).
+  for (int32_t g = 0; g < m->air_gap_count; ++g)
+    for (int32_t k = 0; k < m->air_gaps[g].cells; ++k) {
+      int32_t node[10];
+      double weight[10];
+      AgeStencil(m->air_gaps[g], k, node, weight);
+      for (int i = 0; i < 10; ++i) {
+        const int32_t ri = p->node_to_free[node[i]];
+        if (ri < 0)
+          continue;
+        for (int j = 0; j < 10; ++j) {
+          const int32_t cj = p->node_to_free[node[j]];
+          if (cj < 0)
+            continue;
+          keys.push_back((static_cast<uint64_t>(ri) << 32) | static_cast<uint32_t>(cj));
+          packed.push_back(-1);
+        }
+      }
+    }
   std::vector<int32_t> order(keys.size());
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(),
                    [&keys](int32_t a, int32_t b) { return keys[a] < keys[b]; });
   p->row_offsets.assign(n + 1, 0);
-  p->contrib.resize(order.size());
+  p->contrib.clear();
+  p->contrib.reserve(static_cast<size_t>(T) * 9);
   p->slot_off.clear();
   uint64_t last = ~0ull;
   for (size_t k = 0; k < order.size(); ++k) {
     const uint64_t key = keys[order[k]];
     if (key != last) {
-      p->slot_off.push_back(static_cast<int32_t>(k));
+      p->slot_off.push_back(static_cast<int32_t>(p->contrib.size()));
       p->cols.push_back(static_cast<int32_t>(key & 0xffffffffu));
       p->row_offsets[(key >> 32) + 1]++;
       last = key;
     }
-    p->contrib[k] = packed[order[k]];
+    if (packed[order[k]] >= 0)
+      p->contrib.push_back(packed[order[k]]);
   }
-  p->slot_off.push_back(static_cast<int32_t>(order.size()));
+  p->slot_off.push_back(static_cast<int32_t>(p->contrib.size()));
   for (int32_t r = 0; r < n; ++r)
     p->row_offsets[r + 1] += p->row_offsets[r];
   p->nnz = static_cast<int32_t>(p->cols.size());
@@ -520,6 +617,14 @@ int BuildHostPlan(const femm_model_t *m, HostPlan *p, std::string *error, const 
         p->diag_slot[r] = k;
     if (p->diag_slot[r] < 0)
       return fail(FEMM_ASSEMBLY_FAILED, "missing diagonal");
+  }
+  if (m->air_gap_count > 0) {
+    const int age_built = BuildAirGapTerms(m, p);
+    if (age_built != FEMM_OK)
+      return fail(age_built, "air-gap element matrix");
+    // The rhs gather starts from pm_rhs[row] + age_rhs[row] (nonlinear_assembly.cu:155).
+    for (int32_t r = 0; r < n; ++r)
+      p->pm_rhs[r] = p->pm_rhs[r] + p->age_rhs[r];
   }
   std::vector<int32_t> rorder(rkeys.size());
   std::iota(rorder.begin(), rorder.end(), 0);

@@ -51,7 +51,14 @@ struct HostPlan {
   std::vector<int32_t> slot_off, contrib;
   // row-owned rhs contributors (packed: t << 3 | i << 1 | negative), element order
   std::vector<int32_t> row_off, rcontrib;
-  std::vector<double> pm_rhs, circuit_rhs;     // [n], [C][n]
+  // [n] PM load (plus age_rhs: the rhs gather's first sum, pm + age, done here once),
+  // [C][n] circuit loads per ampere
+  std::vector<double> pm_rhs, circuit_rhs;
+  // native air-gap elements: fixed stiffness pre-summed per CSR slot (added after the
+  // triangle terms) and the fixed-node load per row; empty without AGEs
+  std::vector<double> age_vals, age_rhs;       // [nnz], [n]
+  std::vector<femm_air_gap_t> ages;            // points re-pointed into age_points
+  std::vector<femm_age_point_t> age_points;
   // SELL-64
   int32_t nslices = 0, sell_size = 0;
   std::vector<int32_t> slice_ptr, slice_width, sell_cols, slot_to_sell, diag_sell;

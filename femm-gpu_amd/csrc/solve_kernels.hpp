@@ -427,6 +427,10 @@ __device__ void GatherValues(const DevicePlan &P, const Span &S, const double *e
                             ((c >> 1) & 3)];
       sum += (c & 1) ? -v : v;
     }
+    // The fixed air-gap element stiffness comes after every triangle term, as the host
+    // assembler emits it (nonlinear_assembly.cu:134-137).
+    if (P.age_vals != nullptr)
+      sum += P.age_vals[slot];
     const int pos = P.slot_to_sell[slot];
     vals[pos] = sum;
     if (vals32)
@@ -434,15 +438,15 @@ __device__ void GatherValues(const DevicePlan &P, const Span &S, const double *e
   }
 }
 
-// Row-owned rhs (physics/nonlinear_assembly.cu:142-186): PM + circuit sources, then the
-// ordered element terms.  Also the Jacobi inverse (:188-206).
+// Row-owned rhs (physics/nonlinear_assembly.cu:142-186): PM (+ air-gap element) load and
+// circuit sources, then the ordered element terms.  Also the Jacobi inverse (:188-206).
 __device__ bool GatherRhs(const DevicePlan &P, const Span &S, const double *elem,
                           const double *vals, const double *cur, double *rhs, double *inv,
                           float *inv32)
 {
   bool ok = true;
   for (int row = S.row0 + threadIdx.x; row < S.row1; row += kBlockThreads) {
-    double s = P.pm_rhs[row];
+    double s = P.pm_rhs[row];   // PM load + fixed air-gap element load, summed on the host
     for (int c = 0; c < P.C; ++c)
       s += cur[c] * P.circuit_rhs[static_cast<size_t>(c) * P.n + row];
     const int end = P.row_off[row + 1];
@@ -1328,7 +1332,8 @@ __device__ void TrueResidual(const DevicePlan &P, const double *vals, const doub
 // space is not restarted.  Acceptance is the reference's test on the FP64 true residual,
 // ||b - A x|| <= tol ||b||.  A breakdown in FP32, a replacement that fails to halve the
 // true residual, or running out of replacements finishes the solve in FP64 PCG
-// from the current x.
+// from the current x.  Called with maxit = 0 and fp64_maxit > 0, that FP64 PCG (at most
+// fp64_maxit iterations) is the whole solve.
 template <bool kBlock, class Team>
 __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, const double *inv,
                                      const float *vals32, const float *inv32,
@@ -1336,7 +1341,7 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
                                      const double *rhs, double *x, double *r, double *p,
                                      double *ap, float *d32, float *r32, float *p32, float *ap32,
                                      double tol, int maxit, bool warm, int max_updates,
-                                     double delta, Team &R)
+                                     double delta, Team &R, int fp64_maxit = 0)
 {
   RefinedInfo out{{FEMM_INTERNAL_ERROR, 0, INFINITY}, 0};
   constexpr bool block = kBlock;
@@ -1612,7 +1617,8 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
   // FP64 finish from the current x (a breakdown or stall of the FP32 iteration, or only the
   // final true-residual check).  With block-Jacobi the mixed form supports 4-row blocks.
   const LinearInfo li = Pcg64<false, kBlock ? 2 : 0>(P, vals, inv, B64, rhs, x, r, p, ap, tol,
-                                                     fallback ? max(0, maxit - it) : 0, true, R);
+                                                     fallback ? max(0, maxit - it) : fp64_maxit,
+                                                     true, R);
   out.linear.iterations += li.iterations;
   out.linear.residual = li.residual;
   out.linear.status = li.status;
@@ -1748,18 +1754,23 @@ __device__ __attribute__((always_inline)) double ForcingTolerance(const femm_new
 
 // Safeguarded acceptance: the linearization that meets both gates is solved to
 // cfg.acceptance_linear_tolerance when that is tighter than the options' exact tolerance,
-// and the item is then returned at that candidate.
+// and the item is then returned at that candidate.  An item whose tight re-solve did not
+// converge (`relaxed`: the FP64 residual floor of a large model can lie above it) falls
+// back to the options' exact tolerance, as without the setting.
 __device__ __attribute__((always_inline)) bool TightAcceptance(const femm_newton_options_t &o,
-                                                                const femm_solve_config_t &cfg)
+                                                                const femm_solve_config_t &cfg,
+                                                                bool relaxed = false)
 {
-  return cfg.acceptance_linear_tolerance > 0.0 &&
+  return !relaxed && cfg.acceptance_linear_tolerance > 0.0 &&
          cfg.acceptance_linear_tolerance < o.linear_relative_tolerance;
 }
 
 __device__ __attribute__((always_inline)) double AcceptTolerance(const femm_newton_options_t &o,
-                                                                  const femm_solve_config_t &cfg)
+                                                                  const femm_solve_config_t &cfg,
+                                                                  bool relaxed = false)
 {
-  return TightAcceptance(o, cfg) ? cfg.acceptance_linear_tolerance : o.linear_relative_tolerance;
+  return TightAcceptance(o, cfg, relaxed) ? cfg.acceptance_linear_tolerance
+                                          : o.linear_relative_tolerance;
 }
 
 // Inexact-Newton inner tolerance (physics/nonlinear_model.cpp:166-174).
@@ -1811,13 +1822,18 @@ __device__ bool OptionsValid(const femm_newton_options_t &o)
 
 // One linear solve of the current assembly into w.x (x0 = w.x when warm).
 // Solve forms (template argument of the solve kernels): FP64 or mixed precision, point or
-// block-Jacobi preconditioner.
+// block-Jacobi preconditioner.  With `fp64` a mixed form solves in FP64 (the FP64 finish
+// of ReliableSolve run from w.x for the whole solve): the safeguarded policy's acceptance
+// re-solve, whose solution is the returned state.  FP32 iterations with FP64 residual
+// replacement reach the same ||b - A x|| as FP64 PCG but leave more error in the soft
+// (low-reluctivity) modes that this norm hardly sees; the FP64 re-solve from the mixed
+// solution removes it.
 constexpr int kFormMixed = 1, kFormBlock = 2;
 
 template <int kForm, class Team>
 __device__ LinearInfo LinearSolve(const DevicePlan &P, const ItemPointers &w,
                                   const femm_solve_config_t &cfg, double tol, int maxit,
-                                  femm_item_result_t &res, Team &R)
+                                  femm_item_result_t &res, Team &R, bool fp64 = false)
 {
   constexpr bool kMixed = (kForm & kFormMixed) != 0, kBlock = (kForm & kFormBlock) != 0;
   LinearInfo li;
@@ -1833,9 +1849,10 @@ __device__ LinearInfo LinearSolve(const DevicePlan &P, const ItemPointers &w,
   if constexpr (kMixed) {
     const BlockPrec<float> B32{w.bjac32, bs, VectorStride(P.n)};
     const RefinedInfo ri = ReliableSolve<kBlock>(P, w.vals, w.inv, w.vals32, w.inv32, B64, B32, w.rhs, w.x,
-                                         w.r, w.p, w.ap, w.d32, w.r32, w.p32, w.ap32, tol, maxit,
-                                         cfg.pcg_warm_start != 0, cfg.max_refinements,
-                                         cfg.inner_relative_tolerance, R);
+                                         w.r, w.p, w.ap, w.d32, w.r32, w.p32, w.ap32, tol,
+                                         fp64 ? 0 : maxit, cfg.pcg_warm_start != 0,
+                                         cfg.max_refinements, cfg.inner_relative_tolerance, R,
+                                         fp64 ? maxit : 0);
     li = ri.linear;
     res.refinements += ri.refinements;
   } else {
@@ -1937,6 +1954,7 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
   double omega = 1.0, prev = INFINITY;
   bool pending = false;   // step gate passed; equation residual not yet checked
   bool exact_next = false; // safeguarded: re-solve the current linearization exactly
+  bool relaxed = false;    // safeguarded: the tight acceptance re-solve did not converge
   bool eq_current = false;
   for (int it = 0; status == FEMM_NONLINEAR_SOLVE_NOT_CONVERGED; ++it) {
     if (safeguarded) {
@@ -1971,7 +1989,7 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
       }
     }
     const double tol = !safeguarded ? LinearTolerance(o, adaptive, prev)
-                       : exact_next ? AcceptTolerance(o, cfg)
+                       : exact_next ? AcceptTolerance(o, cfg, relaxed)
                                     : ForcingTolerance(o, cfg, norm / scale);
     // A re-solve of the same linearization starts from its last solution.
     const bool resolve = exact_next;
@@ -1981,8 +1999,15 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
         w.x[i] = a[P.free_nodes[i]];
       R.Sync();
     }
-    const LinearInfo li = LinearSolve<kForm>(P, w, cfg, tol, o.max_linear_iterations, res, R);
+    const LinearInfo li =
+        LinearSolve<kForm>(P, w, cfg, tol, o.max_linear_iterations, res, R, resolve);
     if (li.status != FEMM_OK) {
+      if (resolve && li.status == FEMM_LINEAR_SOLVE_NOT_CONVERGED &&
+          TightAcceptance(o, cfg, relaxed)) {
+        relaxed = true;      // solve this linearization again to the exact tolerance
+        exact_next = true;
+        continue;
+      }
       status = li.status;
       break;
     }
@@ -1997,13 +2022,13 @@ __device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers 
     if (safeguarded) {
       const bool gates =
           step < 100.0 * o.relative_tolerance && norm / scale <= o.relative_tolerance;
-      if (gates && tol > AcceptTolerance(o, cfg)) {
+      if (gates && tol > AcceptTolerance(o, cfg, relaxed)) {
         // Both gates met on an inexact solve: solve this linearization again to the
         // acceptance tolerance (the assembly is still the current state's) before accepting.
         exact_next = true;
         continue;
       }
-      if (gates && !(cfg.inexact_forcing > 0.0) && !TightAcceptance(o, cfg)) {
+      if (gates && !(cfg.inexact_forcing > 0.0) && !TightAcceptance(o, cfg, relaxed)) {
         status = FEMM_OK;
         break;
       }

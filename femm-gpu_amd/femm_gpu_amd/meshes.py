@@ -22,7 +22,7 @@ import math
 
 import numpy as np
 
-from .model import AXISYMMETRIC, MU0, PLANAR, Material, Model
+from .model import AXISYMMETRIC, MU0, PLANAR, AirGap, Material, Model
 
 
 def tanh_bh_curve(ms=1.9, a=300.0, h_max=2e5, points=40):
@@ -440,3 +440,148 @@ def motor_postprocess(probes=32):
     angles = (np.arange(probes) + 0.5) * (0.5 * math.pi / probes)
     return dict(selected_group=ROTOR_GROUP, air_group=AIR_GROUP,
                 radius_m=AIRGAP_PROBE_MM * 1e-3, angles_rad=angles)
+
+
+class AgeMachine:
+    """Full 360 deg 12-slot / 4-pole surface-PM machine on polar grids of `cells` equal
+    angular cells, its air gap (34..35 mm) either a native air-gap element joining the
+    separately meshed rotor and stator rings (gap="age", FEMM's sliding band) or
+    `gap_intervals` layers of triangles (gap="triangles", a conforming mesh of the same
+    machine).  No jitter: every ring is uniform, as FEMM requires of the rings an air-gap
+    element joins.
+
+    Rotor (rotor frame): A = 0 on the shaft circle (10 mm), free space to 11 mm, B-H iron to
+    30 mm, four parallel-magnetised NdFeB poles (mu_r 1.05) between 30 and 34 mm spanning
+    90 k + 9 .. 90 k + 81 deg, magnetised along 90 k + 45 deg, alternately outward and
+    inward.  Stator: slot openings 35..37 mm (1.5 deg half-width), 12 slots 37..58 mm of
+    6 deg half-width centred at 15 + 30 m deg with coils 38..57 mm of phases A, -C, B, -A,
+    C, -B (repeating; 3 circuits, J per ampere = +-1), B-H iron elsewhere, A = 0 on the
+    outer circle (75 mm).  Labels follow element centroids.
+
+    Groups: 0 free space, 1 rotor body (iron and magnets), 2 stator iron, 5 + m slot m."""
+
+    ROTOR_RINGS_MM = ((10.0, 11.0, 1), (11.0, 30.0, 5), (30.0, 34.0, 2))
+    STATOR_RINGS_MM = ((35.0, 37.0, 1), (37.0, 38.0, 1), (38.0, 57.0, 4), (57.0, 58.0, 1),
+                       (58.0, 75.0, 3))
+    PHASES = ((0, 1.0), (2, -1.0), (1, 1.0), (0, -1.0), (2, 1.0), (1, -1.0))
+
+    def __init__(self, cells=96, gap="age", gap_intervals=2, radial_scale=1, sector=False):
+        if gap not in ("age", "triangles"):
+            raise ValueError("gap is 'age' or 'triangles'")
+        self.cells, self.gap, self.sector = int(cells), gap, bool(sector)
+        # sector: one pole pitch (90 deg), anti-periodic edges; else the full circle
+        self.span = 90.0 if self.sector else 360.0
+
+        def radii(bands):
+            r = [bands[0][0]]
+            for r0, r1, k in bands:
+                r += list(np.linspace(r0, r1, k * radial_scale + 1)[1:])
+            return np.asarray(r) * 1e-3
+
+        rotor_r, stator_r = radii(self.ROTOR_RINGS_MM), radii(self.STATOR_RINGS_MM)
+        if gap == "triangles":
+            gap_r = np.linspace(34.0, 35.0, gap_intervals + 1)[1:-1] * 1e-3
+            grids = [np.concatenate([rotor_r, gap_r, stator_r])]
+        else:
+            grids = [rotor_r, stator_r]
+        n = self.cells
+        m = n + 1 if self.sector else n            # nodes per ring
+        theta = np.arange(m) * (math.radians(self.span) / n)
+        xy, tri, rc, tc, offsets, pairs = [], [], [], [], [], []
+        base = 0
+        for r in grids:
+            offsets.append(base)
+            R, TH = np.meshgrid(r, theta, indexing="ij")
+            xy.append(np.stack([R * np.cos(TH), R * np.sin(TH)], -1).reshape(-1, 2))
+            for i in range(len(r) - 1):
+                for j in range(n):
+                    a, b = base + i * m + j, base + i * m + (j + 1) % m
+                    c, d = base + (i + 1) * m + j, base + (i + 1) * m + (j + 1) % m
+                    if (i + j) % 2 == 0:
+                        tri += [(a, c, d), (a, d, b)]
+                    else:
+                        tri += [(a, c, b), (b, c, d)]
+                    rc += [0.5 * (r[i] + r[i + 1]) * 1e3] * 2
+                    tc += [(j + 0.5) * self.span / n] * 2
+            if self.sector:   # the edge at 90 deg is the anti-periodic image of the one at 0
+                for i in range(len(r)):
+                    shaft_ring = r is grids[0] and i == 0
+                    outer_ring = r is grids[-1] and i == len(r) - 1
+                    if not (shaft_ring or outer_ring):
+                        pairs.append((base + i * m, base + i * m + n, -1))
+            base += len(r) * m
+        self.xy = np.concatenate(xy)
+        self.tri = _ccw(self.xy, np.asarray(tri, np.int32))
+        self.rc, self.tc = np.asarray(rc), np.asarray(tc)
+        self.area = _area(self.xy, self.tri)
+        self.pairs = np.asarray(pairs, np.int32).reshape(-1, 3)
+        last = grids[-1]
+        shaft = offsets[0] + np.arange(m)
+        outer = offsets[-1] + (len(last) - 1) * m + np.arange(m)
+        self.dirichlet = np.concatenate([shaft, outer]).astype(np.int32)
+        if gap == "age":
+            self.inner_ring = offsets[0] + (len(rotor_r) - 1) * m + np.arange(n)
+            self.outer_ring = offsets[1] + np.arange(n)
+
+    AIR, ROTOR, STATOR, MAG0, COIL0 = 0, 1, 2, 3, 7
+
+    def labels(self, rotor_steps=0):
+        """Per-triangle labels with the rotor's magnets turned by `rotor_steps` cells."""
+        rc, tc = self.rc, self.tc
+        phi = np.mod(tc - rotor_steps * self.span / self.cells, 360.0)
+        label = np.full(rc.shape, self.AIR, np.int32)
+        label[(rc > 11.0) & (rc < 30.0)] = self.ROTOR
+        band = (rc > 30.0) & (rc < 34.0)
+        for k in range(4):
+            label[band & (phi > 90 * k + 9) & (phi < 90 * k + 81)] = self.MAG0 + k
+        slot = np.rint((tc - 15.0) / 30.0).astype(int) % 12
+        dist = np.abs(np.mod(tc - (15.0 + 30.0 * slot) + 180.0, 360.0) - 180.0)
+        stator = rc > 35.0
+        in_slot = (rc > 37.0) & (rc < 58.0) & (dist < 6.0)
+        in_opening = (rc > 35.0) & (rc < 37.0) & (dist < 1.5)
+        label[stator & ~in_slot & ~in_opening] = self.STATOR
+        coil = (rc > 38.0) & (rc < 57.0) & (dist < 6.0)
+        label[coil] = self.COIL0 + slot[coil]
+        return label
+
+    def air_gap(self):
+        """The sliding-band air-gap element at rotor angle 0: record k holds the inner ring's
+        nodes k - 1 and k and the outer ring's (FEMM's preprocessor, femm/writepoly.cpp:
+        1649-1667), shifts 0 (node 0 of both rings at angle 0).  In the sector the ring
+        positions -1 and `cells` are the anti-periodic images of nodes cells - 1 and 0
+        (weight -1)."""
+        n = self.cells
+        k = np.arange(n + 1)
+        prev, cur = (k - 1) % n, k % n
+        nodes = np.stack([self.inner_ring[prev], self.inner_ring[cur], self.outer_ring[prev],
+                          self.outer_ring[cur]], 1)
+        weights = np.ones((n + 1, 4))
+        if self.sector:
+            weights[0, [0, 2]] = -1.0
+            weights[n, [1, 3]] = -1.0
+        return AirGap(inner_radius_m=34e-3, outer_radius_m=35e-3, arc_length_deg=self.span,
+                      nodes=nodes, weights=weights, antiperiodic=self.sector)
+
+    def model(self, hc=9e5, ms=1.9, rotor_steps=0, name=None):
+        tlabel = self.labels(rotor_steps)
+        b, h = tanh_bh_curve(ms=ms)
+        mats = [Material(1.0 / MU0, group=0), Material(1.0, bh_curve=0, group=1),
+                Material(1.0, bh_curve=0, group=2)]
+        turn = rotor_steps * self.span / self.cells
+        for k in range(4):
+            mats.append(Material(1.0 / (MU0 * 1.05), hc=hc,
+                                 magnetization_deg=90.0 * k + 45.0 + 180.0 * (k % 2) + turn,
+                                 group=1))
+        for m in range(12):
+            circuit, sign = self.PHASES[m % 6]
+            mats.append(Material(1.0 / MU0, source_j_per_a=sign, circuit=circuit, group=5 + m))
+        gaps = [self.air_gap()] if self.gap == "age" else []
+        kind = "sector" if self.sector else "full"
+        return Model(self.xy, self.tri, tlabel, mats, [(b, h)], self.dirichlet,
+                     np.zeros(len(self.dirichlet)), self.pairs, 3, 0.05, PLANAR,
+                     name=name or f"age_machine_{kind}_{self.gap}_{self.cells}", air_gaps=gaps)
+
+
+def age_machine(cells=96, gap="age", sector=False, **kw):
+    """:class:`AgeMachine` model at rotor angle 0."""
+    return AgeMachine(cells, gap, sector=sector).model(**kw)

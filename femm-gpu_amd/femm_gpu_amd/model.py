@@ -49,6 +49,24 @@ class CConstraint(ctypes.Structure):
     _fields_ = [("node_a", ctypes.c_int32), ("node_b", ctypes.c_int32), ("sign", ctypes.c_int32)]
 
 
+class CAgePoint(ctypes.Structure):
+    _fields_ = [("node", ctypes.c_int32 * 4), ("weight", ctypes.c_double * 4)]
+
+
+class CAirGap(ctypes.Structure):
+    _fields_ = [
+        ("antiperiodic", ctypes.c_int32),
+        ("cells", ctypes.c_int32),
+        ("center_xy", ctypes.c_double * 2),
+        ("inner_radius_m", ctypes.c_double),
+        ("outer_radius_m", ctypes.c_double),
+        ("arc_length_deg", ctypes.c_double),
+        ("inner_shift", ctypes.c_double),
+        ("outer_shift", ctypes.c_double),
+        ("points", ctypes.POINTER(CAgePoint)),
+    ]
+
+
 class CModel(ctypes.Structure):
     _fields_ = [
         ("problem_type", ctypes.c_int32),
@@ -70,6 +88,8 @@ class CModel(ctypes.Structure):
         ("constraints", ctypes.POINTER(CConstraint)),
         ("circuit_count", ctypes.c_int32),
         ("depth_m", ctypes.c_double),
+        ("air_gap_count", ctypes.c_int32),
+        ("air_gaps", ctypes.POINTER(CAirGap)),
     ]
 
 
@@ -129,6 +149,31 @@ class Material:
 
 
 @dataclass
+class AirGap:
+    """FEMM's native air-gap element (femm_air_gap_t): the annulus between an inner and an
+    outer ring cut into `cells` equal angular cells.  nodes / weights are [cells + 1, 4]:
+    per cell boundary the inner ring's (start, end) nodes and the outer ring's (start, end)
+    nodes, with weights +-1."""
+    inner_radius_m: float
+    outer_radius_m: float
+    arc_length_deg: float
+    nodes: np.ndarray
+    weights: np.ndarray
+    inner_shift: float = 0.0
+    outer_shift: float = 0.0
+    antiperiodic: bool = False
+    center_xy: tuple = (0.0, 0.0)
+
+    def __post_init__(self):
+        self.nodes = np.ascontiguousarray(self.nodes, dtype=np.int32).reshape(-1, 4)
+        self.weights = np.ascontiguousarray(self.weights, dtype=np.float64).reshape(-1, 4)
+
+    @property
+    def cells(self) -> int:
+        return int(self.nodes.shape[0]) - 1
+
+
+@dataclass
 class Model:
     node_xy: np.ndarray                      # [N, 2] float64, metres
     triangles: np.ndarray                    # [T, 3] int32, CCW
@@ -142,6 +187,7 @@ class Model:
     depth_m: float = 1.0
     problem_type: int = PLANAR
     name: str = "model"
+    air_gaps: list = field(default_factory=list)  # list[AirGap]
 
     def __post_init__(self):
         self.node_xy = np.ascontiguousarray(self.node_xy, dtype=np.float64).reshape(-1, 2)
@@ -187,6 +233,19 @@ class Model:
         for i, (a, b, s) in enumerate(self.constraints):
             cons[i] = CConstraint(int(a), int(b), int(s))
         self._mats, self._cons = mats, cons
+        gaps = (CAirGap * max(1, len(self.air_gaps)))()
+        self._age_points = []
+        for g, age in enumerate(self.air_gaps):
+            pts = (CAgePoint * max(1, age.nodes.shape[0]))()
+            for k in range(age.nodes.shape[0]):
+                pts[k].node[:] = [int(v) for v in age.nodes[k]]
+                pts[k].weight[:] = [float(w) for w in age.weights[k]]
+            self._age_points.append(pts)
+            gaps[g] = CAirGap(int(bool(age.antiperiodic)), age.cells,
+                              (ctypes.c_double * 2)(*age.center_xy), age.inner_radius_m,
+                              age.outer_radius_m, age.arc_length_deg, age.inner_shift,
+                              age.outer_shift, pts)
+        self._gaps = gaps
         self._c = CModel(
             self.problem_type, self.node_count, _ptr(self.node_xy, ctypes.c_double),
             self.triangle_count, _ptr(self.triangles, ctypes.c_int32),
@@ -195,7 +254,7 @@ class Model:
             _ptr(self._curve_b, ctypes.c_double), _ptr(self._curve_h, ctypes.c_double),
             len(self.dirichlet_nodes), _ptr(self.dirichlet_nodes, ctypes.c_int32),
             _ptr(self.dirichlet_values, ctypes.c_double), len(self.constraints), cons,
-            self.circuit_count, self.depth_m)
+            self.circuit_count, self.depth_m, len(self.air_gaps), gaps)
         return self._c
 
     def element_areas(self) -> np.ndarray:

@@ -71,11 +71,14 @@ def mixed_config(**kw) -> SolveConfig:
     """FP32 Jacobi PCG with FP64 reliable updates (true residual replaced whenever the FP32
     residual has dropped 10x), each Newton correction starting from the current state.  In
     the safeguarded policy (learned solves, femm_newton_batch_safeguarded) the step that meets
-    both gates is solved again to 1e-11 and returned at that candidate, so the state is fixed
-    to about 1e-11 times the operator's conditioning; the reference driver ignores it."""
+    both gates is solved again in FP64 PCG to 1e-12 and returned at that candidate, so the
+    state is fixed to about 1e-12 times the operator's conditioning (a re-solve that does not
+    converge falls back to the exact tolerance); the reference driver ignores it.  The warm
+    starts carry error in the soft (low-reluctivity) modes that a 1e-11 residual does not
+    see on every model: the FP64 re-solve has to iterate past it."""
     values = dict(precision=MIXED, pcg_warm_start=1, verify_equation_residual=1,
                   max_refinements=64, inner_relative_tolerance=0.1, partitions=1, block_size=1,
-                  acceptance_linear_tolerance=1e-11)
+                  acceptance_linear_tolerance=1e-12)
     values.update(kw)
     return SolveConfig(**values)
 
@@ -162,6 +165,10 @@ class Plan:
             _ip(out["free_nodes"]), _ip(out["node_to_free"]),
             out["node_sign"].ctypes.data_as(P(ctypes.c_int8)), _dp(out["boundary"])),
             "femm_plan_copy_symbolic")
+        out["age_values"], out["age_rhs"] = np.zeros(self.nnz), np.zeros(self.n)
+        _check(self._lib.femm_plan_copy_air_gap_terms(self._handle, _dp(out["age_values"]),
+                                                      _dp(out["age_rhs"])),
+               "femm_plan_copy_air_gap_terms")
         return out
 
     def assemble_batch(self, states, currents, newton):
@@ -302,7 +309,9 @@ class Plan:
     def set_postprocess(self, selected_label=-1, air_label=-1, selected_group=-1, air_group=-1,
                         radius_m=0.0, angles_rad=(), max_mask_iterations=256,
                         mask_relative_tolerance=1e-12):
-        """Weighted-stress mask and probe plan (BuildFrozenPostprocessPlan)."""
+        """Weighted-stress mask and probe plan (BuildFrozenPostprocessPlan); on a model with
+        air-gap elements the elements' own postprocess (Maxwell stress on the gap's mean
+        circle, Fourier radial B at `angles_rad`), which ignores the selections and radius."""
         self._angles = np.ascontiguousarray(angles_rad, np.float64)
         opts = PostprocessOptions(selected_label, air_label, selected_group, air_group,
                                   radius_m, _dp(self._angles), len(self._angles),

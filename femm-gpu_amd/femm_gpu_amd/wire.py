@@ -12,7 +12,7 @@ from pathlib import Path
 
 import numpy as np
 
-from .model import CMaterial, CModel, Material, Model
+from .model import AirGap, CAgePoint, CAirGap, CMaterial, CModel, Material, Model
 from .solver import library
 
 P = ctypes.POINTER
@@ -76,11 +76,43 @@ def parse_artifact(text, name="artifact"):
                           np.int32).reshape(-1, 3)
         currents = np.zeros(max(1, m.circuit_count))
         lib.femm_artifact_currents(handle, currents.ctypes.data_as(P(ctypes.c_double)))
+        gaps = [_air_gap(m.air_gaps[g]) for g in range(m.air_gap_count)]
         model = Model(xy, tri, mat, mats, curves, dn, dv, cons, m.circuit_count, m.depth_m,
-                      m.problem_type, name=str(name))
+                      m.problem_type, name=str(name), air_gaps=gaps)
         return model, currents[: m.circuit_count]
     finally:
         lib.femm_artifact_destroy(handle)
+
+
+def _air_gap(c) -> AirGap:
+    rec = [c.points[k] for k in range(c.cells + 1)]
+    return AirGap(inner_radius_m=c.inner_radius_m, outer_radius_m=c.outer_radius_m,
+                  arc_length_deg=c.arc_length_deg, nodes=[list(r.node) for r in rec],
+                  weights=[list(r.weight) for r in rec], inner_shift=c.inner_shift,
+                  outer_shift=c.outer_shift, antiperiodic=bool(c.antiperiodic),
+                  center_xy=(c.center_xy[0], c.center_xy[1]))
+
+
+def rotate_air_gaps(model: Model, angle_deg: float, posed_deg: float = 0.0) -> Model:
+    """The sliding-band model turned from `posed_deg` to `angle_deg` (femm_air_gaps_rotate:
+    the reference's ApplySlidingBandAngle, io/sample_request.cpp:448-491): a copy of `model`
+    whose air-gap elements carry the rotated inner records and shift."""
+    c = model.c_struct()
+    count = c.air_gap_count
+    total = sum(g.cells + 1 for g in model.air_gaps)
+    out = (CAirGap * max(1, count))()
+    pts = (CAgePoint * max(1, total))()
+    code = library().femm_air_gaps_rotate(count, c.air_gaps, ctypes.c_double(posed_deg),
+                                          ctypes.c_double(angle_deg), out, pts,
+                                          ctypes.c_size_t(total))
+    if code != 0:
+        raise ValueError(f"femm_air_gaps_rotate: status {code}")
+    rotated = Model(model.node_xy, model.triangles, model.triangle_material, model.materials,
+                    model.curves, model.dirichlet_nodes, model.dirichlet_values,
+                    model.constraints, model.circuit_count, model.depth_m, model.problem_type,
+                    name=f"{model.name}@{angle_deg:g}deg",
+                    air_gaps=[_air_gap(out[g]) for g in range(count)])
+    return rotated
 
 
 def load_artifact(path):
@@ -88,4 +120,5 @@ def load_artifact(path):
     return parse_artifact(Path(path).read_bytes(), name=str(path))
 
 
-__all__ = ["ArtifactError", "cli", "sha256_hex", "parse_artifact", "load_artifact"]
+__all__ = ["ArtifactError", "cli", "sha256_hex", "parse_artifact", "load_artifact",
+           "rotate_air_gaps"]

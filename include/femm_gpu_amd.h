@@ -97,6 +97,12 @@ void femm_plan_destroy(femm_plan *plan);
 /* sizes[6] = {node_count, triangle_count, free_count, nnz, circuit_count, quadrature_points}. */
 int femm_plan_sizes(const femm_plan *plan, int32_t sizes[6]);
 
+/* The fixed air-gap element terms of the plan (PreparedNonlinearAssembly::age_values and
+ * age_rhs, physics/nonlinear_topology.h:41-51; built at nonlinear_topology.cpp:180-232):
+ * age_values[nnz] per CSR slot, added after the triangle terms of every assembly, and
+ * age_rhs[free], the load of fixed nodes; zeros for a model without air-gap elements. */
+int femm_plan_copy_air_gap_terms(const femm_plan *plan, double *age_values, double *age_rhs);
+
 /* Symbolic CSR and DOF map (PreparedNonlinearAssembly::symbolic,
  * physics/nonlinear_topology.h:41-51): row_offsets[free+1], column_indices[nnz],
  * diagonal_slots[free], free_nodes[free], node_to_free[N], node_sign[N], boundary[N]. */
@@ -322,7 +328,14 @@ typedef struct femm_postprocess_options {
 /* Geometry-only postprocess plan: weighted-stress mask (Laplace solve on the GPU,
  * thresholded at 0.5), mask gradients and probe location.  Replaces
  * PrepareWeightedStressMask / SolveWeightedStressMask (postprocess/weighted_stress.cu:24-183)
- * and BuildFrozenPostprocessPlanWithMask (postprocess/field_postprocess.cpp:221-339). */
+ * and BuildFrozenPostprocessPlanWithMask (postprocess/field_postprocess.cpp:221-339).
+ * A plan whose model has air-gap elements gets their postprocess instead, as the reference's
+ * adapters choose for sliding-band artifacts (app/sample_adapter.cpp:59-61): force and torque
+ * from the Maxwell stress of every element's cells on its mean circle and radial B at the
+ * angles from the first element's discrete Fourier series (ComputeAirGapElementPostprocess,
+ * postprocess/airgap_postprocess.cpp:41-160); labels, groups, radius and mask settings are
+ * ignored, non-finite angles are FEMM_INVALID_ARGUMENT, and a first element whose cells and
+ * harmonics exceed 60 KiB of LDS (about 3,800 cells) is FEMM_UNSUPPORTED_FEATURE. */
 int femm_plan_set_postprocess(femm_plan *plan, const femm_postprocess_options_t *options);
 
 /* The 0/1 mask of the current postprocess plan, mask[node_count]. */

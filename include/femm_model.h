@@ -63,6 +63,32 @@ typedef struct femm_node_constraint {
   int32_t sign;
 } femm_node_constraint_t;
 
+/* One cell boundary of an air-gap element (reference AirGapQuadPoint,
+ * nonlinear_model.h:63-69): (node[0], weight[0]) and (node[1], weight[1]) are the inner-ring
+ * nodes at the start and end of the cell, (node[2], weight[2]) and (node[3], weight[3]) the
+ * outer-ring ones; weights are +1 or -1 (an anti-periodic image). */
+typedef struct femm_age_point {
+  int32_t node[4];
+  double weight[4];
+} femm_age_point_t;
+
+/* FEMM's native air-gap element (reference AirGapElement, nonlinear_model.h:71-81): the
+ * annulus between an inner and an outer ring, meshed independently, cut into `cells` equal
+ * angular cells; no triangles are built inside it.  inner_shift / outer_shift place each
+ * ring's nodes within a cell (fraction of a cell), so the rotor can turn without remeshing.
+ * Planar models only. */
+typedef struct femm_air_gap {
+  int32_t antiperiodic;            /* 1: the arc closes with a sign change */
+  int32_t cells;                   /* equal angular cells, >= 2 */
+  double center_xy[2];             /* m */
+  double inner_radius_m;
+  double outer_radius_m;
+  double arc_length_deg;           /* angular extent of the element */
+  double inner_shift;
+  double outer_shift;
+  const femm_age_point_t *points;  /* [cells + 1] */
+} femm_air_gap_t;
+
 typedef struct femm_model {
   int32_t problem_type;            /* FEMM_PLANAR or FEMM_AXISYMMETRIC (x = r, y = z) */
   int32_t node_count;
@@ -83,6 +109,8 @@ typedef struct femm_model {
   const femm_node_constraint_t *constraints;
   int32_t circuit_count;
   double depth_m;                  /* planar depth; ignored for r-z */
+  int32_t air_gap_count;           /* native air-gap elements (may be 0) */
+  const femm_air_gap_t *air_gaps;
 } femm_model_t;
 
 /* Newton policy (reference NonlinearOptions, nonlinear_model.h:99-106, and

@@ -49,6 +49,18 @@ int femm_artifact_model(const femm_artifact *artifact, femm_model_t *model);
 int femm_artifact_currents(const femm_artifact *artifact, double *currents);
 void femm_artifact_destroy(femm_artifact *artifact);
 
+/* The rotor angle of a sliding-band model (ApplySlidingBandAngle,
+ * gpu_solver/src/io/sample_request.cpp:448-491): the air-gap elements of a model posed at
+ * posed_deg, turned to angle_deg.  Each element's inner shift advances by
+ * (angle_deg - posed_deg) x cells / arc_length_deg; its whole cells rotate the inner-side
+ * (node 0, node 1) pairs of the first `cells` records cyclically and the last record's inner
+ * side repeats the first's; outer sides stay.  out[count] gets the elements, out_points (at
+ * least the sum of cells + 1, point_capacity) their records.  FEMM_INVALID_ARGUMENT for a
+ * non-finite angle difference, an element without cells or arc, or too small a buffer. */
+int femm_air_gaps_rotate(int32_t count, const femm_air_gap_t *gaps, double posed_deg,
+                         double angle_deg, femm_air_gap_t *out, femm_age_point_t *out_points,
+                         size_t point_capacity);
+
 /* SHA-256 of length bytes as 64 lower-case hex digits and a NUL (Sha256Hex,
  * io/strict_json.cpp:284-400): artifact identities bind the exact bytes on disk. */
 int femm_sha256_hex(const void *bytes, size_t length, char hex[65]);

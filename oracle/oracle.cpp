@@ -181,6 +181,14 @@ struct Model {
   int circuits = 1;
   double depth = 0.0;
   bool curves_ok = true;
+  // Native air-gap elements (AirGapElement, nonlinear_model.h:63-81).
+  struct AirGap {
+    bool antiperiodic = false;
+    int cells = 0;
+    double cx = 0.0, cy = 0.0, ri = 0.0, ro = 0.0, arc_deg = 0.0, ci = 0.0, co = 0.0;
+    std::vector<femm_age_point_t> points;   // [cells + 1]
+  };
+  std::vector<AirGap> ages;
 };
 
 bool LoadModel(const femm_model_t *d, Model *m)
@@ -213,6 +221,24 @@ bool LoadModel(const femm_model_t *d, Model *m)
   m->cons.assign(d->constraints, d->constraints + d->constraint_count);
   m->circuits = d->circuit_count;
   m->depth = d->depth_m;
+  if (d->air_gap_count < 0 || (d->air_gap_count > 0 && d->air_gaps == nullptr))
+    return false;
+  m->ages.resize(d->air_gap_count);
+  for (int g = 0; g < d->air_gap_count; ++g) {
+    const femm_air_gap_t &src = d->air_gaps[g];
+    Model::AirGap &a = m->ages[g];
+    a.antiperiodic = src.antiperiodic != 0;
+    a.cells = src.cells;
+    a.cx = src.center_xy[0];
+    a.cy = src.center_xy[1];
+    a.ri = src.inner_radius_m;
+    a.ro = src.outer_radius_m;
+    a.arc_deg = src.arc_length_deg;
+    a.ci = src.inner_shift;
+    a.co = src.outer_shift;
+    if (src.cells >= 0 && src.points != nullptr)
+      a.points.assign(src.points, src.points + src.cells + 1);
+  }
   return true;
 }
 
@@ -357,6 +383,114 @@ bool AxElement(const double r[3], const double z[3], AxPoint pts[7], double *are
 }
 
 // physics/nonlinear_model.cpp:223-319 (AGE checks omitted: this oracle has no AGE).
+// ---------------------------------------------------------------------------------------
+// Native air-gap element.  The cell matrix of BuildNativeFemmAirGapMatrix
+// (native_airgap_matrix.cpp:5-22; its text is fkn/prob1big.cpp:158-213), restated from
+// the element it is: a cell is the rectangle u, v in [0, 1] (arc, radius) whose potential
+// is bilinear between four virtual corners; each corner value is the cubic Hermite
+// interpolant of the ring nodes around it with central-difference slopes (Catmull-Rom), the
+// ring's nodes sitting at u = j - 2 + shift.  The stiffness K int (dA/du)^2 + Ki int
+// (dA/dv)^2 over the rectangle is taken with the mean of the 2 x 2 Gauss rule and the corner
+// (trapezoid) rule.  The product library computes the same matrix in closed form.
+
+constexpr double kAgeNu0 = 1.0 / (4.0e-7 * 3.141592653589793238462643383279502884);
+
+// Weights of p0..p3 in the Hermite cubic on [p1, p2] at parameter t, slopes (p2 - p0) / 2
+// and (p3 - p1) / 2.
+void HermiteCentral(double t, double w[4])
+{
+  const double h00 = 2 * t * t * t - 3 * t * t + 1, h10 = t * t * t - 2 * t * t + t;
+  const double h01 = -2 * t * t * t + 3 * t * t, h11 = t * t * t - t * t;
+  w[0] = -0.5 * h10;
+  w[1] = h00 - 0.5 * h11;
+  w[2] = h01 + 0.5 * h10;
+  w[3] = 0.5 * h11;
+}
+
+bool AgeMatrix(double K, double Ki, double ci, double co, double out[10][10])
+{
+  if (!(K > 0.0) || !(Ki > 0.0) || !std::isfinite(K) || !std::isfinite(Ki) ||
+      !std::isfinite(ci) || !std::isfinite(co))
+    return false;
+  double T[4][10] = {};
+  double wi[4], wo[4];
+  HermiteCentral(1.0 - ci, wi);
+  HermiteCentral(1.0 - co, wo);
+  for (int q = 0; q < 4; ++q) {
+    T[0][q] = wi[q];
+    T[1][q + 1] = wi[q];
+    T[2][q + 5] = wo[q];
+    T[3][q + 6] = wo[q];
+  }
+  // Bilinear shape gradients of corner c = (cu, cv) at (u, v).
+  auto du = [](int c, double v) { return ((c & 1) ? 1.0 : -1.0) * ((c >> 1) ? v : 1.0 - v); };
+  auto dv = [](int c, double u) { return ((c >> 1) ? 1.0 : -1.0) * ((c & 1) ? u : 1.0 - u); };
+  const double g = 0.5 / std::sqrt(3.0);
+  const double gauss[2] = {0.5 - g, 0.5 + g};
+  double S[4][4] = {};
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) {
+      double gq = 0.0, tq = 0.0;
+      for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y) {
+          gq += 0.25 * (K * du(a, gauss[y]) * du(b, gauss[y]) + Ki * dv(a, gauss[x]) * dv(b, gauss[x]));
+          tq += 0.25 * (K * du(a, y) * du(b, y) + Ki * dv(a, x) * dv(b, x));
+        }
+      S[a][b] = 0.5 * (gq + tq);
+    }
+  for (int i = 0; i < 10; ++i)
+    for (int j = 0; j < 10; ++j) {
+      double acc = 0.0;
+      for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b)
+          acc += T[a][i] * S[a][b] * T[b][j];
+      out[i][j] = acc;
+    }
+  return true;
+}
+
+// K, Ki and the relative shifts of nonlinear_model.cpp:573-587.
+void AgeParameters(const Model::AirGap &g, double *K, double *Ki, double *ci, double *co)
+{
+  const double dt = (kPi / 180.0) * g.arc_deg / static_cast<double>(g.cells);
+  *K = 2.0 * (g.ro - g.ri) / (dt * (g.ro + g.ri));
+  *Ki = 1.0 / *K;
+  *ci = g.ci;
+  *co = g.co;
+  if (*ci > *co) {
+    *ci -= *co;
+    *co = 0.0;
+  } else {
+    *ci = 1.0 - *co + *ci;
+    *co = 1.0;
+  }
+}
+
+// BuildAirGapStencil (postprocess/airgap_postprocess.cpp:10-38).
+void AgeStencil(const Model::AirGap &g, int k, int32_t node[10], double weight[10])
+{
+  const int n = g.cells;
+  const int prev = k == 0 ? n - 1 : k - 1, next = k + 1, next2 = k + 2 > n ? 1 : k + 2;
+  const femm_age_point_t &P = g.points[prev], &C = g.points[k], &X = g.points[next],
+                         &Y = g.points[next2];
+  const int32_t nodes[10] = {P.node[0], C.node[0], C.node[1], X.node[1], Y.node[1],
+                             P.node[2], C.node[2], C.node[3], X.node[3], Y.node[3]};
+  const double weights[10] = {P.weight[0], C.weight[0], C.weight[1], X.weight[1], Y.weight[1],
+                              P.weight[2], C.weight[2], C.weight[3], X.weight[3], Y.weight[3]};
+  for (int q = 0; q < 10; ++q) {
+    node[q] = nodes[q];
+    weight[q] = weights[q];
+  }
+  if (g.antiperiodic && k == 0) {
+    weight[0] = -weight[0];
+    weight[5] = -weight[5];
+  }
+  if (g.antiperiodic && k + 1 == n) {
+    weight[4] = -weight[4];
+    weight[9] = -weight[9];
+  }
+}
+
 int Validate(const Model &m)
 {
   if (m.nodes.empty() || m.tri.empty() || m.mat.empty() ||
@@ -405,6 +539,23 @@ int Validate(const Model &m)
     }
     Terms t;
     if (!ElementTerms(p, &t))
+      return FEMM_MESH_INVALID;
+  }
+  // nonlinear_model.cpp:281-297.
+  for (const Model::AirGap &g : m.ages) {
+    if (m.problem != FEMM_PLANAR)
+      return FEMM_UNSUPPORTED_FEATURE;
+    if (g.cells < 2 || g.points.size() != static_cast<size_t>(g.cells) + 1 || !(g.ri > 0.0) ||
+        !(g.ro > g.ri) || !(g.arc_deg > 0.0) || !std::isfinite(g.cx) || !std::isfinite(g.cy) ||
+        !std::isfinite(g.ci) || !std::isfinite(g.co))
+      return FEMM_MESH_INVALID;
+    for (const femm_age_point_t &pt : g.points)
+      for (int q = 0; q < 4; ++q)
+        if (pt.node[q] < 0 || static_cast<size_t>(pt.node[q]) >= m.nodes.size() ||
+            !std::isfinite(pt.weight[q]))
+          return FEMM_MESH_INVALID;
+    const double dt = g.arc_deg / static_cast<double>(g.cells);
+    if (!(dt > 0.0) || !std::isfinite(dt))
       return FEMM_MESH_INVALID;
   }
   DofMap d;
@@ -615,6 +766,22 @@ int Assemble(const Model &m, const std::vector<double> &a_old, const std::vector
       }
     }
   }
+  // Air-gap elements after every triangle (nonlinear_model.cpp:569-622).
+  for (const Model::AirGap &g : m.ages) {
+    double K, Ki, ci, co, mat[10][10];
+    AgeParameters(g, &K, &Ki, &ci, &co);
+    if (!AgeMatrix(K, Ki, ci, co, mat))
+      return FEMM_MESH_INVALID;
+    for (int k = 0; k < g.cells; ++k) {
+      int32_t node[10];
+      double weight[10];
+      AgeStencil(g, k, node, weight);
+      for (int i = 0; i < 10; ++i)
+        for (int j = 0; j < 10; ++j)
+          if (!AddEntry(&jac, node[i], node[j], mat[i][j] * weight[i] * weight[j] * kAgeNu0))
+            return FEMM_NUMERICAL_NONFINITE;
+    }
+  }
   for (const auto &row : jac)
     for (const auto &e : row)
       if (!std::isfinite(e.second))
@@ -641,6 +808,7 @@ struct Symbolic {
   std::vector<std::array<int32_t, 4>> mat_entries;
   std::vector<std::array<int32_t, 3>> rhs_entries;
   std::vector<double> pm_rhs, circuit_rhs;
+  std::vector<double> age_values, age_rhs;
 };
 
 // physics/nonlinear_topology.cpp:12-268 (planar and r-z; no AGE).  Per free
@@ -669,6 +837,16 @@ int PrepareSymbolic(const Model &m, Symbolic *s)
       for (int j = 0; j < 3; ++j)
         if (n2f[t[i]] >= 0 && n2f[t[j]] >= 0)
           st[n2f[t[i]]][n2f[t[j]]] = -1;
+  for (const Model::AirGap &g : m.ages)
+    for (int k = 0; k < g.cells; ++k) {
+      int32_t node[10];
+      double weight[10];
+      AgeStencil(g, k, node, weight);
+      for (int i = 0; i < 10; ++i)
+        for (int j = 0; j < 10; ++j)
+          if (n2f[node[i]] >= 0 && n2f[node[j]] >= 0)
+            st[n2f[node[i]]][n2f[node[j]]] = -1;
+    }
   Assembly &a = s->sym;
   a.free_nodes = d.free_roots;
   a.boundary = bval;
@@ -746,6 +924,29 @@ int PrepareSymbolic(const Model &m, Symbolic *s)
         if (n2f[t[i]] >= 0 && n2f[t[j]] >= 0)
           contrib[st[n2f[t[i]]][n2f[t[j]]]].push_back(
               {static_cast<int32_t>(e), i, j, d.sign[t[i]] * d.sign[t[j]]});
+    }
+  }
+  // Fixed AGE terms per slot and fixed-node loads (nonlinear_topology.cpp:180-232).
+  s->age_values.assign(nnz, 0.0);
+  s->age_rhs.assign(nf, 0.0);
+  for (const Model::AirGap &g : m.ages) {
+    double K, Ki, ci, co, mat[10][10];
+    AgeParameters(g, &K, &Ki, &ci, &co);
+    if (!AgeMatrix(K, Ki, ci, co, mat))
+      return FEMM_MESH_INVALID;
+    for (int k = 0; k < g.cells; ++k) {
+      int32_t node[10];
+      double weight[10];
+      AgeStencil(g, k, node, weight);
+      for (int i = 0; i < 10; ++i)
+        for (int j = 0; j < 10; ++j) {
+          const double value = mat[i][j] * weight[i] * weight[j] * kAgeNu0;
+          const int32_t row = n2f[node[i]], col = n2f[node[j]];
+          if (row >= 0 && col >= 0)
+            s->age_values[st[row][col]] += static_cast<double>(d.sign[node[i]] * d.sign[node[j]]) * value;
+          else if (row >= 0)
+            s->age_rhs[row] -= static_cast<double>(d.sign[node[i]]) * value * bval[node[j]];
+        }
     }
   }
   s->mat_offsets.assign(nnz + 1, 0);
@@ -1621,6 +1822,92 @@ bool SmoothedB(const Model &m, const std::vector<double> &a, const double *bx, c
   return true;
 }
 
+// ComputeAirGapElementPostprocess (postprocess/airgap_postprocess.cpp:41-160).  A cell's
+// field at its centre is the gradient of the bilinear interpolant of its four virtual
+// corners (Catmull-Rom values of the rings at the element's own shifts): Br across the arc
+// over R dt, Bt = -dA/dr across the gap.  Sums run over elements and cells in order.
+int AgePostprocess(const Model &m, const std::vector<double> &a, const double *angles, int count,
+                   double out[3], double *radial)
+{
+  if (m.ages.empty() || a.size() != m.nodes.size())
+    return FEMM_INVALID_ARGUMENT;
+  constexpr double kMu0 = 4.0e-7 * 3.141592653589793238462643383279502884;
+  out[0] = out[1] = out[2] = 0.0;
+  std::vector<double> first_br;
+  for (size_t g = 0; g < m.ages.size(); ++g) {
+    const Model::AirGap &age = m.ages[g];
+    const int n = age.cells;
+    const double dt = (kPi / 180.0) * age.arc_deg / static_cast<double>(n);
+    const double radius = 0.5 * (age.ri + age.ro), dr = age.ro - age.ri;
+    if (!(dt > 0.0) || !(radius > 0.0) || !(dr > 0.0) || n < 2)
+      return FEMM_MESH_INVALID;
+    double wi[4], wo[4];
+    HermiteCentral(1.0 - age.ci, wi);
+    HermiteCentral(1.0 - age.co, wo);
+    for (int k = 0; k < n; ++k) {
+      int32_t node[10];
+      double weight[10], v[10];
+      AgeStencil(age, k, node, weight);
+      for (int q = 0; q < 10; ++q)
+        v[q] = a[node[q]] * weight[q];
+      double corner[4] = {0.0, 0.0, 0.0, 0.0};   // inner u=0, inner u=1, outer u=0, outer u=1
+      for (int q = 0; q < 4; ++q) {
+        corner[0] += wi[q] * v[q];
+        corner[1] += wi[q] * v[q + 1];
+        corner[2] += wo[q] * v[q + 5];
+        corner[3] += wo[q] * v[q + 6];
+      }
+      const double br = ((corner[1] - corner[0]) + (corner[3] - corner[2])) / (2.0 * dt * radius);
+      const double bt = ((corner[0] + corner[1]) - (corner[2] + corner[3])) / (2.0 * dr);
+      if (!std::isfinite(br) || !std::isfinite(bt))
+        return FEMM_NUMERICAL_NONFINITE;
+      const double theta = (static_cast<double>(k) + 0.5) * dt;
+      const double normal = br * br - bt * bt, shear = 2.0 * br * bt;
+      const double scale = m.depth * radius * dt / (2.0 * kMu0);
+      out[0] += scale * (normal * std::cos(theta) - shear * std::sin(theta));
+      out[1] += scale * (normal * std::sin(theta) + shear * std::cos(theta));
+      out[2] += m.depth * radius * radius * dt * br * bt / kMu0;
+      if (g == 0)
+        first_br.push_back(br);
+    }
+  }
+  const Model::AirGap &age = m.ages.front();
+  const int n = age.cells;
+  const double dt = (kPi / 180.0) * age.arc_deg / static_cast<double>(n);
+  const size_t harmonics = age.antiperiodic ? static_cast<size_t>(n + 1) / 2
+                                            : static_cast<size_t>(n) / 2 + 1;
+  const int32_t base =
+      static_cast<int32_t>(std::lround((age.antiperiodic ? 180.0 : 360.0) / age.arc_deg));
+  std::vector<double> bc(harmonics, 0.0), bs(harmonics, 0.0);
+  std::vector<int32_t> order(harmonics);
+  for (size_t h = 0; h < harmonics; ++h) {
+    order[h] = age.antiperiodic ? base * static_cast<int32_t>(2 * h + 1)
+                                : base * static_cast<int32_t>(h);
+    for (int k = 0; k < n; ++k) {
+      const double phase = (static_cast<double>(k) + 0.5) * dt * static_cast<double>(order[h]);
+      bc[h] += first_br[k] * std::cos(phase);
+      bs[h] += first_br[k] * std::sin(phase);
+    }
+    const bool edge = order[h] == 0 || (!age.antiperiodic && h + 1 == harmonics && n % 2 == 0);
+    const double norm = edge ? static_cast<double>(n) : static_cast<double>(n) / 2.0;
+    bc[h] /= norm;
+    bs[h] /= norm;
+  }
+  for (int p = 0; p < count; ++p) {
+    if (!std::isfinite(angles[p]))
+      return FEMM_INVALID_ARGUMENT;
+    double value = 0.0;
+    for (size_t h = 0; h < harmonics; ++h) {
+      const double phase = static_cast<double>(order[h]) * angles[p];
+      value += bc[h] * std::cos(phase) + bs[h] * std::sin(phase);
+    }
+    radial[p] = value;
+  }
+  if (!std::isfinite(out[0]) || !std::isfinite(out[1]) || !std::isfinite(out[2]))
+    return FEMM_NUMERICAL_NONFINITE;
+  return FEMM_OK;
+}
+
 }  // namespace
 
 struct oracle_assembly {
@@ -1823,6 +2110,35 @@ void oracle_symbolic_copy(const oracle_symbolic *h, int32_t *row_offsets, int32_
 }
 
 void oracle_symbolic_destroy(oracle_symbolic *h) { delete h; }
+
+void oracle_symbolic_age(const oracle_symbolic *h, double *age_values, double *age_rhs)
+{
+  if (age_values)
+    std::copy(h->s.age_values.begin(), h->s.age_values.end(), age_values);
+  if (age_rhs)
+    std::copy(h->s.age_rhs.begin(), h->s.age_rhs.end(), age_rhs);
+}
+
+int oracle_age_postprocess(const femm_model_t *model, const double *a, const double *angles_rad,
+                           int angle_count, double out[3], double *radial_b)
+{
+  Model m;
+  if (!LoadModel(model, &m))
+    return FEMM_INVALID_ARGUMENT;
+  return AgePostprocess(m, std::vector<double>(a, a + m.nodes.size()), angles_rad, angle_count,
+                        out, radial_b);
+}
+
+int oracle_age_matrix(double K, double Ki, double ci, double co, double out[100])
+{
+  double m[10][10];
+  if (!AgeMatrix(K, Ki, ci, co, m))
+    return 0;
+  for (int i = 0; i < 10; ++i)
+    for (int j = 0; j < 10; ++j)
+      out[10 * i + j] = m[i][j];
+  return 1;
+}
 
 void oracle_pcg(int n, const int32_t *row_offsets, const int32_t *column_indices,
                 const double *values, const double *diagonal, const double *rhs,

@@ -57,6 +57,19 @@ void oracle_symbolic_copy(const oracle_symbolic *symbolic, int32_t *row_offsets,
                           int32_t *free_nodes, int32_t *node_to_free, int8_t *node_sign,
                           double *boundary_values);
 void oracle_symbolic_destroy(oracle_symbolic *symbolic);
+/* The fixed air-gap element terms (nonlinear_topology.cpp:180-232): age_values[nnz] per slot
+ * and age_rhs[free_count]. */
+void oracle_symbolic_age(const oracle_symbolic *symbolic, double *age_values, double *age_rhs);
+
+/* Air-gap element postprocess (ComputeAirGapElementPostprocess,
+ * postprocess/airgap_postprocess.cpp:41-160) of the nodal potential a[node_count]:
+ * out[3] = {Fx, Fy, torque}; radial_b[angle_count] from the first element's Fourier series. */
+int oracle_age_postprocess(const femm_model_t *model, const double *a, const double *angles_rad,
+                           int angle_count, double out[3], double *radial_b);
+
+/* The 10 x 10 air-gap element cell matrix (BuildNativeFemmAirGapMatrix,
+ * native_airgap_matrix.cpp:5-22), row-major into out[100]; 0 for invalid inputs. */
+int oracle_age_matrix(double K, double Ki, double ci, double co, double out[100]);
 
 /* Jacobi PCG on one CSR system (pcg_kernels.cu:10-344, sequential sums).
  * info[3] = {status, iterations, residual_l2}. */

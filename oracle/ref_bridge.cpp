@@ -3,15 +3,19 @@
 // oracle/_ref/libref_bridge.so.
 //
 // TEST INFRASTRUCTURE ONLY: used by tests to pin the CPU oracle against the
-// reference's host assembler, symbolic CSR builder, B-H spline and artifact
-// parser.  The reference's field postprocess is not bridged: it calls a helper
-// defined in a CUDA translation unit (postprocess/weighted_stress.cu).  It is loaded with RTLD_LAZY because the
-// reference's AGE matrix translation unit needs build-generated code and is
-// not compiled; models without air-gap elements never call it.
+// reference's host assembler, symbolic CSR builder, B-H spline, artifact
+// parser, sliding-band remap and air-gap element postprocess.  The reference's
+// weighted-stress postprocess is not bridged: it calls a helper defined in a CUDA
+// translation unit (postprocess/weighted_stress.cu).  It is loaded with RTLD_LAZY
+// because the reference's AGE matrix translation unit needs build-generated code and is
+// not compiled: the assembly entry points refuse models with air-gap elements
+// (FEMM_UNSUPPORTED_FEATURE) instead of reaching it.
 
 #include "physics/nonlinear_model.h"
 #include "physics/nonlinear_topology.h"
 #include "io/mesh_artifact.h"
+#include "io/sample_request.h"
+#include "postprocess/postprocess.h"
 
 #include "femm_model.h"
 
@@ -70,6 +74,27 @@ bool ToReference(const femm_model_t *d, NonlinearModel *m)
   m->depth_m = d->depth_m;
   m->problem_type = d->problem_type == FEMM_AXISYMMETRIC ? MagneticProblemType::kAxisymmetric
                                                           : MagneticProblemType::kPlanar;
+  for (int g = 0; g < d->air_gap_count; ++g) {
+    const femm_air_gap_t &src = d->air_gaps[g];
+    AirGapElement age;
+    age.antiperiodic = src.antiperiodic != 0;
+    age.center_x_m = src.center_xy[0];
+    age.center_y_m = src.center_xy[1];
+    age.inner_radius_m = src.inner_radius_m;
+    age.outer_radius_m = src.outer_radius_m;
+    age.arc_length_deg = src.arc_length_deg;
+    age.inner_shift = src.inner_shift;
+    age.outer_shift = src.outer_shift;
+    for (int k = 0; k <= src.cells; ++k) {
+      AirGapQuadPoint point;
+      for (int q = 0; q < 4; ++q) {
+        point.node[q] = src.points[k].node[q];
+        point.weight[q] = src.points[k].weight[q];
+      }
+      age.quad_points.push_back(point);
+    }
+    m->air_gap_elements.push_back(std::move(age));
+  }
   return true;
 }
 
@@ -136,6 +161,8 @@ int ref_assemble(const femm_model_t *d, const double *a_old, const double *curre
                  ref_assembly **out)
 {
   *out = nullptr;
+  if (d->air_gap_count > 0)
+    return static_cast<int>(Status::kUnsupportedFeature);
   NonlinearModel m;
   if (!ToReference(d, &m))
     return static_cast<int>(Status::kInvalidMaterial);
@@ -180,6 +207,8 @@ void ref_assembly_destroy(ref_assembly *h) { delete h; }
 int ref_prepare_symbolic(const femm_model_t *d, ref_symbolic **out)
 {
   *out = nullptr;
+  if (d->air_gap_count > 0)
+    return static_cast<int>(Status::kUnsupportedFeature);
   NonlinearModel m;
   if (!ToReference(d, &m))
     return static_cast<int>(Status::kInvalidMaterial);
@@ -311,5 +340,88 @@ void ref_artifact_copy(const void *handle, double *node_xy, int32_t *tri_nodes, 
 }
 
 void ref_artifact_destroy(void *handle) { delete static_cast<ParsedArtifact *>(handle); }
+
+/* The parsed artifact's air-gap elements: returns their count (-1 without a sliding band)
+ * and the number of records of all elements in *total; with non-null outputs,
+ * params[g][8] = {ri, ro, arc, inner shift, outer shift, centre x, centre y, antiperiodic}
+ * and every element's records concatenated into points. */
+int ref_artifact_air_gaps(const void *handle, double *params, femm_age_point_t *points,
+                          int32_t *total)
+{
+  const auto *p = static_cast<const ParsedArtifact *>(handle);
+  if (!p->artifact.has_sliding_band)
+    return -1;
+  size_t at = 0;
+  const auto &ages = p->artifact.model.air_gap_elements;
+  for (size_t g = 0; g < ages.size(); ++g) {
+    const AirGapElement &a = ages[g];
+    if (params) {
+      const double v[8] = {a.inner_radius_m, a.outer_radius_m, a.arc_length_deg, a.inner_shift,
+                           a.outer_shift, a.center_x_m, a.center_y_m, a.antiperiodic ? 1.0 : 0.0};
+      std::copy(v, v + 8, params + 8 * g);
+    }
+    for (const AirGapQuadPoint &qp : a.quad_points) {
+      if (points)
+        for (int q = 0; q < 4; ++q) {
+          points[at].node[q] = qp.node[q];
+          points[at].weight[q] = qp.weight[q];
+        }
+      ++at;
+    }
+  }
+  *total = static_cast<int32_t>(at);
+  return static_cast<int>(ages.size());
+}
+
+/* ApplySlidingBandAngle (io/sample_request.cpp:448-491) on a sliding-band artifact posed at
+ * rotor angle 0: the air-gap elements of the model rotated to angle_deg.  inner_shift[g] and
+ * points (every element's cells + 1 records, concatenated) are written; returns 1 on success. */
+int ref_sliding_band(const femm_model_t *d, double angle_deg, double *inner_shift,
+                     femm_age_point_t *points)
+{
+  GpuFemmMeshArtifact artifact;
+  if (!ToReference(d, &artifact.model))
+    return 0;
+  artifact.has_sliding_band = true;
+  artifact.rotor_angle_deg = 0.0;
+  NonlinearModel rotated;
+  if (!ApplySlidingBandAngle(angle_deg, artifact, &rotated))
+    return 0;
+  size_t at = 0;
+  for (size_t g = 0; g < rotated.air_gap_elements.size(); ++g) {
+    const AirGapElement &age = rotated.air_gap_elements[g];
+    inner_shift[g] = age.inner_shift;
+    for (const AirGapQuadPoint &qp : age.quad_points) {
+      for (int q = 0; q < 4; ++q) {
+        points[at].node[q] = qp.node[q];
+        points[at].weight[q] = qp.weight[q];
+      }
+      ++at;
+    }
+  }
+  return 1;
+}
+
+/* ComputeAirGapElementPostprocess (postprocess/airgap_postprocess.cpp:41-160) for the nodal
+ * potential a[node_count]: out[3] = {Fx, Fy, torque}; radial_b[angle_count] from the first
+ * element's Fourier series.  Returns the reference status. */
+int ref_age_postprocess(const femm_model_t *d, const double *a, const double *angles_rad,
+                        int angle_count, double out[3], double *radial_b)
+{
+  NonlinearModel m;
+  if (!ToReference(d, &m))
+    return static_cast<int>(Status::kInvalidMaterial);
+  NonlinearSolveResult solution;
+  solution.a_wb_per_m.assign(a, a + d->node_count);
+  FrozenPostprocessOptions options;
+  options.airgap_angles_rad.assign(angles_rad, angles_rad + angle_count);
+  const FrozenPostprocessResult r = ComputeAirGapElementPostprocess(m, solution, options);
+  out[0] = r.force_x_n;
+  out[1] = r.force_y_n;
+  out[2] = r.torque_nm;
+  for (size_t k = 0; k < r.airgap_samples.size() && k < static_cast<size_t>(angle_count); ++k)
+    radial_b[k] = r.airgap_samples[k].radial_b_t;
+  return static_cast<int>(r.status);
+}
 
 }  // extern "C"

@@ -186,14 +186,17 @@ def postprocess_ring():
 
 def write_artifact(path, model, currents, schema="gpu_femm_planar_dc_mesh_v1", turns=None,
                    source_sha="a" * 64, base_sha="b" * 64, rotor_angle_deg=0.0,
-                   displacement_mm=(0.0, 0.0)):
+                   displacement_mm=(0.0, 0.0), age_scalar=False, edit=None):
     """Write `model` as a strictly valid mesh artifact (the layout the reference's preparer
     writes, io/mesh_artifact.cpp:14-440): one material and one region per material label
     (group number = label, FEMM's region id = label), integer ids, named series circuits,
     node constraints for the magnetostatic schema, identities for the motor schema and, for the
     neutral schemas, canonical_identity_sha256 = SHA-256 of the canonical `resolved` (sorted
     keys, no whitespace, numbers as written).  `turns[label]` sets a coil region's turns
-    (default: its J per ampere x its area).  Returns the SHA-256 of the bytes written."""
+    (default: its J per ampere x its area).  The sliding-band schema gpu_femm_mesh_v2 also
+    carries the model's air-gap elements (mesh_artifact.cpp:610-697; `age_scalar` writes a
+    single element as an object, the spelling MATLAB's jsonencode produces).  `edit(resolved)`
+    may change the document before it is written.  Returns the SHA-256 of the bytes written."""
     import hashlib
     import json
 
@@ -212,7 +215,7 @@ def write_artifact(path, model, currents, schema="gpu_femm_planar_dc_mesh_v1", t
             t = 0.0
         regions.append(dict(id=i, group_number=i, material_id=i, circuit_index=int(m.circuit),
                             turns=float(t), pm_magnetization_deg=float(m.magnetization_deg)))
-    motor = schema == "gpu_femm_mesh_v1"
+    motor = schema in ("gpu_femm_mesh_v1", "gpu_femm_mesh_v2")
     resolved = dict(
         source_fem_sha256=source_sha,
         model=dict(depth_mm=model.depth_m * 1e3,
@@ -234,6 +237,22 @@ def write_artifact(path, model, currents, schema="gpu_femm_planar_dc_mesh_v1", t
         resolved["base_motor_fem_sha256"] = base_sha
         resolved["pose"] = dict(rotor_angle_deg=rotor_angle_deg,
                                 displacement_mm=list(displacement_mm))
+    if schema == "gpu_femm_mesh_v2":
+        ages = []
+        for g, age in enumerate(model.air_gaps):
+            points = [dict(n0=int(n[0]), w0=float(w[0]), n1=int(n[1]), w1=float(w[1]),
+                           n2=int(n[2]), w2=float(w[2]), n3=int(n[3]), w3=float(w[3]))
+                      for n, w in zip(age.nodes, age.weights)]
+            ages.append(dict(name=f"gap_{g}",
+                             periodicity="antiperiodic" if age.antiperiodic else "periodic",
+                             center_mm=[1e3 * age.center_xy[0], 1e3 * age.center_xy[1]],
+                             ri_mm=1e3 * age.inner_radius_m, ro_mm=1e3 * age.outer_radius_m,
+                             arc_length_deg=age.arc_length_deg, sector_count=age.cells,
+                             inner_shift=age.inner_shift, outer_shift=age.outer_shift,
+                             quad_points=points))
+        resolved["air_gap_elements"] = ages[0] if age_scalar and len(ages) == 1 else ages
+    if edit is not None:
+        edit(resolved)
     canonical = json.dumps(resolved, sort_keys=True, separators=(",", ":"))
     doc = dict(schema_version=schema, source_fem_sha256=source_sha,
                canonical_identity_sha256=hashlib.sha256(canonical.encode()).hexdigest(),

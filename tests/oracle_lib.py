@@ -141,10 +141,21 @@ def prepare_symbolic(model, lib="oracle"):
         _ip(out["rhs_entries"]), _dp(out["pm_rhs"]), _dp(out["circuit_rhs"]),
         _ip(out["free_nodes"]), _ip(out["node_to_free"]), _bp(out["node_sign"]),
         _dp(out["boundary"]))
+    if lib == "oracle":
+        out["age_values"], out["age_rhs"] = np.zeros(nnz), np.zeros(n)
+        l.oracle_symbolic_age(handle, _dp(out["age_values"]), _dp(out["age_rhs"]))
     getattr(l, pre + "symbolic_destroy")(handle)
     out["circuit_rhs"] = out["circuit_rhs"][: model.circuit_count * n].reshape(
         model.circuit_count, n)
     return 0, out
+
+
+def age_matrix(K, Ki, ci, co):
+    """The oracle's 10 x 10 air-gap element cell matrix, or None for invalid inputs."""
+    out = np.zeros(100)
+    ok = oracle().oracle_age_matrix(ctypes.c_double(K), ctypes.c_double(Ki), ctypes.c_double(ci),
+                                    ctypes.c_double(co), _dp(out))
+    return out.reshape(10, 10) if ok else None
 
 
 def pcg(rows, cols, values, diag, rhs, tol, maxit):
@@ -327,10 +338,60 @@ def ref_parse_artifact(path):
     depth, problem, circuits = c_d(), c_i32(), c_i32()
     l.ref_artifact_copy(handle, _dp(xy), _ip(tri), _ip(mat), mats, _ip(dn), _dp(currents), cons,
                         ctypes.byref(depth), ctypes.byref(problem), ctypes.byref(circuits))
+    ages = None
+    total = c_i32()
+    count = l.ref_artifact_air_gaps(handle, None, None, ctypes.byref(total))
+    if count >= 0:
+        from femm_gpu_amd.model import CAgePoint
+        params = np.zeros(max(1, 8 * count))
+        pts = (CAgePoint * max(1, total.value))()
+        l.ref_artifact_air_gaps(handle, _dp(params), pts, ctypes.byref(total))
+        recs = [pts[k] for k in range(total.value)]
+        ages = dict(params=params[: 8 * count].reshape(count, 8),
+                    nodes=np.array([list(r.node) for r in recs], np.int32).reshape(-1, 4),
+                    weights=np.array([list(r.weight) for r in recs]).reshape(-1, 4))
     l.ref_artifact_destroy(handle)
-    return dict(node_xy=xy.reshape(N, 2), triangles=tri.reshape(T, 3), triangle_material=mat,
+    out = dict(node_xy=xy.reshape(N, 2), triangles=tri.reshape(T, 3), triangle_material=mat,
                 materials=[(m.reluctivity_zero, m.alpha, m.source_j_per_a, m.hc,
                             m.magnetization_deg, m.bh_curve, m.circuit, m.group) for m in mats],
                 dirichlet_nodes=dn, currents=currents[: circuits.value],
                 constraints=[(c.node_a, c.node_b, c.sign) for c in list(cons)[:S]],
                 depth_m=depth.value, problem_type=problem.value, curve_count=K)
+    out["air_gaps"] = ages   # None without a sliding band
+    return out
+
+
+def ref_sliding_band(model, angle_deg):
+    """The reference's ApplySlidingBandAngle on `model` posed at 0 deg: (inner shifts per
+    element, records [total, 4] nodes, [total, 4] weights)."""
+    from femm_gpu_amd.model import CAgePoint
+    total = sum(g.cells + 1 for g in model.air_gaps)
+    shifts = np.zeros(max(1, len(model.air_gaps)))
+    pts = (CAgePoint * max(1, total))()
+    ok = ref().ref_sliding_band(ctypes.byref(model.c_struct()), ctypes.c_double(angle_deg),
+                                _dp(shifts), pts)
+    if not ok:
+        return None
+    nodes = np.array([list(pts[k].node) for k in range(total)], np.int32)
+    weights = np.array([list(pts[k].weight) for k in range(total)])
+    return shifts[: len(model.air_gaps)], nodes, weights
+
+
+def ref_age_postprocess(model, a, angles_rad):
+    """The reference's ComputeAirGapElementPostprocess: (status, [Fx, Fy, torque], radial B)."""
+    a = np.ascontiguousarray(a, np.float64)
+    angles = np.ascontiguousarray(angles_rad, np.float64)
+    out, radial = np.zeros(3), np.zeros(max(1, len(angles)))
+    st = ref().ref_age_postprocess(ctypes.byref(model.c_struct()), _dp(a), _dp(angles),
+                                   len(angles), _dp(out), _dp(radial))
+    return st, out, radial[: len(angles)]
+
+
+def age_postprocess(model, a, angles_rad):
+    """The oracle's air-gap element postprocess: (status, [Fx, Fy, torque], radial B)."""
+    a = np.ascontiguousarray(a, np.float64)
+    angles = np.ascontiguousarray(angles_rad, np.float64)
+    out, radial = np.zeros(3), np.zeros(max(1, len(angles)))
+    st = oracle().oracle_age_postprocess(ctypes.byref(model.c_struct()), _dp(a), _dp(angles),
+                                         len(angles), _dp(out), _dp(radial))
+    return st, out, radial[: len(angles)]

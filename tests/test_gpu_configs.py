@@ -108,10 +108,13 @@ def test_config3_part1_motor_256_items_chosen_form():
     say(f"  chosen form: {time.perf_counter() - t0:.2f} s, pass {int(_passing(res).sum())}/256, "
         f"newton {res['newton_iterations'].mean():.2f}, pcg {res['linear_iterations'].mean():.0f}")
     assert _passing(res).all()
-    # The same items past the gates: FP64 safeguarded Newton from the chosen states.
+    # The same items past the gates: FP64 safeguarded Newton from the chosen states scaled
+    # by 1 + 1e-6 (a chosen state can already meet the tight gates; the perturbed start makes
+    # every item converge again on its own).
     conv = Batch(torch, plan, cur, len(post["angles_rad"]))
+    start = (out.a * (1.0 + 1e-6)).contiguous()
     plan.solve_batch_device(conv.cur.data_ptr(), 256, *conv.ptrs(), stream,
-                            accepted_options(**TIGHT), reference_config(), warm_ptr=out.a.data_ptr(),
+                            accepted_options(**TIGHT), reference_config(), warm_ptr=start.data_ptr(),
                             safeguarded=True)
     plan.postprocess_batch_device(256, conv.a.data_ptr(), conv.loads.data_ptr(),
                                   conv.radial.data_ptr(), stream)

@@ -150,3 +150,64 @@ def test_motor_batch_reproduces_frozen_femm(tmp_path):
     assert sample.run_motor_sample(str(req), str(single)) == 0
     assert json.loads(single.read_text(encoding="utf-8")) == r
     assert doc["cache"]["chunk_count"] == 2 and doc["cache"]["mesh_cache_hits"] == 3
+
+
+def test_sliding_band_motor_sample_and_batch(tmp_path):
+    """gpu_femm_motor_sample_v1 / gpu_femm_motor_batch_v1 on a sliding-band artifact
+    (gpu_femm_mesh_v2, a full 12-slot / 4-pole machine whose air gap is a native air-gap
+    element): the adapter turns the element to each item's rotor angle
+    (ApplySlidingBandRotorAngle) and reports the element's Maxwell-stress torque and Fourier
+    air-gap B (app/sample_adapter.cpp:59-61).  Against the oracle on the same turned model with
+    the production options: flux linkage within 1e-7, torque and forces within 1e-7 of the
+    torque, radial B within 1e-7; batch items equal the single-sample responses bit for bit,
+    and items at the same angle share one operator."""
+    import oracle_lib as ol
+    from femm_gpu_amd import meshes, wire
+    from femm_gpu_amd.model import production_options
+    say("case sliding-band motor sample and batch")
+    fam = meshes.AgeMachine(96, "age")
+    model = fam.model()
+    currents = [3e6, -1.5e6, -1.5e6]
+    art = tmp_path / "machine.v2.json"
+    sha = ff.write_artifact(art, model, currents, schema="gpu_femm_mesh_v2")
+    angles_deg = [5.0, 40.0, 95.0, 200.0]
+    base = dict(protocol=sample.MOTOR, mesh_artifact_path=str(art), mesh_artifact_sha256=sha,
+                base_motor_fem_sha256="b" * 64, source_fem_sha256="a" * 64,
+                circuit_currents_A=currents, selected_group_number=1, air_group_number=0,
+                airgap_radius_mm=34.5, airgap_angles_deg=angles_deg, rotor_angle_deg=0.0,
+                displacement_mm=[0.0, 0.0])
+    singles = {}
+    for rotor in (0.0, 1.3, 2.6):
+        req, resp = tmp_path / f"m{rotor}.json", tmp_path / f"m{rotor}.out.json"
+        req.write_text(json.dumps(dict(base, rotor_angle_deg=rotor)), encoding="utf-8")
+        assert sample.run_motor_sample(str(req), str(resp)) == 0
+        doc = json.loads(resp.read_text(encoding="utf-8"))
+        assert (doc["status"], doc["solve_status"]) == ("PASS", "OK")
+        singles[rotor] = doc
+        turned = wire.rotate_air_gaps(model, rotor)
+        ref = ol.newton_solve(turned, np.asarray(currents), production_options())
+        assert ref["status"] == 0
+        st, loads, radial = ol.age_postprocess(turned, ref["a"],
+                                               np.radians(np.asarray(angles_deg)))
+        assert st == 0
+        flux = np.asarray(doc["circuit_flux_linkage_Wb"])
+        got = np.array([doc["Fx_N"], doc["Fy_N"], doc["torque_Nm"]])
+        say(f"  rotor {rotor} deg: torque {got[2]:.9e} (oracle {loads[2]:.9e}), flux "
+            f"{np.abs(flux - ref['flux']).max() / np.abs(ref['flux']).max():.2e}")
+        assert np.abs(flux - ref["flux"]).max() <= 1e-7 * np.abs(ref["flux"]).max()
+        assert np.abs(got - loads).max() <= 1e-7 * abs(loads[2])
+        rb = np.asarray(doc["airgap_radial_flux_density_T"])
+        assert np.abs(rb - radial).max() <= 1e-7 * np.abs(radial).max()
+    assert singles[0.0]["torque_Nm"] != singles[1.3]["torque_Nm"]
+    items = [("a", 1.3), ("b", 0.0), ("c", 2.6), ("d", 1.3)]
+    batch = dict(protocol=sample.MOTOR_BATCH, max_items_per_chunk=4,
+                 items=[dict(task_id=t, request=dict(base, rotor_angle_deg=r)) for t, r in items])
+    req, resp = tmp_path / "batch.json", tmp_path / "batch.out.json"
+    req.write_text(json.dumps(batch), encoding="utf-8")
+    assert sample.run_motor_batch(str(req), str(resp)) == 0
+    doc = json.loads(resp.read_text(encoding="utf-8"))
+    # batch level: PASS or PARTIAL_FAILURE (app/batch_response.cpp:297)
+    assert (doc["status"], doc["solve_status"]) == ("PASS", "PASS")
+    for item, (t, rotor) in zip(doc["items"], items):
+        assert item["task_id"] == t and item["response"] == singles[rotor], t
+    assert doc["cache"]["chunk_count"] == 1

@@ -518,7 +518,7 @@ def test_assembly_and_symbolic_bitwise_equal_reference_sources(case):
     st_o, smine = ol.prepare_symbolic(model)
     st_r, stheirs = ol.prepare_symbolic(model, lib="ref")
     assert st_o == st_r == 0
-    for key in smine:
+    for key in stheirs:
         assert np.array_equal(smine[key], stheirs[key]), key
     # The symbolic pattern equals the numeric assembly's pattern (zero slots kept).
     assert np.array_equal(smine["rows"], mine["rows"])

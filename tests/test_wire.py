@@ -165,8 +165,11 @@ def _mutations():
         "invalid periodic node constraint"
     yield "constraint_self", res(lambda r: r["node_constraints"][0].__setitem__(
         "node_b", r["node_constraints"][0]["node_a"])), "invalid periodic node constraint"
+    # a planar-DC document relabelled as a sliding-band motor mesh lacks the motor header
     yield "sliding_band_schema", lambda d: json.dumps(dict(d, schema_version="gpu_femm_mesh_v2")), \
-        "unsupported artifact schema: sliding-band and sector motor meshes"
+        "unexpected JSON object fields"
+    yield "sector_schema", lambda d: json.dumps(dict(d, schema_version="gpu_femm_mesh_v4")), \
+        "unsupported artifact schema: sector motor meshes"
     yield "duplicate_key", lambda d: json.dumps(d)[:-1] + ', "schema_version": "x"}', \
         "duplicate JSON object key"
     yield "trailing", lambda d: json.dumps(d) + " 0", "trailing JSON data"
