@@ -49,7 +49,7 @@ __global__ __launch_bounds__(kBlockThreads) void SpmvKernel(DevicePlan P, Device
   for (int rep = 0; rep < reps; ++rep) {
     // The PCG's own product: two slices per step in FP32, one in FP64.
     if constexpr (sizeof(T) == 4)
-      Spmv2<T, T>(P, WholeItem(P), vals, x + item * n, y + item * n,
+      Spmv2Fit<T, T>(P, WholeItem(P), vals, x + item * n, y + item * n,
                   static_cast<const T *>(nullptr));
     else
       Spmv<T, T, true>(P, WholeItem(P), vals, x + item * n, y + item * n,
@@ -448,6 +448,9 @@ int UploadPlan(const HostPlan &h, DeviceBuffers *dev)
   P.Q = h.Q;
   P.nslices = h.nslices;
   P.sell_size = h.sell_size;
+  P.max_slice_width = 0;
+  for (const int32_t w : h.slice_width)
+    P.max_slice_width = std::max(P.max_slice_width, w);
   P.planar = h.planar;
   P.any_nonlinear = h.any_nonlinear ? 1 : 0;
   P.any_boundary_value = h.any_boundary_value ? 1 : 0;
@@ -584,9 +587,9 @@ size_t WorkBytes(const HostPlan &h, int32_t capacity, bool mixed, bool blocks)
   const size_t ci = static_cast<size_t>(capacity);
   const size_t nv = static_cast<size_t>(VectorStride(h.n));   // per-item vector stride
   size_t bytes = Bytes<double>(ci * h.T * kElemStride) + Bytes<double>(ci * h.sell_size) +
-                 6 * Bytes<double>(ci * nv) + Bytes<double>(ci * h.N);
+                 8 * Bytes<double>(ci * nv) + Bytes<double>(ci * h.N);
   if (mixed)
-    bytes += Bytes<float>(ci * h.sell_size) + 5 * Bytes<float>(ci * nv);
+    bytes += Bytes<float>(ci * h.sell_size) + 7 * Bytes<float>(ci * nv);
   if (blocks)
     bytes += Bytes<double>(ci * kMaxBlock * nv) + (mixed ? Bytes<float>(ci * kMaxBlock * nv) : 0);
   bytes += Bytes<double>(ci * 2 * kMaxPartitions * 4) + Bytes<uint32_t>(ci * kBarrierWords) +
@@ -609,6 +612,8 @@ DeviceWork CarveWork(const HostPlan &h, int32_t capacity, bool mixed, bool block
   w.r = Carve<double>(cur, ci * nv);
   w.p = Carve<double>(cur, ci * nv);
   w.ap = Carve<double>(cur, ci * nv);
+  w.p_alt = Carve<double>(cur, ci * nv);
+  w.q_buf = Carve<double>(cur, ci * nv);
   w.trial = Carve<double>(cur, ci * h.N);
   if (mixed) {
     w.vals32 = Carve<float>(cur, ci * h.sell_size);
@@ -617,6 +622,8 @@ DeviceWork CarveWork(const HostPlan &h, int32_t capacity, bool mixed, bool block
     w.ap32 = Carve<float>(cur, ci * nv);
     w.d32 = Carve<float>(cur, ci * nv);
     w.inv32 = Carve<float>(cur, ci * nv);
+    w.p32_alt = Carve<float>(cur, ci * nv);
+    w.q32_buf = Carve<float>(cur, ci * nv);
   }
   if (blocks) {
     w.bjac = Carve<double>(cur, ci * kMaxBlock * nv);

@@ -19,6 +19,7 @@ constexpr int kElemStride = 12;       // per-triangle 3x3 Newton matrix + 3 rhs 
 // Device pointers of the immutable plan (uploaded once per geometry).
 struct DevicePlan {
   int32_t N, T, n, nnz, C, Q, nslices, sell_size;
+  int32_t max_slice_width;      // widest SELL slice
   int32_t planar, any_nonlinear, any_boundary_value;
   double depth;
   const int32_t *tri_nodes, *tri_mat, *tri_circuit;
@@ -45,8 +46,10 @@ struct DeviceWork {
   double *vals;      // [items][sell_size]
   float *vals32;     // [items][sell_size]   (mixed precision)
   double *rhs, *inv, *x, *r, *p, *ap;   // [items][n]
+  double *p_alt, *q_buf;                // [items][n] second direction and A p of team solves
   double *trial;     // [items][N] Armijo trial state (safeguarded Newton)
   float *r32, *p32, *ap32, *d32, *inv32; // [items][n]   (mixed precision)
+  float *p32_alt, *q32_buf;             // [items][n]   (mixed precision, team solves)
   double *bjac;      // [items][kMaxBlock][n] block-Jacobi planes (block_size > 1 only)
   float *bjac32;     // [items][kMaxBlock][n] (mixed precision with block-Jacobi)
   // Partitioned solves (several workgroups per item): per-item reduction partials

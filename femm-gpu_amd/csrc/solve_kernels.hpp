@@ -219,9 +219,12 @@ struct GridTeam {
   int G, g, buf;
   uint32_t passed = 0;   // barriers passed so far (the same count in every member)
 
+  // A team of one workgroup (G == 1) reduces and synchronises like BlockTeam.
   __device__ __attribute__((always_inline)) void Sync()
   {
     __syncthreads();
+    if (G == 1)
+      return;
     ++passed;
     if (threadIdx.x < 64)
       TeamBarrier(bar, abort, G, g, passed);
@@ -232,6 +235,8 @@ struct GridTeam {
   __device__ __attribute__((always_inline)) void Sum(double (&v)[K])
   {
     R.Sum(v);
+    if (G == 1)
+      return;
     double *mine = partials + (static_cast<size_t>(buf) * G + g) * 4;
     if (threadIdx.x == 0) {
 #pragma unroll
@@ -253,6 +258,8 @@ struct GridTeam {
   __device__ __attribute__((always_inline)) double Max(double v)
   {
     v = R.Max(v);
+    if (G == 1)
+      return v;
     double *mine = partials + (static_cast<size_t>(buf) * G + g) * 4;
     if (threadIdx.x == 0)
       mine[0] = v;
@@ -340,7 +347,7 @@ __device__ bool EvalBh(const DevicePlan &P, int off, int cnt, double field, doub
 // dnu/dB^2, then the 3x3 Newton block K + C and the per-node rhs terms
 // C a_old - (K + C) A_boundary (physics/nonlinear_assembly.cu:56-98 and the
 // term-by-term formulas of :100-186).  Returns true if every element is valid.
-__device__ bool ElementPass(const DevicePlan &P, const Span &S, const double *a, double *elem,
+__device__ __attribute__((always_inline)) bool ElementPass(const DevicePlan &P, const Span &S, const double *a, double *elem,
                             bool newton)
 {
   bool ok = true;
@@ -415,7 +422,7 @@ __device__ bool ElementPass(const DevicePlan &P, const Span &S, const double *a,
 // Slot-owned ordered gather into SELL positions (physics/nonlinear_assembly.cu:100-140):
 // contributor order is the host element/i/j emission order; one lane owns a slot.
 template <typename Real32>
-__device__ void GatherValues(const DevicePlan &P, const Span &S, const double *elem, double *vals,
+__device__ __attribute__((always_inline)) void GatherValues(const DevicePlan &P, const Span &S, const double *elem, double *vals,
                              Real32 *vals32)
 {
   for (int slot = S.slot0 + threadIdx.x; slot < S.slot1; slot += kBlockThreads) {
@@ -440,7 +447,7 @@ __device__ void GatherValues(const DevicePlan &P, const Span &S, const double *e
 
 // Row-owned rhs (physics/nonlinear_assembly.cu:142-186): PM (+ air-gap element) load and
 // circuit sources, then the ordered element terms.  Also the Jacobi inverse (:188-206).
-__device__ bool GatherRhs(const DevicePlan &P, const Span &S, const double *elem,
+__device__ __attribute__((always_inline)) bool GatherRhs(const DevicePlan &P, const Span &S, const double *elem,
                           const double *vals, const double *cur, double *rhs, double *inv,
                           float *inv32)
 {
@@ -477,26 +484,34 @@ __device__ bool GatherRhs(const DevicePlan &P, const Span &S, const double *elem
 // flight; every slice costs one memory round trip instead of two.  The sum order
 // over a row's entries is the CSR order either way.
 constexpr int kUnrolledWidth = 10;
+constexpr int kNarrowWidth = 9;   // the FP32 product's unroll when no slice is wider
 
-template <typename T>
+template <typename T, int U = kUnrolledWidth>
 struct SliceLoads {
-  int c[kUnrolledWidth];
-  T a[kUnrolledWidth];
+  int c[U];
+  T a[U];
   int width;
 };
+
+// Row of lane `lane` of slice s, -1 for padding lanes past the last row.
+__device__ inline int SellRow(const DevicePlan &P, int s, int lane)
+{
+  const int row = s * kSliceRows + lane;
+  return row < P.n ? row : -1;
+}
 
 // Branch-free loads of one slice: entries past the slice width repeat the last column
 // (a valid address in the same cache line) with a zero value, so the whole slice is one
 // straight-line run of loads and later adds only +-0 terms.
-template <typename T>
+template <typename T, int U>
 __device__ inline void LoadSlice(const DevicePlan &P, const T *__restrict__ vals, int s, int lane,
-                                 SliceLoads<T> &L)
+                                 SliceLoads<T, U> &L)
 {
   const int base = P.slice_ptr[s] + lane;
   L.width = P.slice_width[s];
   const int last = L.width - 1;   // every slice holds a diagonal, so width >= 1
 #pragma unroll
-  for (int k = 0; k < kUnrolledWidth; ++k) {
+  for (int k = 0; k < U; ++k) {
     const int pos = base + max(min(k, last), 0) * kSliceRows;
     L.c[k] = P.sell_cols[pos];
     const T value = vals[pos];
@@ -520,7 +535,10 @@ __device__ inline T WideSlice(const DevicePlan &P, const T *__restrict__ vals,
 
 // Two slices per wave per step (slices s and s + kWaves): twice the gathers in flight per
 // round trip for the FP32 iterations, whose loads take half the registers of FP64 ones.
-template <typename T, typename V>
+// U entries per slice are loaded whatever its width (the last column repeated): a plan
+// whose slices are all at most kNarrowWidth wide takes the narrower unroll (Spmv2Fit),
+// one load of each kind fewer per slice.
+template <typename T, typename V, int U = kUnrolledWidth>
 __device__ __attribute__((always_inline)) double Spmv2(const DevicePlan &P, const Span &S,
                                                         const T *__restrict__ vals,
                         const V *__restrict__ v, V *__restrict__ y, const V *__restrict__ w)
@@ -531,29 +549,30 @@ __device__ __attribute__((always_inline)) double Spmv2(const DevicePlan &P, cons
   if (first >= S.slice1)
     return local;
   auto clamp = [&](int s) { return s < S.slice1 ? s : first; };
-  SliceLoads<T> c0, c1;
+  SliceLoads<T, U> c0, c1;
   LoadSlice(P, vals, first, lane, c0);
   LoadSlice(P, vals, clamp(first + kWaves), lane, c1);
   for (int s = first; s < S.slice1; s += 2 * kWaves) {
     const int s1 = s + kWaves;
+    const int row0 = SellRow(P, s, lane), row1 = s1 < S.slice1 ? SellRow(P, s1, lane) : -1;
     T sum0 = 0, sum1 = 0;
-    if (c0.width <= kUnrolledWidth && c1.width <= kUnrolledWidth) {
-      V g0[kUnrolledWidth], g1[kUnrolledWidth];
+    if (c0.width <= U && c1.width <= U) {
+      V g0[U], g1[U];
 #pragma unroll
-      for (int k = 0; k < kUnrolledWidth; ++k)
+      for (int k = 0; k < U; ++k)
         g0[k] = v[c0.c[k]];
 #pragma unroll
-      for (int k = 0; k < kUnrolledWidth; ++k)
+      for (int k = 0; k < U; ++k)
         g1[k] = v[c1.c[k]];
-      SliceLoads<T> n0, n1;
+      SliceLoads<T, U> n0, n1;
       // Past the end, the first slice is loaded again (in bounds, never used).
       LoadSlice(P, vals, clamp(s + 2 * kWaves), lane, n0);
       LoadSlice(P, vals, clamp(s + 3 * kWaves), lane, n1);
 #pragma unroll
-      for (int k = 0; k < kUnrolledWidth; ++k)
+      for (int k = 0; k < U; ++k)
         sum0 += c0.a[k] * static_cast<T>(g0[k]);
 #pragma unroll
-      for (int k = 0; k < kUnrolledWidth; ++k)
+      for (int k = 0; k < U; ++k)
         sum1 += c1.a[k] * static_cast<T>(g1[k]);
       c0 = n0;
       c1 = n1;
@@ -564,20 +583,28 @@ __device__ __attribute__((always_inline)) double Spmv2(const DevicePlan &P, cons
       LoadSlice(P, vals, clamp(s + 2 * kWaves), lane, c0);
       LoadSlice(P, vals, clamp(s + 3 * kWaves), lane, c1);
     }
-    const int row0 = s * kSliceRows + lane;
-    if (row0 < P.n) {
+    if (row0 >= 0) {
       y[row0] = static_cast<V>(sum0);
       if (w)
         local += static_cast<double>(w[row0]) * static_cast<double>(sum0);
     }
-    const int row1 = s1 * kSliceRows + lane;
-    if (s1 < S.slice1 && row1 < P.n) {
+    if (row1 >= 0) {
       y[row1] = static_cast<V>(sum1);
       if (w)
         local += static_cast<double>(w[row1]) * static_cast<double>(sum1);
     }
   }
   return local;
+}
+
+template <typename T, typename V>
+__device__ __attribute__((always_inline)) double Spmv2Fit(const DevicePlan &P, const Span &S,
+                                                           const T *__restrict__ vals,
+                           const V *__restrict__ v, V *__restrict__ y, const V *__restrict__ w)
+{
+  if (P.max_slice_width <= kNarrowWidth)
+    return Spmv2<T, V, kNarrowWidth>(P, S, vals, v, y, w);
+  return Spmv2<T, V>(P, S, vals, v, y, w);
 }
 
 // kPipeline = false: the plain two-round-trip loop, for rarely used products (FP64
@@ -617,8 +644,8 @@ __device__ __attribute__((always_inline)) double Spmv(const DevicePlan &P, const
           sum += vals[pos] * static_cast<T>(v[P.sell_cols[pos]]);
         }
       }
-      const int row = s * kSliceRows + lane;
-      if (row < P.n) {
+      const int row = SellRow(P, s, lane);
+      if (row >= 0) {
         y[row] = static_cast<V>(sum);
         if (w)
           local += static_cast<double>(w[row]) * static_cast<double>(sum);
@@ -633,6 +660,7 @@ __device__ __attribute__((always_inline)) double Spmv(const DevicePlan &P, const
   for (int s = S.slice0 + wave; s < S.slice1; s += kWaves) {
     T sum = 0;
     const int next = s + kWaves;
+    const int row = SellRow(P, s, lane);
     if (cur.width <= kUnrolledWidth) {
       V g[kUnrolledWidth];
 #pragma unroll
@@ -655,11 +683,65 @@ __device__ __attribute__((always_inline)) double Spmv(const DevicePlan &P, const
       if (next < S.slice1)
         LoadSlice(P, vals, next, lane, cur);
     }
-    const int row = s * kSliceRows + lane;
-    if (row < P.n) {
+    if (row >= 0) {
       y[row] = static_cast<V>(sum);
       if (w)
         local += static_cast<double>(w[row]) * static_cast<double>(sum);
+    }
+  }
+  return local;
+}
+
+// Team form of the product that opens a PCG iteration: the search direction
+// p = z + beta p_old is formed on the fly from the gathered z and p_old (the same two
+// operations as the direction pass, so p and y are bitwise those of the unfused
+// iteration) and stored for the workgroup's own rows, y = A p, and the local sum of
+// p[row] * y[row] is returned.  The direction pass and the team barrier after it drop out
+// of the iteration; p_old and p_new must be different vectors (other workgroups gather
+// p_old while this one writes p_new).
+template <typename T>
+__device__ __attribute__((always_inline)) double SpmvDirection(
+    const DevicePlan &P, const Span &S, const T *__restrict__ vals, const T *__restrict__ z,
+    const T *__restrict__ pold, T beta, T *__restrict__ pnew, T *__restrict__ y)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double local = 0.0;
+  SliceLoads<T> cur;
+  cur.width = 0;
+  if (S.slice0 + wave < S.slice1)
+    LoadSlice(P, vals, S.slice0 + wave, lane, cur);
+  for (int s = S.slice0 + wave; s < S.slice1; s += kWaves) {
+    T sum = 0;
+    const int next = s + kWaves;
+    const int row = SellRow(P, s, lane);
+    if (cur.width <= kUnrolledWidth) {
+      T g[kUnrolledWidth];
+#pragma unroll
+      for (int k = 0; k < kUnrolledWidth; ++k)
+        g[k] = z[cur.c[k]] + beta * pold[cur.c[k]];
+      SliceLoads<T> nxt;
+      // The last slice prefetches itself again (in bounds, never used).
+      LoadSlice(P, vals, next < S.slice1 ? next : s, lane, nxt);
+#pragma unroll
+      for (int k = 0; k < kUnrolledWidth; ++k)
+        sum += cur.a[k] * g[k];
+      cur = nxt;
+    } else {
+      const int base = P.slice_ptr[s] + lane;
+      for (int k = 0; k < cur.width; ++k) {
+        const int pos = base + k * kSliceRows;
+        const int c = P.sell_cols[pos];
+        sum += vals[pos] * (z[c] + beta * pold[c]);
+      }
+      cur.width = 0;
+      if (next < S.slice1)
+        LoadSlice(P, vals, next, lane, cur);
+    }
+    if (row >= 0) {
+      const T pr = z[row] + beta * pold[row];
+      pnew[row] = pr;
+      y[row] = sum;
+      local += static_cast<double>(pr) * static_cast<double>(sum);
     }
   }
   return local;
@@ -948,11 +1030,23 @@ struct LinearInfo {
 // acceptance test ||b - A x|| <= tol ||b|| is unchanged.
 // kBlock: 0 = point Jacobi; 1 = block-Jacobi of B.bs rows (4 in-thread, others by lane
 // exchange); 2 = block-Jacobi of 4 rows only (the lane-exchange code is not compiled).
+// stop_on_stall (a tolerance below what FP64 rounding lets the true residual reach, as a
+// tight acceptance re-solve may ask): a restart whose true residual is not below half the
+// previous one ends the solve with the iteration cap's status instead of restarting until
+// maxit; the reference's solves keep the default, restarting until maxit.
+// A team of workgroups (Team other than BlockTeam) forms each search direction inside
+// the next product (SpmvDirection) instead of a pass of its own: p alternates between
+// p and p_alt, A p goes to q_buf and z = M^-1 r to ap, one team barrier fewer per
+// iteration with the same arithmetic.  One workgroup per item keeps the separate direction pass (its
+// barrier is a block barrier) and needs neither p_alt nor q_buf.
 template <bool kPipeline = true, int kBlock = 0, class Team = BlockTeam>
-__device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const double *inv,
+__device__ __attribute__((always_inline)) LinearInfo Pcg64(const DevicePlan &P, const double *vals, const double *inv,
                             const BlockPrec<double> &B, const double *rhs, double *x, double *r,
-                            double *p, double *ap, double tol, int maxit, bool warm, Team &R)
+                            double *p, double *ap, double tol, int maxit, bool warm, Team &R,
+                            double *p_alt = nullptr, double *q_buf = nullptr,
+                            bool stop_on_stall = false)
 {
+  constexpr bool kFuse = !std::is_same_v<Team, BlockTeam>;
   constexpr bool block = kBlock != 0;
   constexpr bool only4 = kBlock == 2;
   const bool fast4 = only4 || B.bs == 4;
@@ -1071,8 +1165,23 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
     info.residual *= scale;
     return info;
   }
+  double *pc = p;                  // current search direction
+  double *pn = kFuse ? p_alt : p;  // the next one (fused form)
+  double *qv = kFuse ? q_buf : ap; // A p (the unfused form keeps z over it after the update)
+  bool fresh = true;               // pc holds the direction itself (start or restart)
+  double beta = 0.0;
+  double last_true = INFINITY;     // true residual at the last restart
   for (int it = 0; it < maxit; ++it) {
-    double pap[1] = {Spmv<double, double, kPipeline>(P, R.S, vals, p, ap, p)};
+    double pap[1];
+    if (kFuse && !fresh) {
+      pap[0] = SpmvDirection<double>(P, R.S, vals, ap, pc, beta, pn, qv);
+      double *t = pc;
+      pc = pn;
+      pn = t;
+    } else {
+      pap[0] = Spmv<double, double, kPipeline>(P, R.S, vals, pc, qv, pc);
+    }
+    fresh = false;
     R.Sum(pap);
     if (!isfinite(pap[0]) || !isfinite(rho)) {
       info.status = FEMM_NUMERICAL_NONFINITE;
@@ -1085,13 +1194,13 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
     const double alpha = rho / pap[0];
     double s1[2] = {0.0, 0.0};
     if constexpr (block) {
-      // x, r updated; z = M^-1 r written over Ap (dead until the next product).
+      // x, r updated; z = M^-1 r written to ap (over A p in the unfused form).
       if (fast4) {
         const int q1 = (R.S.row1 + 3) / 4;
         for (int q = R.S.row0 / 4 + threadIdx.x; q < q1; q += kBlockThreads) {
           const int row = 4 * q;
-          Rows4<double> xv = LoadRows4(x + row), pv = LoadRows4(p + row),
-                        rv = LoadRows4(r + row), av = LoadRows4(ap + row);
+          Rows4<double> xv = LoadRows4(x + row), pv = LoadRows4(pc + row),
+                        rv = LoadRows4(r + row), av = LoadRows4(qv + row);
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             xv.v[e] = xv.v[e] + alpha * pv.v[e];
@@ -1112,8 +1221,8 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
           const bool valid = i < R.S.row1;
           double ri = 0.0;
           if (valid) {
-            x[i] = x[i] + alpha * p[i];
-            ri = r[i] - alpha * ap[i];
+            x[i] = x[i] + alpha * pc[i];
+            ri = r[i] - alpha * qv[i];
             r[i] = ri;
             s1[0] += ri * ri;
           }
@@ -1135,9 +1244,9 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
           const int c = c0 + u * kBlockThreads;
           if (c < c1) {
             xv[u] = LoadChunk(x + c * V);
-            pv[u] = LoadChunk(p + c * V);
+            pv[u] = LoadChunk(pc + c * V);
             rv[u] = LoadChunk(r + c * V);
-            av[u] = LoadChunk(ap + c * V);
+            av[u] = LoadChunk(qv + c * V);
             iv[u] = LoadChunk(inv + c * V);
           }
         }
@@ -1151,10 +1260,14 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
               const double ri = rv[u].v[e] - alpha * av[u].v[e];
               rv[u].v[e] = ri;
               s1[0] += ri * ri;
-              s1[1] += ri * (ri * iv[u].v[e]);
+              const double zi = ri * iv[u].v[e];
+              s1[1] += ri * zi;
+              iv[u].v[e] = zi;
             }
             StoreChunk(x + c * V, xv[u]);
             StoreChunk(r + c * V, rv[u]);
+            if constexpr (kFuse)
+              StoreChunk(ap + c * V, iv[u]);   // z for the next product
           }
         }
       }
@@ -1182,6 +1295,9 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
         return info;
       }
       info.residual = sqrt(tr[0]);
+      if (stop_on_stall && !(info.residual <= threshold) && !(info.residual < 0.5 * last_true))
+        break;   // the restarts no longer lower the true residual: report the cap's status
+      last_true = info.residual;
       if (info.residual <= threshold) {
         bool finite = true;
         for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
@@ -1203,7 +1319,7 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
 #pragma unroll
           for (int e = 0; e < 4; ++e)
             rs[0] += rv.v[e] * zv.v[e];
-          StoreRows4(p + row, zv);
+          StoreRows4(pc + row, zv);
         }
       } else if constexpr (block && !only4) {
         for (int base = wave_row0; base < R.S.row1; base += kBlockThreads) {
@@ -1212,14 +1328,14 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
           const double ri = valid ? r[i] : 0.0;
           const double zi = BlockApply(B, i, valid, ri);
           if (valid) {
-            p[i] = zi;
+            pc[i] = zi;
             rs[0] += ri * zi;
           }
         }
       } else {
         for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
           const double zi = r[i] * inv[i];
-          p[i] = zi;
+          pc[i] = zi;
           rs[0] += r[i] * zi;
         }
       }
@@ -1229,13 +1345,17 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
         return info;
       }
       rho = rs[0];
+      fresh = true;   // the restart direction is z itself
       continue;
     }
     if (!isfinite(s1[1]) || !(s1[1] > 0.0)) {
       info.status = FEMM_LINEAR_SOLVE_BREAKDOWN;
       return info;
     }
-    const double beta = s1[1] / rho;
+    beta = s1[1] / rho;
+    rho = s1[1];
+    if constexpr (kFuse)
+      continue;   // the next product forms p = z + beta p (z is published by the sum above)
     if constexpr (block) {
       constexpr int V = Chunk<double>::N;
       const int c1 = (R.S.row1 + V - 1) / V;
@@ -1273,7 +1393,6 @@ __device__ LinearInfo Pcg64(const DevicePlan &P, const double *vals, const doubl
         }
       }
     }
-    rho = s1[1];
     R.Sync();
   }
   // Iteration cap: report the true residual (pcg_kernels.cu:302-343).
@@ -1302,7 +1421,7 @@ struct RefinedInfo {
 
 // r = b - A64 x in FP64; returns ||r||_2 and max|r| through the reducer.
 template <class Team>
-__device__ void TrueResidual(const DevicePlan &P, const double *vals, const double *rhs,
+__device__ __attribute__((always_inline)) void TrueResidual(const DevicePlan &P, const double *vals, const double *rhs,
                              const double *x, double *r, double *ap, double *norm, double *amax,
                              Team &R)
 {
@@ -1335,16 +1454,20 @@ __device__ void TrueResidual(const DevicePlan &P, const double *vals, const doub
 // from the current x.  Called with maxit = 0 and fp64_maxit > 0, that FP64 PCG (at most
 // fp64_maxit iterations) is the whole solve.
 template <bool kBlock, class Team>
-__device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, const double *inv,
+__device__ __attribute__((always_inline)) RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, const double *inv,
                                      const float *vals32, const float *inv32,
                                      const BlockPrec<double> &B64, const BlockPrec<float> &B32,
                                      const double *rhs, double *x, double *r, double *p,
                                      double *ap, float *d32, float *r32, float *p32, float *ap32,
                                      double tol, int maxit, bool warm, int max_updates,
-                                     double delta, Team &R, int fp64_maxit = 0)
+                                     double delta, Team &R, int fp64_maxit = 0,
+                                     float *p32_alt = nullptr, float *q32_buf = nullptr,
+                                     double *p_alt = nullptr, double *q_buf = nullptr,
+                                     bool stop_on_stall = false)
 {
   RefinedInfo out{{FEMM_INTERNAL_ERROR, 0, INFINITY}, 0};
   constexpr bool block = kBlock;
+  constexpr bool kFuse = !std::is_same_v<Team, BlockTeam>;   // as in Pcg64
   if (maxit < 0 || !(tol > 0.0) || !(tol < 1.0) || !isfinite(tol)) {
     out.linear.status = FEMM_INVALID_ARGUMENT;
     return out;
@@ -1428,8 +1551,28 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
   double last_true = rnorm;     // FP64 true residual at the last replacement
   bool fallback = false;
   int it = 0;
+  float *pc = p32;                      // current search direction
+  float *pn = kFuse ? p32_alt : p32;    // the next one (fused form)
+  float *qv = kFuse ? q32_buf : ap32;   // A p; z = M^-1 r goes to ap32
+  bool fresh = true;
+  float beta = 0.0f;
   while (it < maxit) {
-    double pap[1] = {Spmv2(P, R.S, vals32, p32, ap32, p32)};
+    // Two slices per step for a workgroup that owns the item (the same sums in the same
+    // order as one slice per step); team members form the direction inside the product.
+    double pap[1];
+    if constexpr (kFuse) {
+      if (fresh) {
+        pap[0] = Spmv<float, float, true>(P, R.S, vals32, pc, qv, pc);
+      } else {
+        pap[0] = SpmvDirection<float>(P, R.S, vals32, ap32, pc, beta, pn, qv);
+        float *t = pc;
+        pc = pn;
+        pn = t;
+      }
+      fresh = false;
+    } else {
+      pap[0] = Spmv2Fit(P, R.S, vals32, pc, qv, pc);
+    }
     R.Sum(pap);
     if (!(pap[0] > 0.0) || !(rho > 0.0) || !isfinite(pap[0]) || !isfinite(rho)) {
       fallback = true;
@@ -1438,12 +1581,12 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
     const float alpha = static_cast<float>(rho / pap[0]);
     double s1[2] = {0.0, 0.0};
     if constexpr (block) {
-      // d, r updated; z = M^-1 r written over Ap (dead until the next product).
+      // d, r updated; z = M^-1 r written to ap32 (over A p in the unfused form).
       const int q1 = (R.S.row1 + 3) / 4;
       for (int q = R.S.row0 / 4 + threadIdx.x; q < q1; q += kBlockThreads) {
         const int row = 4 * q;
-        Rows4<float> dv = LoadRows4(d32 + row), pv = LoadRows4(p32 + row),
-                     rv = LoadRows4(r32 + row), av = LoadRows4(ap32 + row);
+        Rows4<float> dv = LoadRows4(d32 + row), pv = LoadRows4(pc + row),
+                     rv = LoadRows4(r32 + row), av = LoadRows4(qv + row);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           dv.v[e] = dv.v[e] + alpha * pv.v[e];
@@ -1469,9 +1612,9 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
           const int c = c0 + u * kBlockThreads;
           if (c < c1) {
             dv[u] = LoadChunk(d32 + c * V);
-            pv[u] = LoadChunk(p32 + c * V);
+            pv[u] = LoadChunk(pc + c * V);
             rv[u] = LoadChunk(r32 + c * V);
-            av[u] = LoadChunk(ap32 + c * V);
+            av[u] = LoadChunk(qv + c * V);
             iv[u] = LoadChunk(inv32 + c * V);
           }
         }
@@ -1485,10 +1628,14 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
               const float ri = rv[u].v[e] - alpha * av[u].v[e];
               rv[u].v[e] = ri;
               s1[0] += static_cast<double>(ri) * ri;
-              s1[1] += static_cast<double>(ri) * (ri * iv[u].v[e]);
+              const float zi = ri * iv[u].v[e];
+              s1[1] += static_cast<double>(ri) * zi;
+              iv[u].v[e] = zi;
             }
             StoreChunk(d32 + c * V, dv[u]);
             StoreChunk(r32 + c * V, rv[u]);
+            if constexpr (kFuse)
+              StoreChunk(ap32 + c * V, iv[u]);   // z for the next product
           }
         }
       }
@@ -1535,7 +1682,7 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
         for (int c = R.S.row0 / 4 + threadIdx.x; c < q1; c += kBlockThreads) {
           const int row = 4 * c;
           const Rows4<double> r64 = LoadRows4(r + row);
-          Rows4<float> rv, pv = LoadRows4(p32 + row);
+          Rows4<float> rv, pv = LoadRows4(pc + row);
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             rv.v[e] = static_cast<float>(r64.v[e] / sigma);
@@ -1547,16 +1694,19 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
           for (int e = 0; e < 4; ++e)
             s2[1] += static_cast<double>(rv.v[e]) * zv.v[e];
           StoreRows4(r32 + row, rv);
-          StoreRows4(p32 + row, pv);
+          StoreRows4(pc + row, pv);
           StoreRows4(ap32 + row, zv);
         }
       } else {
         for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads) {
           const float ri = static_cast<float>(r[i] / sigma);
           r32[i] = ri;
-          p32[i] = static_cast<float>(q * static_cast<double>(p32[i]));
+          pc[i] = static_cast<float>(q * static_cast<double>(pc[i]));
           s2[0] += static_cast<double>(ri) * ri;
-          s2[1] += static_cast<double>(ri) * (ri * inv32[i]);
+          const float zi = ri * inv32[i];
+          s2[1] += static_cast<double>(ri) * zi;
+          if constexpr (kFuse)
+            ap32[i] = zi;
         }
       }
       R.Sum(s2);
@@ -1568,17 +1718,21 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
       fallback = true;
       break;
     }
-    const float beta = static_cast<float>(rho_next / rho);
+    beta = static_cast<float>(rho_next / rho);
+    if constexpr (kFuse) {
+      rho = rho_next;
+      continue;   // the next product forms p = z + beta p (z is published by the sums above)
+    }
     if constexpr (block) {
       constexpr int V = Chunk<float>::N;
       const int c1 = (R.S.row1 + V - 1) / V;
       for (int c = R.S.row0 / V + threadIdx.x; c < c1; c += kBlockThreads) {
         const Chunk<float> zv = LoadChunk(ap32 + c * V);
-        Chunk<float> pv = LoadChunk(p32 + c * V);
+        Chunk<float> pv = LoadChunk(pc + c * V);
 #pragma unroll
         for (int e = 0; e < V; ++e)
           pv.v[e] = zv.v[e] + beta * pv.v[e];
-        StoreChunk(p32 + c * V, pv);
+        StoreChunk(pc + c * V, pv);
       }
     } else {
       constexpr int V = Chunk<float>::N;
@@ -1591,7 +1745,7 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
           if (c < c1) {
             rv[u] = LoadChunk(r32 + c * V);
             iv[u] = LoadChunk(inv32 + c * V);
-            pv[u] = LoadChunk(p32 + c * V);
+            pv[u] = LoadChunk(pc + c * V);
           }
         }
 #pragma unroll
@@ -1601,7 +1755,7 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
 #pragma unroll
             for (int e = 0; e < V; ++e)
               pv[u].v[e] = rv[u].v[e] * iv[u].v[e] + beta * pv[u].v[e];
-            StoreChunk(p32 + c * V, pv[u]);
+            StoreChunk(pc + c * V, pv[u]);
           }
         }
       }
@@ -1618,7 +1772,7 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
   // final true-residual check).  With block-Jacobi the mixed form supports 4-row blocks.
   const LinearInfo li = Pcg64<false, kBlock ? 2 : 0>(P, vals, inv, B64, rhs, x, r, p, ap, tol,
                                                      fallback ? max(0, maxit - it) : fp64_maxit,
-                                                     true, R);
+                                                     true, R, p_alt, q_buf, stop_on_stall);
   out.linear.iterations += li.iterations;
   out.linear.residual = li.residual;
   out.linear.status = li.status;
@@ -1632,12 +1786,14 @@ __device__ RefinedInfo ReliableSolve(const DevicePlan &P, const double *vals, co
 
 struct ItemPointers {
   double *elem, *vals, *rhs, *inv, *x, *r, *p, *ap, *trial;
+  double *p_alt, *q_buf;   // team solves: second search direction, A p
   float *vals32, *r32, *p32, *ap32, *d32, *inv32;
+  float *p32_alt, *q32_buf;
   double *bjac;      // [kMaxBlock][n] block-Jacobi planes (nullptr: not allocated)
   float *bjac32;
 };
 
-__device__ ItemPointers ItemWork(const DevicePlan &P, const DeviceWork &W, int item, bool mixed)
+__device__ __attribute__((always_inline)) ItemPointers ItemWork(const DevicePlan &P, const DeviceWork &W, int item, bool mixed)
 {
   ItemPointers ip;
   const size_t n = VectorStride(P.n), s = P.sell_size;
@@ -1649,6 +1805,8 @@ __device__ ItemPointers ItemWork(const DevicePlan &P, const DeviceWork &W, int i
   ip.r = W.r + item * n;
   ip.p = W.p + item * n;
   ip.ap = W.ap + item * n;
+  ip.p_alt = W.p_alt + item * n;
+  ip.q_buf = W.q_buf + item * n;
   ip.trial = W.trial + static_cast<size_t>(item) * P.N;
   ip.vals32 = mixed ? W.vals32 + item * s : nullptr;
   ip.r32 = mixed ? W.r32 + item * n : nullptr;
@@ -1656,6 +1814,8 @@ __device__ ItemPointers ItemWork(const DevicePlan &P, const DeviceWork &W, int i
   ip.ap32 = mixed ? W.ap32 + item * n : nullptr;
   ip.d32 = mixed ? W.d32 + item * n : nullptr;
   ip.inv32 = mixed ? W.inv32 + item * n : nullptr;
+  ip.p32_alt = mixed ? W.p32_alt + item * n : nullptr;
+  ip.q32_buf = mixed ? W.q32_buf + item * n : nullptr;
   ip.bjac = W.bjac ? W.bjac + item * kMaxBlock * n : nullptr;
   ip.bjac32 = (mixed && W.bjac32) ? W.bjac32 + item * kMaxBlock * n : nullptr;
   return ip;
@@ -1663,7 +1823,7 @@ __device__ ItemPointers ItemWork(const DevicePlan &P, const DeviceWork &W, int i
 
 // Full device assembly for one item; returns false if any element or diagonal is invalid.
 template <class Team>
-__device__ bool AssembleItem(const DevicePlan &P, const ItemPointers &w, const double *a,
+__device__ __attribute__((always_inline)) bool AssembleItem(const DevicePlan &P, const ItemPointers &w, const double *a,
                              const double *cur, bool newton, Team &R)
 {
   const bool e_ok = ElementPass(P, R.S, a, w.elem, newton);
@@ -1678,7 +1838,7 @@ __device__ bool AssembleItem(const DevicePlan &P, const ItemPointers &w, const d
 // (solvers/nonlinear_solver.cu:549-558).  Requires a fresh assembly at `a`.
 // `xf` receives A at the free roots (n values).
 template <class Team>
-__device__ double EquationNorm(const DevicePlan &P, const ItemPointers &w, const double *a,
+__device__ __attribute__((always_inline)) double EquationNorm(const DevicePlan &P, const ItemPointers &w, const double *a,
                                Team &R, double *xf)
 {
   for (int i = R.S.row0 + threadIdx.x; i < R.S.row1; i += kBlockThreads)
@@ -1696,7 +1856,7 @@ __device__ double EquationNorm(const DevicePlan &P, const ItemPointers &w, const
 }
 
 // Points the item's plan at its material variant; false for an index outside the set.
-__device__ bool ApplyVariant(DevicePlan &P, const DeviceVariants &V, const int32_t *variant_of_item,
+__device__ __attribute__((always_inline)) bool ApplyVariant(DevicePlan &P, const DeviceVariants &V, const int32_t *variant_of_item,
                              int item)
 {
   if (variant_of_item == nullptr)
@@ -1784,7 +1944,7 @@ __device__ double LinearTolerance(const femm_newton_options_t &o, bool adaptive,
 }
 
 template <class Team>
-__device__ void WriteFlux(const DevicePlan &P, const double *a, double *flux, Team &R,
+__device__ __attribute__((always_inline)) void WriteFlux(const DevicePlan &P, const double *a, double *flux, Team &R,
                           bool lead = true)
 {
   for (int c = 0; c < P.C; ++c) {
@@ -1827,11 +1987,13 @@ __device__ bool OptionsValid(const femm_newton_options_t &o)
 // re-solve, whose solution is the returned state.  FP32 iterations with FP64 residual
 // replacement reach the same ||b - A x|| as FP64 PCG but leave more error in the soft
 // (low-reluctivity) modes that this norm hardly sees; the FP64 re-solve from the mixed
-// solution removes it.
+// solution removes it.  That re-solve (`fp64` in every form) also stops once its restarts no
+// longer lower the true residual: a large model's FP64 floor can lie above the tight
+// tolerance, and the policy then falls back to the exact tolerance.
 constexpr int kFormMixed = 1, kFormBlock = 2;
 
 template <int kForm, class Team>
-__device__ LinearInfo LinearSolve(const DevicePlan &P, const ItemPointers &w,
+__device__ __attribute__((always_inline)) LinearInfo LinearSolve(const DevicePlan &P, const ItemPointers &w,
                                   const femm_solve_config_t &cfg, double tol, int maxit,
                                   femm_item_result_t &res, Team &R, bool fp64 = false)
 {
@@ -1852,12 +2014,13 @@ __device__ LinearInfo LinearSolve(const DevicePlan &P, const ItemPointers &w,
                                          w.r, w.p, w.ap, w.d32, w.r32, w.p32, w.ap32, tol,
                                          fp64 ? 0 : maxit, cfg.pcg_warm_start != 0,
                                          cfg.max_refinements, cfg.inner_relative_tolerance, R,
-                                         fp64 ? maxit : 0);
+                                         fp64 ? maxit : 0, w.p32_alt, w.q32_buf, w.p_alt,
+                                         w.q_buf, fp64);
     li = ri.linear;
     res.refinements += ri.refinements;
   } else {
     li = Pcg64<true, kBlock ? 1 : 0>(P, w.vals, w.inv, B64, w.rhs, w.x, w.r, w.p, w.ap, tol, maxit,
-                             cfg.pcg_warm_start != 0, R);
+                             cfg.pcg_warm_start != 0, R, w.p_alt, w.q_buf, fp64);
   }
   R.Sync();
   res.linear_iterations += li.iterations;
@@ -1907,7 +2070,7 @@ __device__ __attribute__((always_inline)) double StepOf(const DevicePlan &P,
 // start (nonlinear_solver.cu:565-576), independent of any warm start.  The two policies
 // share one body, the assembly and the linear solve.
 template <int kForm, class Team>
-__device__ femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers &w,
+__device__ __attribute__((always_inline)) femm_item_result_t ItemSolve(const DevicePlan &P, const ItemPointers &w,
                                         const double *cur, double *a, bool inputs_ok,
                                         const femm_newton_options_t &o,
                                         const femm_solve_config_t &cfg, const double *warm,
@@ -2312,12 +2475,8 @@ __global__ __launch_bounds__(kBlockThreads) void GroupSolveKernel(const GroupEnt
     return;
   const DevicePlan P = E.plan;
   const DeviceWork W = E.work;
-  if (G == 1) {
-    BlockTeam R{BlockReducer{lds, 0}, WholeItem(P)};
-    SolveItem<kForm, kMode>(P, W, item, true, true, E.currents, args, nullptr, E.a, E.flux,
-                             E.results, earlier, R);
-    return;
-  }
+  // One team type for every entry (a one-workgroup team reduces and synchronises as a
+  // block): one inlined copy of the solve keeps the kernel's registers in bounds.
   GridTeam R{BlockReducer{lds, 0},
              Partition(P, G, g),
              W.team_partials + static_cast<size_t>(item) * 2 * kMaxPartitions * 4,
