@@ -1,11 +1,14 @@
 // BF16 MFMA kernels of the warm-start MLP (forward, backward, Adam) and the
 // femm_mlp_* C entry points used to train and check the network on its own.
 //
-// GEMM tiling: one wave64 owns one 16x16 output tile and walks K in steps of 16 with
-// v_mfma_f32_16x16x16_bf16: lane l supplies A[l%16][4(l/16)+i] and B[4(l/16)+i][l%16]
-// (i = 0..3) and receives D[4(l/16)+i][l%16].  Four waves per workgroup cover a 32x32
-// block; every output element is accumulated by one lane in a fixed K order, so
-// results are bitwise repeatable.
+// GEMM tiling: one wave64 owns one 16x16 output tile and walks K in steps of 32 with
+// gfx950's v_mfma_f32_16x16x32_bf16: lane l supplies A[l%16][8(l/16)+i] and
+// B[8(l/16)+i][l%16] (i = 0..7) and receives D[4(l/16)+i][l%16] (i = 0..3).  An operand
+// stored K-contiguous (A as M x K, or B given as its transpose, N x K) arrives as one
+// 16-byte load per lane; the others as eight 2-byte loads.  A remaining 16-wide K step
+// (K is padded to 16) uses v_mfma_f32_16x16x16_bf16 with four values per lane.  Four
+// waves per workgroup cover a 32x32 block; every output element is accumulated by one
+// lane in a fixed K order, so results are bitwise repeatable.
 
 #include "api_guard.hpp"
 #include "mlp.hpp"
@@ -23,6 +26,8 @@ namespace
 {
 
 typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef short short8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kTile = 16;
@@ -48,7 +53,28 @@ __global__ __launch_bounds__(kThreads) void GemmBf16(const bf16 *__restrict__ A,
     return;
   const int row = lane & 15, quad = lane >> 4;
   float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-  for (int k0 = 0; k0 < K; k0 += kTile) {
+  int k0 = 0;
+  for (; k0 + 2 * kTile <= K; k0 += 2 * kTile) {
+    const int kq = k0 + 8 * quad;
+    short8_t a, b;
+    if constexpr (kTransA) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        a[i] = static_cast<short>(A[static_cast<size_t>(kq + i) * M + tm + row]);
+    } else {
+      a = *reinterpret_cast<const short8_t *>(A + static_cast<size_t>(tm + row) * K + kq);
+    }
+    if constexpr (kTransB) {
+      b = *reinterpret_cast<const short8_t *>(B + static_cast<size_t>(tn + row) * K + kq);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        b[i] = static_cast<short>(B[static_cast<size_t>(kq + i) * N + tn + row]);
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a),
+                                                  __builtin_bit_cast(bf16x8_t, b), acc, 0, 0, 0);
+  }
+  if (k0 < K) {   // K % 32 == 16
     short4_t a, b;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
