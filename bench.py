@@ -46,7 +46,6 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "femm-gpu_amd"))
 
 HBM_PEAK_GBS = 8000.0
-RESULT_BYTES = 48
 TRAIN_SEED = 777
 WARM_BUDGET = 16
 
@@ -68,27 +67,36 @@ def algorithmic_bytes(plan, results, B, mixed, block_size=1):
     """Algorithmic HBM bytes of one batched solve from the per-item kernel counters.
 
     PCG iteration (per item): s_v nnz + (4 nnz + 4 (n + 1)) / S + 13 s_x n with S = B (the
-    CSR pattern is shared by every item); s_v = s_x = 8 (FP64) or 4 (FP32 iterations).
+    CSR pattern is shared by every item); s_v = s_x = 4 for the FP32 iterations of the
+    mixed form and 8 for FP64 ones (every iteration of the FP64 form; the FP64 finishes and
+    acceptance re-solves of the mixed form, counted by the kernel as fp64_iterations).
     With block-Jacobi of b rows the vector passes are 12 + b (the b planes replace D^-1
     and z is stored), and each Newton iteration also rebuilds the planes (8 nnz + 8 n read,
     (8 + s_x) b n written).
     FP64 residual replacement (mixed form): 8 nnz + (4 nnz + 4 (n + 1)) / S + 76 n.
     Newton iteration (per item): element pass (read A: 8 N, write 12 doubles per
     triangle), slot and row gathers (read 12 doubles per triangle, write 8 nnz + 16 n),
-    step and update passes (24 N), equation-residual SpMV (8 nnz + 32 n)."""
+    step and update passes (24 N), equation-residual SpMV (8 nnz + 32 n); a trial
+    assembly of the safeguarded policy costs the same less the step pass (8 N read,
+    16 N for the trial state)."""
     N, T, n, nnz = plan.N, plan.T, plan.n, plan.nnz
-    s = 4 if mixed else 8
     shared = (4 * nnz + 4 * (n + 1)) / B
     passes = 13 if block_size <= 1 else 12 + block_size
-    pcg_iter = s * nnz + shared + passes * s * n
+    pcg64 = 8 * nnz + shared + passes * 8 * n
+    pcg32 = 4 * nnz + shared + passes * 4 * n
+    pcg_iter = pcg32 if mixed else pcg64
     replacement = 8 * nnz + shared + 76 * n
     assembly = 8 * N + 96 * T + 96 * T + 8 * nnz + 16 * n
     newton = assembly + 24 * N + 8 * nnz + 32 * n
+    trial = assembly + 16 * N + 8 * nnz + 32 * n
     if block_size > 1:
         newton += 8 * nnz + 8 * n + (8 + (4 if mixed else 0)) * block_size * n
     total = 0.0
     for r in results:
-        total += (r["linear_iterations"] * pcg_iter + r["newton_iterations"] * newton
+        it64 = int(r["fp64_iterations"]) if mixed else int(r["linear_iterations"])
+        it32 = int(r["linear_iterations"]) - it64 if mixed else 0
+        total += (it32 * pcg32 + it64 * pcg64 + r["newton_iterations"] * newton
+                  + r["trial_assemblies"] * trial
                   + (r["refinements"] * replacement if mixed else 0.0) + 2 * assembly)
     return total, pcg_iter
 
@@ -256,7 +264,7 @@ def main():
     stream = timer.stream.cuda_stream
     a = torch.zeros((B, plan.N), dtype=torch.float64, device="cuda")
     flux = torch.zeros((B, plan.C), dtype=torch.float64, device="cuda")
-    res = torch.zeros(B * RESULT_BYTES, dtype=torch.uint8, device="cuda")
+    res = torch.zeros(B * RESULT_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
     loads = torch.zeros((B, 3), dtype=torch.float64, device="cuda")
     radial = torch.zeros((B, max(1, args.probes)), dtype=torch.float64, device="cuda")
 
@@ -277,6 +285,8 @@ def main():
                     solves_per_s=float(passing.sum()) / (ms * 1e-3) if ms > 0 else None,
                     newton_mean=float(r["newton_iterations"].mean()),
                     pcg_mean=float(r["linear_iterations"].mean()),
+                    pcg_fp64_mean=float(r["fp64_iterations"].mean()),
+                    trial_mean=float(r["trial_assemblies"].mean()),
                     pcg_max=int(r["linear_iterations"].max()),
                     max_equation_residual=float(r["equation_residual"].max()),
                     achieved_gbs=bytes_total / (solve_ms * 1e-3) / 1e9 if solve_ms > 0 else None,
@@ -347,7 +357,7 @@ def main():
         train_cur = torch.from_numpy(meshes.motor_currents(S, seed=TRAIN_SEED)).to("cuda")
         ta = torch.zeros((S, plan.N), dtype=torch.float64, device="cuda")
         tf = torch.zeros((S, plan.C), dtype=torch.float64, device="cuda")
-        tr = torch.zeros(S * RESULT_BYTES, dtype=torch.uint8, device="cuda")
+        tr = torch.zeros(S * RESULT_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
         plan.solve_batch_device(train_cur.data_ptr(), S, ta.data_ptr(), tf.data_ptr(),
                                 tr.data_ptr(), stream, options,
                                 mixed_config(block_size=args.block_size))
@@ -448,7 +458,9 @@ def main():
                 "solve_ms": f["solve_ms"], "postprocess_ms": f["postprocess_ms"],
                 "steps": f["steps"], "warmup": f["warmup"], "items_passing": f["passing"],
                 "newton_mean": f["newton_mean"], "pcg_iterations_mean": f["pcg_mean"],
-                "pcg_iterations_max": f["pcg_max"], "achieved_GBps": f["achieved_gbs"]}
+                "pcg_iterations_max": f["pcg_max"],
+                "pcg_fp64_iterations_mean": f["pcg_fp64_mean"],
+                "trial_assemblies_mean": f["trial_mean"], "achieved_GBps": f["achieved_gbs"]}
 
     line = {
         "metric": "nonlinear solves/sec @1e-8 resid, 60k nodes batch 256: plain FP64 vs warm+mixed",

@@ -1417,6 +1417,7 @@ __device__ __attribute__((always_inline)) LinearInfo Pcg64(const DevicePlan &P, 
 struct RefinedInfo {
   LinearInfo linear;
   int refinements;   // FP64 true-residual evaluations
+  int fp64_iterations;   // of linear.iterations, those of the FP64 finish
 };
 
 // r = b - A64 x in FP64; returns ||r||_2 and max|r| through the reducer.
@@ -1465,7 +1466,7 @@ __device__ __attribute__((always_inline)) RefinedInfo ReliableSolve(const Device
                                      double *p_alt = nullptr, double *q_buf = nullptr,
                                      bool stop_on_stall = false)
 {
-  RefinedInfo out{{FEMM_INTERNAL_ERROR, 0, INFINITY}, 0};
+  RefinedInfo out{{FEMM_INTERNAL_ERROR, 0, INFINITY}, 0, 0};
   constexpr bool block = kBlock;
   constexpr bool kFuse = !std::is_same_v<Team, BlockTeam>;   // as in Pcg64
   if (maxit < 0 || !(tol > 0.0) || !(tol < 1.0) || !isfinite(tol)) {
@@ -1774,6 +1775,7 @@ __device__ __attribute__((always_inline)) RefinedInfo ReliableSolve(const Device
                                                      fallback ? max(0, maxit - it) : fp64_maxit,
                                                      true, R, p_alt, q_buf, stop_on_stall);
   out.linear.iterations += li.iterations;
+  out.fp64_iterations = li.iterations;
   out.linear.residual = li.residual;
   out.linear.status = li.status;
   if (li.status == FEMM_OK || fallback)
@@ -2018,9 +2020,11 @@ __device__ __attribute__((always_inline)) LinearInfo LinearSolve(const DevicePla
                                          w.q_buf, fp64);
     li = ri.linear;
     res.refinements += ri.refinements;
+    res.fp64_iterations += ri.fp64_iterations;
   } else {
     li = Pcg64<true, kBlock ? 1 : 0>(P, w.vals, w.inv, B64, w.rhs, w.x, w.r, w.p, w.ap, tol, maxit,
                              cfg.pcg_warm_start != 0, R, w.p_alt, w.q_buf, fp64);
+    res.fp64_iterations += li.iterations;
   }
   R.Sync();
   res.linear_iterations += li.iterations;
@@ -2208,6 +2212,7 @@ __device__ __attribute__((always_inline)) femm_item_result_t ItemSolve(const Dev
           w.trial[v] = a[v] + alpha * (cand - a[v]);
         }
         R.Sync();
+        ++res.trial_assemblies;
         if (!AssembleItem(P, w, w.trial, cur, true, R)) {
           if (!gates)
             status = FEMM_ASSEMBLY_FAILED;
@@ -2309,6 +2314,8 @@ __device__ __attribute__((always_inline)) void SolveItem(
     res.newton_iterations += earlier.newton_iterations;
     res.linear_iterations += earlier.linear_iterations;
     res.refinements += earlier.refinements;
+    res.fp64_iterations += earlier.fp64_iterations;
+    res.trial_assemblies += earlier.trial_assemblies;
   }
   if (lead && threadIdx.x == 0)
     results[item] = res;

@@ -47,6 +47,7 @@ __global__ __launch_bounds__(kBlockThreads) void TangentKernel(DevicePlan P, Dev
     r.status = li.status;
     r.newton_iterations = 0;
     r.linear_iterations = li.iterations;
+    r.fp64_iterations = li.iterations;
     r.refinements = 0;
     r.relative_step = 0.0;
     r.equation_residual = 0.0;

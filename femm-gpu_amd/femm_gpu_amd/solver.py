@@ -42,7 +42,8 @@ class ItemResult(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int32), ("newton_iterations", ctypes.c_int32),
                 ("linear_iterations", ctypes.c_int32), ("refinements", ctypes.c_int32),
                 ("relative_step", ctypes.c_double), ("equation_residual", ctypes.c_double),
-                ("last_linear_tolerance", ctypes.c_double), ("linear_residual", ctypes.c_double)]
+                ("last_linear_tolerance", ctypes.c_double), ("linear_residual", ctypes.c_double),
+                ("fp64_iterations", ctypes.c_int32), ("trial_assemblies", ctypes.c_int32)]
 
 
 class PostprocessOptions(ctypes.Structure):
@@ -56,7 +57,8 @@ class PostprocessOptions(ctypes.Structure):
 RESULT_DTYPE = np.dtype([("status", np.int32), ("newton_iterations", np.int32),
                          ("linear_iterations", np.int32), ("refinements", np.int32),
                          ("relative_step", np.float64), ("equation_residual", np.float64),
-                         ("last_linear_tolerance", np.float64), ("linear_residual", np.float64)])
+                         ("last_linear_tolerance", np.float64), ("linear_residual", np.float64),
+                         ("fp64_iterations", np.int32), ("trial_assemblies", np.int32)])
 
 
 def reference_config(**kw) -> SolveConfig:

@@ -85,6 +85,11 @@ typedef struct femm_item_result {
   double equation_residual;    /* relative equation residual at the returned state */
   double last_linear_tolerance;
   double linear_residual;      /* last verified ||b - A x|| of the final correction */
+  int32_t fp64_iterations;     /* of linear_iterations, those run in FP64: all of them in the
+                                  FP64 forms; the FP64 finishes and acceptance re-solves in
+                                  the mixed form */
+  int32_t trial_assemblies;    /* Newton assemblies at trial states (the safeguarded policy's
+                                  Armijo / acceptance candidates) */
 } femm_item_result_t;
 
 /* Plan: validation, signed DOF map, symbolic CSR, ordered contributor lists and
