@@ -52,7 +52,7 @@ __global__ __launch_bounds__(kBlockThreads) void SpmvKernel(DevicePlan P, Device
       Spmv2Fit<T, T>(P, WholeItem(P), vals, x + item * n, y + item * n,
                   static_cast<const T *>(nullptr));
     else
-      Spmv<T, T, true>(P, WholeItem(P), vals, x + item * n, y + item * n,
+      SpmvFit<T, T>(P, WholeItem(P), vals, x + item * n, y + item * n,
                        static_cast<const T *>(nullptr));
     __syncthreads();
   }

@@ -136,17 +136,32 @@ __global__ void MseGradient(const float *__restrict__ Y, const float *__restrict
     loss_rows[r] = loss;
 }
 
-// db[c] = sum_r dZ[r][c], rows summed in order.
-__global__ void ColumnSum(const float *__restrict__ dZ, float *__restrict__ db, int rows, int cols)
+// db[c] = sum_r dZ[r][c]: a workgroup takes 64 columns; its kSumGroups row groups sum the
+// rows r = g, g + kSumGroups, ... in order (coalesced 64-column rows), and the group sums
+// are added in group order, so the result is repeatable.
+constexpr int kSumGroups = kThreads / 64;
+
+__global__ __launch_bounds__(kThreads) void ColumnSum(const float *__restrict__ dZ,
+                                                      float *__restrict__ db, int rows, int cols)
 {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols)
-    return;
+  __shared__ float part[kSumGroups][64];
+  const int g = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int c = blockIdx.x * 64 + lane;
   float s = 0.0f;
-  for (int r = 0; r < rows; ++r)
-    s += dZ[static_cast<size_t>(r) * cols + c];
-  db[c] = s;
+  if (c < cols)
+    for (int r = g; r < rows; r += kSumGroups)
+      s += dZ[static_cast<size_t>(r) * cols + c];
+  part[g][lane] = s;
+  __syncthreads();
+  if (g == 0 && c < cols) {
+    float t = part[0][lane];
+    for (int h = 1; h < kSumGroups; ++h)
+      t += part[h][lane];
+    db[c] = t;
+  }
 }
+
+unsigned ColumnBlocks(int cols) { return static_cast<unsigned>((cols + 63) / 64); }
 
 __global__ void AdamKernel(float *__restrict__ w, bf16 *__restrict__ w16,
                            const float *__restrict__ g, float *__restrict__ m,
@@ -365,7 +380,7 @@ void Mlp::Backward(int rows, hipStream_t s)
   // Output layer: dW3 = H2^T dY, db3 = colsum dY, dH2 = dY W3^T.
   Gemm<true, false>(act.h2.get(), act.dy16.get(), nullptr, g.get() + L.w3, nullptr, L.hid,
                     L.out_p, R, kLinear, s);
-  hipLaunchKernelGGL(ColumnSum, dim3(Blocks(L.out_p)), dim3(kThreads), 0, s, act.dy.get(),
+  hipLaunchKernelGGL(ColumnSum, dim3(ColumnBlocks(L.out_p)), dim3(kThreads), 0, s, act.dy.get(),
                      g.get() + L.b3, R, L.out_p);
   Gemm<false, true>(act.dy16.get(), w16.get() + L.w3, nullptr, act.dh2.get(), nullptr, R, L.hid,
                     L.out_p, kLinear, s);
@@ -374,7 +389,7 @@ void Mlp::Backward(int rows, hipStream_t s)
   // Hidden layer 2.
   Gemm<true, false>(act.h1.get(), act.dz2_16.get(), nullptr, g.get() + L.w2, nullptr, L.hid, L.hid,
                     R, kLinear, s);
-  hipLaunchKernelGGL(ColumnSum, dim3(Blocks(L.hid)), dim3(kThreads), 0, s, act.dz2.get(),
+  hipLaunchKernelGGL(ColumnSum, dim3(ColumnBlocks(L.hid)), dim3(kThreads), 0, s, act.dz2.get(),
                      g.get() + L.b2, R, L.hid);
   Gemm<false, true>(act.dz2_16.get(), w16.get() + L.w2, nullptr, act.dh1.get(), nullptr, R, L.hid,
                     L.hid, kLinear, s);
@@ -383,7 +398,7 @@ void Mlp::Backward(int rows, hipStream_t s)
   // Hidden layer 1.
   Gemm<true, false>(act.x16.get(), act.dz1_16.get(), nullptr, g.get() + L.w1, nullptr, L.in_p,
                     L.hid, R, kLinear, s);
-  hipLaunchKernelGGL(ColumnSum, dim3(Blocks(L.hid)), dim3(kThreads), 0, s, act.dz1.get(),
+  hipLaunchKernelGGL(ColumnSum, dim3(ColumnBlocks(L.hid)), dim3(kThreads), 0, s, act.dz1.get(),
                      g.get() + L.b1, R, L.hid);
 }
 

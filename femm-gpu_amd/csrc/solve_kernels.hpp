@@ -484,7 +484,7 @@ __device__ __attribute__((always_inline)) bool GatherRhs(const DevicePlan &P, co
 // flight; every slice costs one memory round trip instead of two.  The sum order
 // over a row's entries is the CSR order either way.
 constexpr int kUnrolledWidth = 10;
-constexpr int kNarrowWidth = 9;   // the FP32 product's unroll when no slice is wider
+constexpr int kNarrowWidth = 9;   // the pipelined products' unroll when no slice is wider
 
 template <typename T, int U = kUnrolledWidth>
 struct SliceLoads {
@@ -536,8 +536,8 @@ __device__ inline T WideSlice(const DevicePlan &P, const T *__restrict__ vals,
 // Two slices per wave per step (slices s and s + kWaves): twice the gathers in flight per
 // round trip for the FP32 iterations, whose loads take half the registers of FP64 ones.
 // U entries per slice are loaded whatever its width (the last column repeated): a plan
-// whose slices are all at most kNarrowWidth wide takes the narrower unroll (Spmv2Fit),
-// one load of each kind fewer per slice.
+// whose slices are all at most kNarrowWidth wide takes the narrower unroll (Spmv2Fit,
+// SpmvFit), one load of each kind fewer per slice.
 template <typename T, typename V, int U = kUnrolledWidth>
 __device__ __attribute__((always_inline)) double Spmv2(const DevicePlan &P, const Span &S,
                                                         const T *__restrict__ vals,
@@ -609,7 +609,8 @@ __device__ __attribute__((always_inline)) double Spmv2Fit(const DevicePlan &P, c
 
 // kPipeline = false: the plain two-round-trip loop, for rarely used products (FP64
 // residuals inside the mixed form, equation residuals) where registers matter more.
-template <typename T, typename V, bool kPipeline = true>
+// U: entries loaded per slice by the pipelined loop (SpmvFit picks it as Spmv2Fit does).
+template <typename T, typename V, bool kPipeline = true, int U = kUnrolledWidth>
 __device__ __attribute__((always_inline)) double Spmv(const DevicePlan &P, const Span &S,
                                                        const T *__restrict__ vals,
                        const V *__restrict__ v, V *__restrict__ y, const V *__restrict__ w)
@@ -653,7 +654,7 @@ __device__ __attribute__((always_inline)) double Spmv(const DevicePlan &P, const
     }
     return local;
   }
-  SliceLoads<T> cur;
+  SliceLoads<T, U> cur;
   cur.width = 0;
   if (S.slice0 + wave < S.slice1)
     LoadSlice(P, vals, S.slice0 + wave, lane, cur);
@@ -661,16 +662,16 @@ __device__ __attribute__((always_inline)) double Spmv(const DevicePlan &P, const
     T sum = 0;
     const int next = s + kWaves;
     const int row = SellRow(P, s, lane);
-    if (cur.width <= kUnrolledWidth) {
-      V g[kUnrolledWidth];
+    if (cur.width <= U) {
+      V g[U];
 #pragma unroll
-      for (int k = 0; k < kUnrolledWidth; ++k)
+      for (int k = 0; k < U; ++k)
         g[k] = v[cur.c[k]];
-      SliceLoads<T> nxt;
+      SliceLoads<T, U> nxt;
       // The last slice prefetches itself again (in bounds, never used).
       LoadSlice(P, vals, next < S.slice1 ? next : s, lane, nxt);
 #pragma unroll
-      for (int k = 0; k < kUnrolledWidth; ++k)
+      for (int k = 0; k < U; ++k)
         sum += cur.a[k] * static_cast<T>(g[k]);
       cur = nxt;
     } else {
@@ -690,6 +691,16 @@ __device__ __attribute__((always_inline)) double Spmv(const DevicePlan &P, const
     }
   }
   return local;
+}
+
+template <typename T, typename V>
+__device__ __attribute__((always_inline)) double SpmvFit(const DevicePlan &P, const Span &S,
+                                                          const T *__restrict__ vals,
+                          const V *__restrict__ v, V *__restrict__ y, const V *__restrict__ w)
+{
+  if (P.max_slice_width <= kNarrowWidth)
+    return Spmv<T, V, true, kNarrowWidth>(P, S, vals, v, y, w);
+  return Spmv<T, V, true>(P, S, vals, v, y, w);
 }
 
 // Team form of the product that opens a PCG iteration: the search direction
@@ -1179,7 +1190,10 @@ __device__ __attribute__((always_inline)) LinearInfo Pcg64(const DevicePlan &P, 
       pc = pn;
       pn = t;
     } else {
-      pap[0] = Spmv<double, double, kPipeline>(P, R.S, vals, pc, qv, pc);
+      if constexpr (kPipeline)
+        pap[0] = SpmvFit<double, double>(P, R.S, vals, pc, qv, pc);
+      else
+        pap[0] = Spmv<double, double, false>(P, R.S, vals, pc, qv, pc);
     }
     fresh = false;
     R.Sum(pap);
@@ -1771,7 +1785,7 @@ __device__ __attribute__((always_inline)) RefinedInfo ReliableSolve(const Device
   out.linear.iterations = it;
   // FP64 finish from the current x (a breakdown or stall of the FP32 iteration, or only the
   // final true-residual check).  With block-Jacobi the mixed form supports 4-row blocks.
-  const LinearInfo li = Pcg64<false, kBlock ? 2 : 0>(P, vals, inv, B64, rhs, x, r, p, ap, tol,
+  const LinearInfo li = Pcg64<true, kBlock ? 2 : 0>(P, vals, inv, B64, rhs, x, r, p, ap, tol,
                                                      fallback ? max(0, maxit - it) : fp64_maxit,
                                                      true, R, p_alt, q_buf, stop_on_stall);
   out.linear.iterations += li.iterations;
