@@ -280,6 +280,10 @@ struct GridTeam {
 
 __device__ inline int UniformInt(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Workgroups in a team (a BlockTeam is one workgroup).
+__device__ inline int TeamSize(const BlockTeam &) { return 1; }
+__device__ inline int TeamSize(const GridTeam &R) { return R.G; }
+
 // Workgroup g's share of an item split over G workgroups (wave-uniform values).
 __device__ inline Span Partition(const DevicePlan &P, int G, int g)
 {
@@ -1177,14 +1181,15 @@ __device__ __attribute__((always_inline)) LinearInfo Pcg64(const DevicePlan &P, 
     return info;
   }
   double *pc = p;                  // current search direction
-  double *pn = kFuse ? p_alt : p;  // the next one (fused form)
-  double *qv = kFuse ? q_buf : ap; // A p (the unfused form keeps z over it after the update)
+  const bool fuse = kFuse && TeamSize(R) > 1;   // a one-workgroup team keeps the pass
+  double *pn = fuse ? p_alt : p;   // the next one (fused form)
+  double *qv = fuse ? q_buf : ap;  // A p (the unfused form keeps z over it after the update)
   bool fresh = true;               // pc holds the direction itself (start or restart)
   double beta = 0.0;
   double last_true = INFINITY;     // true residual at the last restart
   for (int it = 0; it < maxit; ++it) {
     double pap[1];
-    if (kFuse && !fresh) {
+    if (fuse && !fresh) {
       pap[0] = SpmvDirection<double>(P, R.S, vals, ap, pc, beta, pn, qv);
       double *t = pc;
       pc = pn;
@@ -1280,7 +1285,7 @@ __device__ __attribute__((always_inline)) LinearInfo Pcg64(const DevicePlan &P, 
             }
             StoreChunk(x + c * V, xv[u]);
             StoreChunk(r + c * V, rv[u]);
-            if constexpr (kFuse)
+            if (fuse)
               StoreChunk(ap + c * V, iv[u]);   // z for the next product
           }
         }
@@ -1368,7 +1373,7 @@ __device__ __attribute__((always_inline)) LinearInfo Pcg64(const DevicePlan &P, 
     }
     beta = s1[1] / rho;
     rho = s1[1];
-    if constexpr (kFuse)
+    if (fuse)
       continue;   // the next product forms p = z + beta p (z is published by the sum above)
     if constexpr (block) {
       constexpr int V = Chunk<double>::N;
@@ -1567,8 +1572,9 @@ __device__ __attribute__((always_inline)) RefinedInfo ReliableSolve(const Device
   bool fallback = false;
   int it = 0;
   float *pc = p32;                      // current search direction
-  float *pn = kFuse ? p32_alt : p32;    // the next one (fused form)
-  float *qv = kFuse ? q32_buf : ap32;   // A p; z = M^-1 r goes to ap32
+  constexpr bool fuse = kFuse;          // (a runtime choice here costs registers: spills)
+  float *pn = fuse ? p32_alt : p32;     // the next one (fused form)
+  float *qv = fuse ? q32_buf : ap32;    // A p; z = M^-1 r goes to ap32
   bool fresh = true;
   float beta = 0.0f;
   while (it < maxit) {
@@ -1649,7 +1655,7 @@ __device__ __attribute__((always_inline)) RefinedInfo ReliableSolve(const Device
             }
             StoreChunk(d32 + c * V, dv[u]);
             StoreChunk(r32 + c * V, rv[u]);
-            if constexpr (kFuse)
+            if constexpr (fuse)
               StoreChunk(ap32 + c * V, iv[u]);   // z for the next product
           }
         }
@@ -1720,7 +1726,7 @@ __device__ __attribute__((always_inline)) RefinedInfo ReliableSolve(const Device
           s2[0] += static_cast<double>(ri) * ri;
           const float zi = ri * inv32[i];
           s2[1] += static_cast<double>(ri) * zi;
-          if constexpr (kFuse)
+          if constexpr (fuse)
             ap32[i] = zi;
         }
       }
@@ -1734,7 +1740,7 @@ __device__ __attribute__((always_inline)) RefinedInfo ReliableSolve(const Device
       break;
     }
     beta = static_cast<float>(rho_next / rho);
-    if constexpr (kFuse) {
+    if constexpr (fuse) {
       rho = rho_next;
       continue;   // the next product forms p = z + beta p (z is published by the sums above)
     }
@@ -2003,15 +2009,16 @@ __device__ bool OptionsValid(const femm_newton_options_t &o)
 // re-solve, whose solution is the returned state.  FP32 iterations with FP64 residual
 // replacement reach the same ||b - A x|| as FP64 PCG but leave more error in the soft
 // (low-reluctivity) modes that this norm hardly sees; the FP64 re-solve from the mixed
-// solution removes it.  That re-solve (`fp64` in every form) also stops once its restarts no
-// longer lower the true residual: a large model's FP64 floor can lie above the tight
-// tolerance, and the policy then falls back to the exact tolerance.
+// solution removes it.  With `stop_on_stall` (the tight acceptance re-solve) the solve also
+// stops once its restarts no longer lower the true residual: a large model's FP64 floor can
+// lie above the tight tolerance, and the policy then falls back to the exact tolerance.
 constexpr int kFormMixed = 1, kFormBlock = 2;
 
 template <int kForm, class Team>
 __device__ __attribute__((always_inline)) LinearInfo LinearSolve(const DevicePlan &P, const ItemPointers &w,
                                   const femm_solve_config_t &cfg, double tol, int maxit,
-                                  femm_item_result_t &res, Team &R, bool fp64 = false)
+                                  femm_item_result_t &res, Team &R, bool fp64 = false,
+                                  bool stop_on_stall = false)
 {
   constexpr bool kMixed = (kForm & kFormMixed) != 0, kBlock = (kForm & kFormBlock) != 0;
   LinearInfo li;
@@ -2031,13 +2038,13 @@ __device__ __attribute__((always_inline)) LinearInfo LinearSolve(const DevicePla
                                          fp64 ? 0 : maxit, cfg.pcg_warm_start != 0,
                                          cfg.max_refinements, cfg.inner_relative_tolerance, R,
                                          fp64 ? maxit : 0, w.p32_alt, w.q32_buf, w.p_alt,
-                                         w.q_buf, fp64);
+                                         w.q_buf, stop_on_stall);
     li = ri.linear;
     res.refinements += ri.refinements;
     res.fp64_iterations += ri.fp64_iterations;
   } else {
     li = Pcg64<true, kBlock ? 1 : 0>(P, w.vals, w.inv, B64, w.rhs, w.x, w.r, w.p, w.ap, tol, maxit,
-                             cfg.pcg_warm_start != 0, R, w.p_alt, w.q_buf, fp64);
+                             cfg.pcg_warm_start != 0, R, w.p_alt, w.q_buf, stop_on_stall);
     res.fp64_iterations += li.iterations;
   }
   R.Sync();
@@ -2180,8 +2187,10 @@ __device__ __attribute__((always_inline)) femm_item_result_t ItemSolve(const Dev
         w.x[i] = a[P.free_nodes[i]];
       R.Sync();
     }
-    const LinearInfo li =
-        LinearSolve<kForm>(P, w, cfg, tol, o.max_linear_iterations, res, R, resolve);
+    // The tight acceptance re-solve gives up on a stall (then the exact tolerance is solved
+    // as the reference would, restarting to the cap).
+    const LinearInfo li = LinearSolve<kForm>(P, w, cfg, tol, o.max_linear_iterations, res, R,
+                                             resolve, resolve && TightAcceptance(o, cfg, relaxed));
     if (li.status != FEMM_OK) {
       if (resolve && li.status == FEMM_LINEAR_SOLVE_NOT_CONVERGED &&
           TightAcceptance(o, cfg, relaxed)) {
