@@ -843,12 +843,14 @@ int femm_plan_set_postprocess(femm_plan *plan, const femm_postprocess_options_t 
   plan->post_ready = false;
   // A model with air-gap elements gets the element's own postprocess, as the reference's
   // adapters choose for sliding-band artifacts (app/sample_adapter.cpp:59-61, 434-436).
-  st = plan->host.ages.empty()
-           ? femm::BuildPostPlan(&plan->model.view, plan->host, *options, &plan->post)
-           : femm::BuildAgePostPlan(plan->host, plan->model.view.depth_m, *options, &plan->post);
-  if (st != FEMM_OK)
-    return st;
-  st = femm::UploadPostPlan(plan->post, plan->dev, &plan->post_dev, &plan->post_base);
+  if (plan->host.ages.empty()) {
+    st = femm::BuildPostPlanDevice(&plan->model.view, plan->host, plan->dev, *options,
+                                   &plan->post, &plan->post_dev, &plan->post_base);
+  } else {
+    st = femm::BuildAgePostPlan(plan->host, plan->model.view.depth_m, *options, &plan->post);
+    if (st == FEMM_OK)
+      st = femm::UploadPostPlan(plan->post, plan->dev, &plan->post_dev, &plan->post_base);
+  }
   if (st != FEMM_OK)
     return st;
   plan->post_ready = true;

@@ -80,6 +80,32 @@ __global__ __launch_bounds__(kBlockThreads) void PcgKernel(DevicePlan P, DeviceW
   }
 }
 
+// One PCG (as PcgKernel, item 0) split over `partitions` co-resident workgroups of a
+// cooperative launch, for one-off solves on large meshes (the weighted-stress mask).
+__global__ __launch_bounds__(kBlockThreads) void TeamPcgKernel(DevicePlan P, DeviceWork W,
+                                                               double tol, int maxit,
+                                                               double *info, int partitions)
+{
+  __shared__ double lds[2 * kWaves * 4];
+  const int g = UniformInt(static_cast<int>(blockIdx.x));
+  GridTeam R{BlockReducer{lds, 0}, Partition(P, partitions, g), W.team_partials, W.team_barrier,
+             W.team_abort, partitions, g, 0};
+  const ItemPointers w = ItemWork(P, W, 0, false);
+  for (int row = R.S.row0 + threadIdx.x; row < R.S.row1; row += kBlockThreads) {
+    const double d = w.vals[P.diag_sell[row]];
+    w.inv[row] = (isfinite(d) && d > 0.0) ? 1.0 / d : 0.0;
+  }
+  R.Sync();
+  const BlockPrec<double> B{nullptr, 1, P.n};
+  const LinearInfo li = Pcg64<true, 0>(P, w.vals, w.inv, B, w.rhs, w.x, w.r, w.p, w.ap, tol,
+                                       maxit, false, R, w.p_alt, w.q_buf);
+  if (g == 0 && threadIdx.x == 0) {
+    info[0] = li.status;
+    info[1] = li.iterations;
+    info[2] = li.residual;
+  }
+}
+
 // Element B (centroid basis) and flux per item (nonlinear_solver.cu:16-83).
 __global__ void ElementBKernel(DevicePlan P, const double *states, int items, double *bx,
                                double *by)
@@ -861,6 +887,28 @@ int LaunchPcg(const DeviceBuffers &dev, int32_t items, double tol, int32_t maxit
   hipLaunchKernelGGL(PcgKernel, dim3(items), dim3(kBlockThreads), 0, stream, dev.plan, dev.work,
                      tol, maxit, info);
   return hipGetLastError() == hipSuccess ? FEMM_OK : FEMM_INTERNAL_ERROR;
+}
+
+int LaunchTeamPcg(const DeviceBuffers &dev, double tol, int32_t maxit, double *info,
+                  hipStream_t stream)
+{
+  if (dev.work.capacity < 1)
+    return FEMM_INVALID_ARGUMENT;
+  // About 32 slices (2,048 rows) per workgroup, as the untuned team split.
+  const void *kernel = reinterpret_cast<const void *>(&TeamPcgKernel);
+  const int parts = std::max(1, std::min({dev.plan.nslices / 32, kMaxPartitions,
+                                          CoResidentBlocks(kernel)}));
+  ClearTeamKernel<<<1, 256, 0, stream>>>(dev.work.team_barrier, dev.work.team_abort, 1);
+  if (hipGetLastError() != hipSuccess)
+    return FEMM_INTERNAL_ERROR;
+  DevicePlan plan = dev.plan;
+  DeviceWork work = dev.work;
+  int partitions = parts;
+  void *params[] = {&plan, &work, &tol, &maxit, &info, &partitions};
+  return hipLaunchCooperativeKernel(kernel, dim3(static_cast<unsigned>(parts)),
+                                    dim3(kBlockThreads), params, 0, stream) == hipSuccess
+             ? FEMM_OK
+             : FEMM_INTERNAL_ERROR;
 }
 
 int LaunchFinalize(const DeviceBuffers &dev, int32_t items, const double *states, double *bx,

@@ -210,6 +210,9 @@ int LaunchSpmv(const DeviceBuffers &dev, int32_t items, bool fp32, int32_t reps,
 // PCG only on work.vals / work.rhs (x0 = 0); info[items][3].
 int LaunchPcg(const DeviceBuffers &dev, int32_t items, double tol, int32_t maxit, double *info,
               hipStream_t stream, int32_t block_size = 1);
+// The same PCG for item 0 over several co-resident workgroups (cooperative launch); info[3].
+int LaunchTeamPcg(const DeviceBuffers &dev, double tol, int32_t maxit, double *info,
+                  hipStream_t stream);
 // Element B and circuit flux for states[items][N].
 int LaunchFinalize(const DeviceBuffers &dev, int32_t items, const double *states, double *bx,
                    double *by, double *flux, hipStream_t stream);

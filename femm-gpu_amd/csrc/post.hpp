@@ -1,5 +1,5 @@
 // Weighted-stress force/torque and air-gap radial B: geometry-only plan
-// (mask, mask gradient, probe location) and its device copy.
+// (mask, mask gradient, probe location) on the device.
 #pragma once
 
 #include <cstdint>
@@ -49,11 +49,14 @@ struct DevicePostPlan {
   const double *probe_lambda, *probe_cos, *probe_sin, *node_xy, *tri_gx, *tri_gy;
 };
 
-// Builds the plan (PrepareWeightedStressMask + SolveWeightedStressMask +
-// BuildFrozenPostprocessPlanWithMask of the reference, weighted_stress.cu:24-183 and
-// field_postprocess.cpp:221-339).  The mask Laplace solve runs on the GPU.
-int BuildPostPlan(const femm_model_t *model, const HostPlan &plan,
-                  const femm_postprocess_options_t &options, HostPostPlan *out);
+// Builds the weighted-stress plan on the GPU (PrepareWeightedStressMask +
+// SolveWeightedStressMask + BuildFrozenPostprocessPlanWithMask of the reference,
+// weighted_stress.cu:24-183 and field_postprocess.cpp:221-339) from the solve plan's device
+// arrays: `out` points into one allocation returned in `*base`; `host` receives the status,
+// the node mask and the mask solve's iterations.  On failure nothing stays allocated.
+int BuildPostPlanDevice(const femm_model_t *model, const HostPlan &plan, const DeviceBuffers &dev,
+                        const femm_postprocess_options_t &options, HostPostPlan *host,
+                        DevicePostPlan *out, void **base);
 
 int UploadPostPlan(const HostPostPlan &host, const DeviceBuffers &dev, DevicePostPlan *out,
                    void **base);

@@ -330,8 +330,9 @@ typedef struct femm_postprocess_options {
   double mask_relative_tolerance; /* mask PCG tolerance (reference 1e-12) */
 } femm_postprocess_options_t;
 
-/* Geometry-only postprocess plan: weighted-stress mask (Laplace solve on the GPU,
- * thresholded at 0.5), mask gradients and probe location.  Replaces
+/* Geometry-only postprocess plan, built on the GPU from the plan's device arrays:
+ * weighted-stress mask (element rules, Laplace solve, thresholded at 0.5), mask gradients,
+ * node incidence and probe location.  Replaces
  * PrepareWeightedStressMask / SolveWeightedStressMask (postprocess/weighted_stress.cu:24-183)
  * and BuildFrozenPostprocessPlanWithMask (postprocess/field_postprocess.cpp:221-339).
  * A plan whose model has air-gap elements gets their postprocess instead, as the reference's
