@@ -538,11 +538,12 @@ bool ParseArtifact(const std::string &text, Artifact *out, std::string *error)
     return false;
   *out = std::move(a);
   out->Repoint();
-  // ValidateNonlinearModel of the mapped model (mesh_artifact.cpp:706-709).
+  // ValidateNonlinearModel of the mapped model (mesh_artifact.cpp:706-709): every model
+  // check of the plan build, without its symbolic CSR.
   const femm_model_t view = out->View();
   HostPlan check;
   std::string why;
-  if (BuildHostPlan(&view, &check, &why) != FEMM_OK) {
+  if (BuildHostPlan(&view, &check, &why, nullptr, true) != FEMM_OK) {
     *out = Artifact{};
     *error = "mapped mesh model validation failed";
     return false;

@@ -2,8 +2,11 @@
 
 #include "json.hpp"
 
+#include <charconv>
 #include <cmath>
+#include <cpuid.h>
 #include <cstdlib>
+#include <immintrin.h>
 #include <cstring>
 #include <limits>
 
@@ -157,10 +160,19 @@ private:
     }
     out->type = Json::Type::kNumber;
     out->text.assign(s_, begin, at_ - begin);
-    char *end = nullptr;
-    out->number = std::strtod(out->text.c_str(), &end);
-    if (end != out->text.c_str() + out->text.size())
+    // Correctly rounded, as strtod, without the locale; out of range (overflow, or an
+    // underflow that strtod takes to a subnormal or zero) goes to strtod for its result.
+    const auto r = std::from_chars(s_.data() + begin, s_.data() + at_, out->number);
+    if (r.ptr != s_.data() + at_)
       return Fail(error, "JSON number conversion failed");
+    if (r.ec == std::errc::result_out_of_range) {
+      char *end = nullptr;
+      out->number = std::strtod(out->text.c_str(), &end);
+      if (end != out->text.c_str() + out->text.size())
+        return Fail(error, "JSON number conversion failed");
+    } else if (r.ec != std::errc()) {
+      return Fail(error, "JSON number conversion failed");
+    }
     if (!std::isfinite(out->number))
       return Fail(error, "nonfinite JSON number");
     return true;
@@ -272,6 +284,80 @@ void Compress(uint32_t h[8], const unsigned char *block)
   }
   for (int k = 0; k < 8; ++k)
     h[k] += v[k];
+}
+
+// The same compression with the x86 SHA extensions (the round and message-schedule
+// instructions of FIPS 180-4's SHA-256; state kept as the ABEF / CDGH register pairs those
+// instructions take).  Four rounds per step: W + K for rounds t..t+3, two rounds from the low
+// half, two from the high half.
+__attribute__((target("sha,sse4.1,ssse3"))) void CompressShaNi(uint32_t h[8],
+                                                              const unsigned char *p,
+                                                              size_t blocks)
+{
+  const __m128i order = _mm_set_epi64x(0x0c0d0e0f08090a0bLL, 0x0405060700010203LL);
+  const __m128i dcba = _mm_loadu_si128(reinterpret_cast<const __m128i *>(h));
+  const __m128i hgfe = _mm_loadu_si128(reinterpret_cast<const __m128i *>(h + 4));
+  const __m128i badc = _mm_shuffle_epi32(dcba, 0xB1);
+  const __m128i efgh = _mm_shuffle_epi32(hgfe, 0x1B);
+  __m128i abef = _mm_alignr_epi8(badc, efgh, 8);
+  __m128i cdgh = _mm_blend_epi16(efgh, badc, 0xF0);
+  for (size_t b = 0; b < blocks; ++b, p += 64) {
+    const __m128i abef0 = abef, cdgh0 = cdgh;
+    __m128i w[4];
+    for (int g = 0; g < 16; ++g) {
+      __m128i m;
+      if (g < 4) {
+        m = _mm_shuffle_epi8(_mm_loadu_si128(reinterpret_cast<const __m128i *>(p + 16 * g)),
+                             order);
+      } else {
+        // W[t] = s1(W[t-2]) + W[t-7] + s0(W[t-15]) + W[t-16] for t = 4g .. 4g+3.
+        const __m128i x = _mm_add_epi32(_mm_sha256msg1_epu32(w[g & 3], w[(g + 1) & 3]),
+                                        _mm_alignr_epi8(w[(g + 3) & 3], w[(g + 2) & 3], 4));
+        m = _mm_sha256msg2_epu32(x, w[(g + 3) & 3]);
+      }
+      w[g & 3] = m;
+      const __m128i wk =
+          _mm_add_epi32(m, _mm_loadu_si128(reinterpret_cast<const __m128i *>(kRound + 4 * g)));
+      // Two rounds give the new ABEF; the old ABEF is the new CDGH.  After the second pair
+      // both variables hold their own roles again.
+      cdgh = _mm_sha256rnds2_epu32(cdgh, abef, wk);
+      abef = _mm_sha256rnds2_epu32(abef, cdgh, _mm_shuffle_epi32(wk, 0x0E));
+    }
+    abef = _mm_add_epi32(abef, abef0);
+    cdgh = _mm_add_epi32(cdgh, cdgh0);
+  }
+  const __m128i feba = _mm_shuffle_epi32(abef, 0x1B);
+  const __m128i ghcd = _mm_shuffle_epi32(cdgh, 0xB1);
+  _mm_storeu_si128(reinterpret_cast<__m128i *>(h), _mm_blend_epi16(feba, ghcd, 0xF0));
+  _mm_storeu_si128(reinterpret_cast<__m128i *>(h + 4), _mm_alignr_epi8(ghcd, feba, 8));
+}
+
+// SHA extensions present (CPUID leaf 7 EBX bit 29, SSSE3 and SSE4.1), unless
+// FEMM_GPU_AMD_PORTABLE_SHA is set to a value other than 0.
+bool UseShaNi()
+{
+  static const bool use = [] {
+    const char *force = std::getenv("FEMM_GPU_AMD_PORTABLE_SHA");
+    if (force != nullptr && *force != '\0' && std::strcmp(force, "0") != 0)
+      return false;
+    unsigned a = 0, b = 0, c = 0, d = 0;
+    if (!__get_cpuid(1, &a, &b, &c, &d) || !(c & (1u << 9)) || !(c & (1u << 19)))
+      return false;
+    if (!__get_cpuid_count(7, 0, &a, &b, &c, &d))
+      return false;
+    return (b & (1u << 29)) != 0;
+  }();
+  return use;
+}
+
+void CompressBlocks(uint32_t h[8], const unsigned char *p, size_t blocks)
+{
+  if (UseShaNi()) {
+    CompressShaNi(h, p, blocks);
+    return;
+  }
+  for (size_t b = 0; b < blocks; ++b)
+    Compress(h, p + 64 * b);
 }
 
 void AppendCanonical(const Json &v, std::string *out)
@@ -400,9 +486,8 @@ std::string Sha256Hex(const std::string &bytes)
                    0x510e527fu, 0x9b05688cu, 0x1f83d9abu, 0x5be0cd19u};
   const size_t n = bytes.size();
   const unsigned char *p = reinterpret_cast<const unsigned char *>(bytes.data());
-  size_t full = n / 64;
-  for (size_t b = 0; b < full; ++b)
-    Compress(h, p + 64 * b);
+  const size_t full = n / 64;
+  CompressBlocks(h, p, full);
   // Padding: the remaining bytes, 0x80, zeros, the bit length (big-endian) in 64 bits.
   unsigned char tail[128] = {};
   const size_t rest = n - 64 * full;
@@ -412,8 +497,7 @@ std::string Sha256Hex(const std::string &bytes)
   const uint64_t bits = static_cast<uint64_t>(n) * 8u;
   for (int k = 0; k < 8; ++k)
     tail[64 * blocks - 1 - k] = static_cast<unsigned char>(bits >> (8 * k));
-  for (size_t b = 0; b < blocks; ++b)
-    Compress(h, tail + 64 * b);
+  CompressBlocks(h, tail, blocks);
   static const char *digits = "0123456789abcdef";
   std::string hex(64, '0');
   for (int k = 0; k < 8; ++k)

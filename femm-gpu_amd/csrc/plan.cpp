@@ -301,7 +301,8 @@ static int BuildAirGapTerms(const femm_model_t *m, HostPlan *p)
   return FEMM_OK;
 }
 
-int BuildHostPlan(const femm_model_t *m, HostPlan *p, std::string *error, const HostPlan *topology)
+int BuildHostPlan(const femm_model_t *m, HostPlan *p, std::string *error, const HostPlan *topology,
+                  bool validate_only)
 {
   auto fail = [error](int code, const char *why) {
     if (error)
@@ -353,7 +354,8 @@ int BuildHostPlan(const femm_model_t *m, HostPlan *p, std::string *error, const 
     if (mt.bh_curve < -1 || mt.bh_curve >= m->curve_count)
       return fail(FEMM_INVALID_MATERIAL, "material curve");
   }
-  std::vector<ElementBasis> basis(T);
+  std::vector<ElementBasis> basis(validate_only ? 0 : T);   // a check keeps no geometry
+  ElementBasis scratch_basis;
   for (int32_t e = 0; e < T; ++e) {
     const int32_t mat = m->triangle_material[e];
     if (mat < 0 || mat >= m->material_count)
@@ -370,7 +372,7 @@ int BuildHostPlan(const femm_model_t *m, HostPlan *p, std::string *error, const 
     const double det = (x[1] - x[0]) * (y[2] - y[0]) - (x[2] - x[0]) * (y[1] - y[0]);
     if (!(det > 0.0) || !std::isfinite(det))
       return fail(FEMM_MESH_INVALID, "triangle orientation");
-    if (!Basis(x, y, true, &basis[e]))
+    if (!Basis(x, y, true, validate_only ? &scratch_basis : &basis[e]))
       return fail(FEMM_MESH_INVALID, "triangle basis");
   }
   const int age_status = ValidateAirGaps(m);
@@ -443,6 +445,19 @@ int BuildHostPlan(const femm_model_t *m, HostPlan *p, std::string *error, const 
   p->T = T;
   p->n = n;
   p->C = C;
+  if (validate_only) {
+    // The one check of the geometry build below: the r-z element basis.
+    for (int32_t e = 0; !planar && e < T; ++e) {
+      double x[3], y[3];
+      for (int k = 0; k < 3; ++k) {
+        x[k] = m->node_xy[2 * m->triangle_nodes[3 * e + k]];
+        y[k] = m->node_xy[2 * m->triangle_nodes[3 * e + k] + 1];
+      }
+      if (!Basis(x, y, false, &scratch_basis))
+        return fail(FEMM_MESH_INVALID, "r-z basis");
+    }
+    return FEMM_OK;
+  }
   p->planar = planar ? 1 : 0;
   p->Q = planar ? 1 : 7;
   p->depth = planar ? m->depth_m : 1.0;

@@ -71,9 +71,10 @@ struct HostPlan {
 // model for which SameTopology holds), only the material-dependent parts are built
 // (materials, splines, per-triangle material / circuit / flux weights, PM and circuit
 // right-hand sides); the DOF map is taken from `topology` and the symbolic CSR and SELL
-// layout are left empty.
+// layout are left empty.  With `validate_only` every model check runs (the reference's
+// ValidateNonlinearModel and the DOF map's) and the plan stops before the symbolic CSR.
 int BuildHostPlan(const femm_model_t *model, HostPlan *plan, std::string *error,
-                  const HostPlan *topology = nullptr);
+                  const HostPlan *topology = nullptr, bool validate_only = false);
 
 // True when `variant` shares `base`'s mesh, Dirichlet nodes and values, node constraints,
 // problem type, depth, circuit count, material count and B-H knot layout, so that only

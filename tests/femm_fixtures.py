@@ -11,6 +11,7 @@ from pathlib import Path
 import numpy as np
 
 from femm_gpu_amd.model import MU0, Material, Model, load_artifact
+from femm_gpu_amd.wire import write_artifact  # noqa: F401  (the fixtures' artifact writer)
 
 GOLDEN = Path(__file__).resolve().parent / "golden" / "femm"
 
@@ -184,81 +185,3 @@ def postprocess_ring():
                  [-1.0, 1.0, 1.0, -1.0], None, 1, 1.0, name="postprocess_ring")
 
 
-def write_artifact(path, model, currents, schema="gpu_femm_planar_dc_mesh_v1", turns=None,
-                   source_sha="a" * 64, base_sha="b" * 64, rotor_angle_deg=0.0,
-                   displacement_mm=(0.0, 0.0), age_scalar=False, edit=None):
-    """Write `model` as a strictly valid mesh artifact (the layout the reference's preparer
-    writes, io/mesh_artifact.cpp:14-440): one material and one region per material label
-    (group number = label, FEMM's region id = label), integer ids, named series circuits,
-    node constraints for the magnetostatic schema, identities for the motor schema and, for the
-    neutral schemas, canonical_identity_sha256 = SHA-256 of the canonical `resolved` (sorted
-    keys, no whitespace, numbers as written).  `turns[label]` sets a coil region's turns
-    (default: its J per ampere x its area).  The sliding-band schema gpu_femm_mesh_v2 also
-    carries the model's air-gap elements (mesh_artifact.cpp:610-697; `age_scalar` writes a
-    single element as an object, the spelling MATLAB's jsonencode produces).  `edit(resolved)`
-    may change the document before it is written.  Returns the SHA-256 of the bytes written."""
-    import hashlib
-    import json
-
-    areas = model.element_areas()
-    mats, regions = [], []
-    for i, m in enumerate(model.materials):
-        b, h = (model.curves[m.bh_curve] if m.bh_curve >= 0 else ([], []))
-        mu = 1.0 / (MU0 * m.reluctivity_zero)
-        mats.append(dict(id=i, name=f"material_{i}", mu_x=mu, mu_y=mu, H_c_A_per_m=float(m.hc),
-                         B_T=[float(x) for x in b], H_A_per_m=[float(x) for x in h],
-                         lam_type=0, lam_fill=1))
-        if m.circuit >= 0:
-            t = turns[i] if turns is not None else m.source_j_per_a * float(
-                areas[model.triangle_material == i].sum())
-        else:
-            t = 0.0
-        regions.append(dict(id=i, group_number=i, material_id=i, circuit_index=int(m.circuit),
-                            turns=float(t), pm_magnetization_deg=float(m.magnetization_deg)))
-    motor = schema in ("gpu_femm_mesh_v1", "gpu_femm_mesh_v2")
-    resolved = dict(
-        source_fem_sha256=source_sha,
-        model=dict(depth_mm=model.depth_m * 1e3,
-                   problem_type="axisymmetric" if model.problem_type == 1 else "planar",
-                   frequency_hz=0),
-        nodes_mm=(model.node_xy * 1e3).tolist(),
-        triangles=dict(node_indices=model.triangles.tolist(),
-                       region_ids=[int(k) for k in model.triangle_material]),
-        regions=regions, materials=mats,
-        circuits=[dict(index=k, name=f"circuit_{k}", type="series", current_A=float(c))
-                  for k, c in enumerate(currents)],
-        outer_dirichlet=dict(node_indices=model.dirichlet_nodes.tolist(),
-                             A_Wb_per_m=model.dirichlet_values.tolist()))
-    if schema == "gpu_femm_magnetostatic_mesh_v1":
-        resolved["node_constraints"] = [
-            dict(node_a=int(a), node_b=int(b), relation="periodic" if s > 0 else "antiperiodic")
-            for a, b, s in model.constraints]
-    if motor:
-        resolved["base_motor_fem_sha256"] = base_sha
-        resolved["pose"] = dict(rotor_angle_deg=rotor_angle_deg,
-                                displacement_mm=list(displacement_mm))
-    if schema == "gpu_femm_mesh_v2":
-        ages = []
-        for g, age in enumerate(model.air_gaps):
-            points = [dict(n0=int(n[0]), w0=float(w[0]), n1=int(n[1]), w1=float(w[1]),
-                           n2=int(n[2]), w2=float(w[2]), n3=int(n[3]), w3=float(w[3]))
-                      for n, w in zip(age.nodes, age.weights)]
-            ages.append(dict(name=f"gap_{g}",
-                             periodicity="antiperiodic" if age.antiperiodic else "periodic",
-                             center_mm=[1e3 * age.center_xy[0], 1e3 * age.center_xy[1]],
-                             ri_mm=1e3 * age.inner_radius_m, ro_mm=1e3 * age.outer_radius_m,
-                             arc_length_deg=age.arc_length_deg, sector_count=age.cells,
-                             inner_shift=age.inner_shift, outer_shift=age.outer_shift,
-                             quad_points=points))
-        resolved["air_gap_elements"] = ages[0] if age_scalar and len(ages) == 1 else ages
-    if edit is not None:
-        edit(resolved)
-    canonical = json.dumps(resolved, sort_keys=True, separators=(",", ":"))
-    doc = dict(schema_version=schema, source_fem_sha256=source_sha,
-               canonical_identity_sha256=hashlib.sha256(canonical.encode()).hexdigest(),
-               resolved=resolved)
-    if motor:
-        doc["base_motor_fem_sha256"] = base_sha
-    data = json.dumps(doc).encode()
-    Path(path).write_bytes(data)
-    return hashlib.sha256(data).hexdigest()

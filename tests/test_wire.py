@@ -58,6 +58,33 @@ def test_sha256_block_boundaries_and_a_million_a():
         "cdc76e5c9914fb9281a1c7e284d73e67f1809a48a497200e046d39ccc7112cd0"
 
 
+_SHA_CHECK = '''
+import hashlib, random, sys
+sys.path.insert(0, sys.argv[1])
+from femm_gpu_amd import wire
+rng = random.Random(5)
+for n in list(range(0, 260)) + [4095, 4096, 4097, 100003]:
+    data = bytes(rng.randrange(256) for _ in range(n))
+    assert wire.sha256_hex(data) == hashlib.sha256(data).hexdigest(), n
+print("ok")
+'''
+
+
+@pytest.mark.parametrize("portable", ["0", "1"])
+def test_sha256_both_compressions_match_hashlib(portable):
+    """Messages of 0-259 bytes and around block multiples hash as hashlib does, with the x86
+    SHA-extension compression (where the CPU has it) and with the portable one
+    (FEMM_GPU_AMD_PORTABLE_SHA=1); each in a fresh process, since the choice is made once."""
+    import os
+    import subprocess
+    import sys
+    env = dict(os.environ, FEMM_GPU_AMD_PORTABLE_SHA=portable)
+    pkg = str(Path(wire.__file__).resolve().parents[1])
+    out = subprocess.run([sys.executable, "-c", _SHA_CHECK, pkg], env=env, capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0 and out.stdout.strip() == "ok", out.stderr
+
+
 # ---- artifacts -----------------------------------------------------------------------------
 
 @pytest.mark.parametrize("path", sorted(ff.GOLDEN.glob("*/*.gpu.json")), ids=lambda p: p.stem)
