@@ -244,14 +244,20 @@ struct GridTeam {
         mine[k] = v[k];
     }
     Sync();
+    static_assert(kMaxPartitions <= kBlockThreads, "one partial per thread");
+    // Every member sums the G partials in the same fixed order: thread h loads partial h
+    // (all loads issued together, one memory round trip instead of G dependent loads) and
+    // the block reduction combines them in its fixed order.
     const double *all = partials + static_cast<size_t>(buf) * G * 4;
+    const int h = threadIdx.x;
+    double x[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      double s = 0.0;
-      for (int h = 0; h < G; ++h)
-        s += all[h * 4 + k];
-      v[k] = Uniform(s);
-    }
+    for (int k = 0; k < K; ++k)
+      x[k] = h < G ? all[h * 4 + k] : 0.0;
+    R.Sum(x);
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      v[k] = x[k];
     buf ^= 1;
   }
 
@@ -264,12 +270,11 @@ struct GridTeam {
     if (threadIdx.x == 0)
       mine[0] = v;
     Sync();
-    const double *all = partials + static_cast<size_t>(buf) * G * 4;
-    double m = all[0];
-    for (int h = 1; h < G; ++h)
-      m = fmax(m, all[h * 4]);
+    const double *all = partials + static_cast<size_t>(buf) * G * 4;   // as in Sum
+    const int h = threadIdx.x;
+    const double m = R.Max(h < G ? all[h * 4] : all[0]);
     buf ^= 1;
-    return Uniform(m);
+    return m;
   }
 
   __device__ __attribute__((always_inline)) int AnyTrue(bool local)
