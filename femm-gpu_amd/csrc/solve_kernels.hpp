@@ -153,8 +153,8 @@ struct BlockTeam {
 // word, which the others watch before they acquire.  No read-modify-write is contended and
 // each workgroup fences twice per barrier.  All workgroups of a launch are co-resident
 // (cooperative launch); a wait that still exceeds kMaxBarrierPolls (seconds) marks the
-// team aborted, and every later barrier of the team returns at once, so no workgroup can
-// wait forever.
+// team aborted, and every later barrier of the team returns at its first poll, so no
+// workgroup can wait forever.
 constexpr long kMaxBarrierPolls = 1L << 27;
 
 // Wave 0's part of team barrier number `target` for workgroup g of G; one out-of-line copy
@@ -162,25 +162,32 @@ constexpr long kMaxBarrierPolls = 1L << 27;
 __device__ __noinline__ void TeamBarrier(uint32_t *bar, int32_t *abort, int G, int g,
                                         uint32_t target)
 {
-  if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
-    return;
+  static_assert(kMaxPartitions <= 4 * 64, "four arrival words per lane");
   const int lane = static_cast<int>(threadIdx.x);
   // The workgroup's writes (completed to the L2 by the block barrier before), then arrival.
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   if (lane == 0)
     __hip_atomic_store(&bar[g], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // Each poll issues all of its loads (the watched words and the abort flag) before it looks
+  // at any of them: one memory round trip per poll.  An aborted team stops at the first poll.
   bool aborted = false;
   if (g == 0) {
     for (long polls = 0;; ++polls) {
+      uint32_t w[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int h = lane + 64 * j;
+        w[j] = h < G ? __hip_atomic_load(&bar[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                     : target;
+      }
+      const int32_t stop = __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       bool here = true;
-      for (int h = lane; h < G; h += 64)
-        here = here && static_cast<int32_t>(__hip_atomic_load(&bar[h], __ATOMIC_RELAXED,
-                                                               __HIP_MEMORY_SCOPE_AGENT) -
-                                            target) >= 0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        here &= static_cast<int32_t>(w[j] - target) >= 0;
       if (__all(here))
         break;
-      if (polls > kMaxBarrierPolls ||
-          __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+      if (stop != 0 || polls > kMaxBarrierPolls) {
         aborted = true;
         break;
       }
@@ -192,13 +199,13 @@ __device__ __noinline__ void TeamBarrier(uint32_t *bar, int32_t *abort, int G, i
       __hip_atomic_store(&bar[kBarrierRelease], target, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
   } else {
-    for (long polls = 0;
-         static_cast<int32_t>(__hip_atomic_load(&bar[kBarrierRelease], __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT) -
-                              target) < 0;
-         ++polls) {
-      if (polls > kMaxBarrierPolls ||
-          __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+    for (long polls = 0;; ++polls) {
+      const uint32_t r =
+          __hip_atomic_load(&bar[kBarrierRelease], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const int32_t stop = __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (static_cast<int32_t>(r - target) >= 0)
+        break;
+      if (stop != 0 || polls > kMaxBarrierPolls) {
         aborted = true;
         break;
       }
