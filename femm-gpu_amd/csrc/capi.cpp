@@ -100,9 +100,11 @@ int EnsurePlanDevice(femm_plan *plan)
   return st;
 }
 
-// Probe solve of the autotuners: two Newton iterations of 200 PCG iterations each (a
-// tolerance no solve reaches) of `items` items at unit circuit currents with `partitions`
-// workgroups per item; *ms is the second of two timed launches.
+// Probe solve of the autotuners: at most two Newton iterations of at most 200 PCG iterations
+// (a tolerance no solve reaches, so the first linear solve ends at its cap and, as the
+// reference driver does with a linear solve that fails, ends the item) of `items` items at
+// unit circuit currents with `partitions` workgroups per item: one assembly and 200 PCG
+// iterations; *ms is the second of two timed launches.
 int ProbeSolve(femm_plan *plan, int32_t items, const femm_solve_config_t &config,
                int32_t partitions, float *ms)
 {
